@@ -1,0 +1,170 @@
+/*
+ * pgbsc.h -- C ABI of the MI355X color-coding subgraph-counting library
+ * (libpgbsc.so).
+ *
+ * The reference (dsc-nec/subgraph) has no FFI: its hot path sits behind the
+ * C++ classes CountMat, CSCGraph, DataTableColMajor, IndexSys and
+ * DivideTemplates.  Each entry point below names the reference interface it
+ * replaces.  Conventions:
+ *   - every function returns PG_OK (0) or a negative PG_ERR_* code; the text of
+ *     the last error is available from pg_last_error(ctx); nothing throws;
+ *   - host arrays are owned by the caller and only read (or written, for
+ *     outputs) during the call; device memory is owned by the context or plan;
+ *   - one context per host thread; every call is stream-ordered on the
+ *     context's stream and synchronous at return;
+ *   - vertex ids and CSC entry indices are the caller's (the reference's)
+ *     numbering; the library's internal relabelling never leaks out.
+ */
+#ifndef PGBSC_H
+#define PGBSC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PGBSC_ABI_VERSION 1
+
+#define PG_OK 0
+#define PG_ERR_INVALID (-1) /* bad argument or malformed input */
+#define PG_ERR_NOMEM (-2)   /* device or host allocation failed */
+#define PG_ERR_HIP (-3)     /* a HIP runtime call failed */
+#define PG_ERR_STATE (-4)   /* call out of order (e.g. no graph uploaded) */
+#define PG_ERR_IO (-5)      /* file could not be opened or is truncated */
+#define PG_ERR_RANGE (-6)   /* size beyond what the library supports */
+
+/* precision of the count tables */
+#define PG_FP32 0 /* reference numerics: FP32 tables, float(1e-12) scaling, FP64 root */
+#define PG_FP64 1 /* FP64 tables, no scaling: exact integer counts below 2^53 */
+
+/* flags */
+#define PG_FLAG_FIX_2VERTEX 1u /* do not multiply two-vertex templates by 1e12
+                                  (the reference does, CountMat.cpp:811-824) */
+
+typedef struct pg_ctx pg_ctx;
+typedef struct pg_plan pg_plan;
+
+/* ---- context (replaces CountMat::initialization, CountMat.cpp:21-77) ---- */
+int pg_create(pg_ctx** out, int device);
+void pg_destroy(pg_ctx* ctx);
+const char* pg_last_error(const pg_ctx* ctx);
+/* Tuning knobs (no reference counterpart; the reference hard-codes
+ * _bufMatCols = 16, CountMat.cpp:62):
+ *   "hub_degree"   vertices above this degree are split across teams (default 2048)
+ *   "hub_chunk"    neighbours per hub work item (default 1024)
+ *   "batch_tile"   colorings per pass, 0 = automatic (default 0)
+ *   "mem_budget_mb" device bytes the tables may use, 0 = 90% of free memory
+ *   "autotune"     1 = pick hub_degree/hub_chunk/batch_tile by timing (default 0) */
+int pg_set_option(pg_ctx* ctx, const char* key, int64_t value);
+int pg_get_option(const pg_ctx* ctx, const char* key, int64_t* value);
+
+/* ---- graph (replaces CSCGraph::createFromEdgeListFile + splitCSC,
+ *      CSCGraph.hpp:96-217, and the binary deserialize, CSCGraph.hpp:439-463) ----
+ * colptr[n+1], rowidx[nnz]: the reference's CSC (column u lists the rows v
+ * adjacent to u).  vals[nnz] (NULL = all 1.0) weights the SpMM
+ * Y[v] += A[v][u] * T[u], A[v][u] being the entry (row v, column u); the
+ * reference stores 1.0 and ignores the values (CSCGraph.hpp:153, 283). */
+int pg_graph_upload(pg_ctx* ctx, int32_t n, int64_t nnz, const int32_t* colptr,
+                    const int32_t* rowidx, const float* vals);
+/* Replace the entry weights: host array (on_device = 0) or a device pointer
+ * on the context's device (on_device = 1); NULL restores unit weights. */
+int pg_graph_set_weights(pg_ctx* ctx, const float* vals, int on_device);
+int pg_graph_stats(const pg_ctx* ctx, int64_t* n, int64_t* nnz, int64_t* n_active,
+                   int64_t* max_degree);
+
+/* ---- plan (replaces DivideTemplates::DivideTp/sort_tps,
+ *      DivideTemplates.cpp:6-327, and IndexSys::initialization,
+ *      IndexSys.cpp:9-57, as consumed by CountMat::compute) ----
+ * sub_size[nsub]; main_idx/aux_idx[nsub] (-1 for leaves);
+ * comb_to_count: for s = 0..nsub-1 in order, C(k, sub_size[s]) ints
+ *   (IndexSys::getCombToCountTable);
+ * split_main/split_aux: for every non-leaf s in order, C(k,size) rows of
+ *   C(size, main size) ints, row i = i-th lexicographic color set
+ *   (IndexSys::getSplitToCountTable()[0/1][s]). */
+int pg_plan_create(pg_ctx* ctx, int k, int nsub, const int32_t* sub_size,
+                   const int32_t* main_idx, const int32_t* aux_idx,
+                   const int32_t* comb_to_count, const int32_t* split_main,
+                   const int32_t* split_aux, pg_plan** out);
+/* Convenience: run the library's DivideTemplates + IndexSys on a template
+ * tree given as an edge list over vertices 0..nv-1 (Graph::build_graph order,
+ * Graph.cpp:82-129), then pg_plan_create. */
+int pg_plan_from_edges(pg_ctx* ctx, int nv, int ne, const int32_t* src, const int32_t* dst,
+                       pg_plan** out);
+void pg_plan_destroy(pg_plan* plan);
+int pg_plan_info(const pg_plan* plan, int* k, int* nsub, int* n_nonleaf, int* scaled_sub);
+/* Host only (no device needed): the DivideTemplates + IndexSys tables of a
+ * template tree, flattened exactly as pg_plan_create takes them.  First
+ * pg_template_sizes for the lengths (nsub, total comb_to_count ints, total
+ * split ints per array), then pg_template_tables to fill caller arrays. */
+int pg_template_sizes(int nv, int ne, const int32_t* src, const int32_t* dst, int32_t* nsub,
+                      int64_t* n_comb, int64_t* n_split);
+int pg_template_tables(int nv, int ne, const int32_t* src, const int32_t* dst, int32_t* sub_size,
+                       int32_t* main_idx, int32_t* aux_idx, int32_t* comb_to_count,
+                       int32_t* split_main, int32_t* split_aux);
+
+/* ---- colorings (replaces CountMat::colorInit, CountMat.cpp:979-991) ----
+ * Color of vertex v (caller numbering) in coloring c of stream `seed`:
+ *   h = splitmix64(seed ^ splitmix64(c * 0x9E3779B97F4A7C15 + v));
+ *   color = ((h >> 32) * k) >> 32.
+ * The device generates the same colors when pg_count gets colors == NULL. */
+void pg_colors_fill(uint64_t seed, int64_t first_coloring, int ncolorings, int32_t n, int k,
+                    uint8_t* out /* [ncolorings][n] */);
+
+/* ---- counting (replaces the loop of CountMat::colorCounting,
+ *      CountMat.cpp:203-205 and 268-322, whose per-subtemplate body is
+ *      countNonBottomePrunedSPMM, CountMat.cpp:556-849) ----
+ * colors: [ncolorings][n] uint8 in caller numbering, or NULL to draw them on
+ * the device from (seed, coloring index).  raw_out[ncolorings] receives each
+ * coloring's raw colorful count exactly as colorCounting returns it (FP32:
+ * the reference's 1e-12 / 1e12 round trip; FP64: exact). */
+int pg_count(pg_ctx* ctx, const pg_plan* plan, int ncolorings, const uint8_t* colors,
+             uint64_t seed, int precision, uint32_t flags, double* raw_out);
+
+/* Mixed batch: plans[p] is counted for ncolorings[p] colorings drawn on the
+ * device from seeds[p]; raw_out is the concatenation of the per-plan results.
+ * No reference counterpart (the reference counts one template per run). */
+int pg_count_multi(pg_ctx* ctx, int nplans, const pg_plan* const* plans,
+                   const int32_t* ncolorings, const uint64_t* seeds, int precision,
+                   uint32_t flags, double* raw_out);
+
+/* ---- differentiation (no reference counterpart: the reference ignores edge
+ *      values, CSCGraph.hpp:283) ----
+ * Forward as pg_count with the current weights, then the adjoint DP:
+ * grad[e] = sum over the colorings of d raw / d A-entry e (caller CSC entry
+ * order, nnz values).  grad_on_device = 1: grad is a device pointer (float
+ * for PG_FP32, double for PG_FP64), else a host double array. */
+int pg_count_grad(pg_ctx* ctx, const pg_plan* plan, int ncolorings, const uint8_t* colors,
+                  uint64_t seed, int precision, uint32_t flags, double* raw_out,
+                  void* grad, int grad_on_device);
+
+/* ---- timing of the last pg_count / pg_count_grad call ----
+ * total device milliseconds, and the dominant kernel's launches and summed
+ * milliseconds (HIP events on the context's stream). */
+int pg_last_timing(const pg_ctx* ctx, double* total_ms, double* spmm_ms, int64_t* spmm_launches,
+                   double* spmm_bytes);
+/* name of that dominant kernel ("k_sub<float,64,0,1>": dtype, team lanes,
+ * aux mode, main mode); timing is collected when option "timing" is 1 */
+int pg_last_timing_kernel(const pg_ctx* ctx, char* buf, int len);
+
+/* ---- host surface helpers used by the CLI and the Python layer ---- */
+/* R-MAT generator: n = 2^scale, edge_factor*n draws with quadrant
+ * probabilities a,b,c (d = 1-a-b-c), self loops dropped, (min,max)
+ * canonical, duplicates dropped, vertex labels permuted by a seeded shuffle.
+ * Writes at most cap undirected edges; *m_out = count (call with cap = 0 to
+ * size).  No reference counterpart (graph500-1.2/ is not built). */
+int pg_rmat_edges(int scale, int edge_factor, double a, double b, double c, uint64_t seed,
+                  int64_t cap, int32_t* src, int32_t* dst, int64_t* m_out);
+/* CSC build exactly as CSCGraph::createFromEdgeListFile (CSCGraph.hpp:96-178):
+ * colptr[n+1], rowidx[2m] */
+int pg_csc_from_edges(int32_t n, int64_t m, const int32_t* src, const int32_t* dst,
+                      int32_t* colptr, int32_t* rowidx);
+/* The text loader's "hole" rule (EdgeList.cpp:70-112) on an in-memory list:
+ * if the largest id is not n-1, the ids that occur are renumbered 0.. in
+ * ascending order.  Returns the resulting n (or a negative PG_ERR_*). */
+int64_t pg_edges_remove_holes(int64_t n, int64_t m, int32_t* src, int32_t* dst);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

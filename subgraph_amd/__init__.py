@@ -1,0 +1,264 @@
+"""subgraph_amd -- PGBSC color-coding subgraph counting on AMD MI355X.
+
+Python face of libpgbsc.so.  The DP runs in hand-written HIP kernels; this
+module only moves arrays across the C ABI (include/pgbsc.h).
+
+Reference surface (dsc-nec/subgraph) and where it lives here:
+  CSCGraph::createFromEdgeListFile  -> csc_from_edges()
+  EdgeList hole removal             -> remove_holes()
+  DivideTemplates + IndexSys        -> Context.plan()
+  CountMat::colorCounting loop      -> Context.count()
+  CountMat::initialization/compute  -> CountMat
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+import numpy as np
+
+from ._lib import (PG_FLAG_FIX_2VERTEX, PG_FP32, PG_FP64, PGError, exported_symbols, lib)
+
+__all__ = ["Context", "Plan", "CountMat", "PGError", "rmat_edges", "csc_from_edges", "colors_fill",
+           "remove_holes", "exported_symbols", "PG_FP32", "PG_FP64"]
+
+_PREC = {"fp32": PG_FP32, "fp64": PG_FP64, PG_FP32: PG_FP32, PG_FP64: PG_FP64}
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def rmat_edges(scale: int, edge_factor: int, seed: int = 1, a: float = 0.57, b: float = 0.19,
+               c: float = 0.19):
+    """Undirected R-MAT edge list (src, dst) with src < dst, deduplicated."""
+    L = lib()
+    m = C.c_int64()
+    rc = L.pg_rmat_edges(scale, edge_factor, a, b, c, seed, 0, None, None, C.byref(m))
+    if rc:
+        raise PGError(rc, "pg_rmat_edges")
+    src = np.empty(m.value, np.int32)
+    dst = np.empty(m.value, np.int32)
+    rc = L.pg_rmat_edges(scale, edge_factor, a, b, c, seed, m.value, src.ctypes.data, dst.ctypes.data,
+                         C.byref(m))
+    if rc:
+        raise PGError(rc, "pg_rmat_edges")
+    return src, dst
+
+
+def csc_from_edges(n: int, src, dst):
+    """The reference's CSC (CSCGraph.hpp:96-178): colptr[n+1], rowidx[2m]."""
+    src, dst = _i32(src), _i32(dst)
+    colptr = np.empty(n + 1, np.int32)
+    rowidx = np.empty(max(2 * len(src), 1), np.int32)
+    rc = lib().pg_csc_from_edges(n, len(src), src, dst, colptr, rowidx.ctypes.data)
+    if rc:
+        raise PGError(rc, "pg_csc_from_edges")
+    return colptr, rowidx[: 2 * len(src)]
+
+
+def remove_holes(n: int, src, dst):
+    """EdgeList.cpp:70-112 on an in-memory list; returns (n, src, dst)."""
+    src, dst = _i32(src).copy(), _i32(dst).copy()
+    r = lib().pg_edges_remove_holes(n, len(src), src, dst)
+    if r < 0:
+        raise PGError(int(r), "pg_edges_remove_holes")
+    return int(r), src, dst
+
+
+def template_tables(nv: int, edges) -> dict:
+    """Host-only DivideTemplates + IndexSys output (what pg_plan_create takes)."""
+    e = np.asarray(edges, dtype=np.int32).reshape(-1, 2)
+    s, d = _i32(e[:, 0]), _i32(e[:, 1])
+    L = lib()
+    nsub, ncomb, nsplit = C.c_int32(), C.c_int64(), C.c_int64()
+    rc = L.pg_template_sizes(nv, len(e), s, d, C.byref(nsub), C.byref(ncomb), C.byref(nsplit))
+    if rc:
+        raise PGError(rc, "pg_template_sizes")
+    out = {"k": nv, "sub_size": np.zeros(nsub.value, np.int32), "main_idx": np.zeros(nsub.value, np.int32),
+           "aux_idx": np.zeros(nsub.value, np.int32), "comb_to_count": np.zeros(ncomb.value, np.int32),
+           "split_main": np.zeros(max(nsplit.value, 1), np.int32),
+           "split_aux": np.zeros(max(nsplit.value, 1), np.int32)}
+    rc = L.pg_template_tables(nv, len(e), s, d, out["sub_size"], out["main_idx"], out["aux_idx"],
+                              out["comb_to_count"], out["split_main"], out["split_aux"])
+    if rc:
+        raise PGError(rc, "pg_template_tables")
+    out["split_main"] = out["split_main"][: nsplit.value]
+    out["split_aux"] = out["split_aux"][: nsplit.value]
+    return out
+
+
+def colors_fill(seed: int, first: int, ncolorings: int, n: int, k: int) -> np.ndarray:
+    """The device color stream, reproduced on the host: [ncolorings][n] uint8."""
+    out = np.empty((ncolorings, n), np.uint8)
+    lib().pg_colors_fill(seed, first, ncolorings, n, k, out.reshape(-1))
+    return out
+
+
+class Plan:
+    """A template's subtemplate chain and index tables on the device."""
+
+    def __init__(self, ctx: "Context", ptr: int, nv: int):
+        self._ctx = ctx
+        self.ptr = ptr
+        k, nsub, nonleaf, scaled = (C.c_int() for _ in range(4))
+        lib().pg_plan_info(ptr, C.byref(k), C.byref(nsub), C.byref(nonleaf), C.byref(scaled))
+        self.k, self.nsub, self.n_nonleaf, self.scaled_sub = k.value, nsub.value, nonleaf.value, scaled.value
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            lib().pg_plan_destroy(self.ptr)
+            self.ptr = None
+
+
+class Context:
+    """One device, one graph, any number of plans (pg_ctx)."""
+
+    def __init__(self, device: int = 0, **options):
+        L = lib()
+        p = C.c_void_p()
+        rc = L.pg_create(C.byref(p), device)
+        if rc:
+            raise PGError(rc, f"pg_create(device={device}): no usable HIP device")
+        self.ptr = p.value
+        self.n = self.nnz = 0
+        for k, v in options.items():
+            self.set_option(k, v)
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            lib().pg_destroy(self.ptr)
+            self.ptr = None
+
+    def _check(self, rc: int, what: str):
+        if rc:
+            raise PGError(rc, f"{what}: {lib().pg_last_error(self.ptr).decode()}")
+
+    def set_option(self, key: str, value: int):
+        self._check(lib().pg_set_option(self.ptr, key.encode(), int(value)), f"pg_set_option({key})")
+
+    def get_option(self, key: str) -> int:
+        v = C.c_int64()
+        self._check(lib().pg_get_option(self.ptr, key.encode(), C.byref(v)), f"pg_get_option({key})")
+        return v.value
+
+    def upload_graph(self, colptr, rowidx, vals=None):
+        colptr, rowidx = _i32(colptr), _i32(rowidx)
+        vp = None
+        if vals is not None:
+            vals = np.ascontiguousarray(vals, np.float32)
+            vp = vals.ctypes.data
+        self._check(lib().pg_graph_upload(self.ptr, len(colptr) - 1, len(rowidx), colptr, rowidx, vp),
+                    "pg_graph_upload")
+        self.n, self.nnz = len(colptr) - 1, len(rowidx)
+
+    def set_weights(self, vals=None, device_ptr: int | None = None):
+        if device_ptr is not None:
+            self._check(lib().pg_graph_set_weights(self.ptr, device_ptr, 1), "pg_graph_set_weights")
+        elif vals is None:
+            self._check(lib().pg_graph_set_weights(self.ptr, None, 0), "pg_graph_set_weights")
+        else:
+            vals = np.ascontiguousarray(vals, np.float32)
+            self._check(lib().pg_graph_set_weights(self.ptr, vals.ctypes.data, 0), "pg_graph_set_weights")
+
+    def stats(self) -> dict:
+        v = [C.c_int64() for _ in range(4)]
+        self._check(lib().pg_graph_stats(self.ptr, *[C.byref(x) for x in v]), "pg_graph_stats")
+        return dict(zip(("n", "nnz", "n_active", "max_degree"), (x.value for x in v)))
+
+    def plan(self, nv: int, edges) -> Plan:
+        e = np.asarray(edges, dtype=np.int32).reshape(-1, 2)
+        p = C.c_void_p()
+        self._check(lib().pg_plan_from_edges(self.ptr, nv, len(e), _i32(e[:, 0]), _i32(e[:, 1]), C.byref(p)),
+                    "pg_plan_from_edges")
+        return Plan(self, p.value, nv)
+
+    def plan_from_tables(self, k, sub_size, main_idx, aux_idx, comb_to_count, split_main, split_aux) -> Plan:
+        p = C.c_void_p()
+        self._check(lib().pg_plan_create(self.ptr, k, len(sub_size), _i32(sub_size), _i32(main_idx),
+                                         _i32(aux_idx), _i32(comb_to_count), _i32(split_main),
+                                         _i32(split_aux), C.byref(p)), "pg_plan_create")
+        return Plan(self, p.value, k)
+
+    def count(self, plan: Plan, ncolorings: int, colors=None, seed: int = 0, precision="fp32",
+              fix_2vertex: bool = False) -> np.ndarray:
+        """Raw colorful count of each coloring (CountMat::colorCounting)."""
+        out = np.zeros(ncolorings, np.float64)
+        cp = None
+        if colors is not None:
+            colors = np.ascontiguousarray(colors, np.uint8)
+            if colors.size != ncolorings * self.n:
+                raise ValueError("colors must be [ncolorings][n]")
+            cp = colors.ctypes.data
+        flags = PG_FLAG_FIX_2VERTEX if fix_2vertex else 0
+        self._check(lib().pg_count(self.ptr, plan.ptr, ncolorings, cp, seed, _PREC[precision], flags, out),
+                    "pg_count")
+        return out
+
+    def count_multi(self, plans, ncolorings, seeds, precision="fp32") -> list[np.ndarray]:
+        arr = (C.c_void_p * len(plans))(*[p.ptr for p in plans])
+        nc = _i32(ncolorings)
+        sd = (C.c_uint64 * len(plans))(*seeds)
+        out = np.zeros(int(nc.sum()), np.float64)
+        self._check(lib().pg_count_multi(self.ptr, len(plans), C.cast(arr, C.c_void_p), nc,
+                                         C.cast(sd, C.c_void_p), _PREC[precision], 0, out), "pg_count_multi")
+        return np.split(out, np.cumsum(nc)[:-1])
+
+    def last_timing(self) -> dict:
+        t, km, kb = C.c_double(), C.c_double(), C.c_double()
+        kl = C.c_int64()
+        lib().pg_last_timing(self.ptr, C.byref(t), C.byref(km), C.byref(kl), C.byref(kb))
+        buf = C.create_string_buffer(128)
+        lib().pg_last_timing_kernel(self.ptr, buf, 128)
+        return {"total_ms": t.value, "kernel": buf.value.decode(), "kernel_ms": km.value,
+                "kernel_launches": kl.value, "kernel_bytes": kb.value}
+
+
+class CountMat:
+    """CountMat.hpp:19-191: initialization(graph, ...) then compute(template)."""
+
+    def initialization(self, colptr, rowidx, thd_num: int, itr_num: int, isPruned: int = 1,
+                       useSPMM: int = 1, vtuneStart: int = -1, calculate_automorphisms: bool = False,
+                       device: int = 0, precision="fp32", seed: int = 0):
+        self.ctx = Context(device)
+        self.ctx.upload_graph(colptr, rowidx)
+        self.itr_num, self.precision, self.seed = itr_num, precision, seed
+        self.calculate_automorphisms = calculate_automorphisms
+
+    def compute(self, nv: int, edges, isEstimate: bool = False) -> float:
+        plan = self.ctx.plan(nv, edges)
+        if isEstimate:
+            return 0.0
+        self.raw = self.ctx.count(plan, self.itr_num, seed=self.seed, precision=self.precision)
+        k = nv
+        prob = math.factorial(k) / float(k) ** k  # CountMat.cpp:248-249
+        aut = automorphisms(nv, edges) if self.calculate_automorphisms else 1
+        return math.floor(float(np.mean(self.raw)) / (prob * aut) + 0.5)
+
+
+def automorphisms(nv: int, edges) -> int:
+    """Edge-preserving vertex permutations of a small template (CountMat.cpp:1498-1572)."""
+    adj = np.zeros((nv, nv), bool)
+    for u, v in np.asarray(edges).reshape(-1, 2):
+        adj[u, v] = adj[v, u] = True
+    deg = adj.sum(1)
+    count = 0
+    perm = [-1] * nv
+    used = [False] * nv
+
+    def rec(i):
+        nonlocal count
+        if i == nv:
+            count += 1
+            return
+        for cnd in range(nv):
+            if used[cnd] or deg[cnd] != deg[i]:
+                continue
+            if all(adj[i, j] == adj[cnd, perm[j]] for j in range(i)):
+                used[cnd] = True
+                perm[i] = cnd
+                rec(i + 1)
+                used[cnd] = False
+
+    rec(0)
+    return count

@@ -1,0 +1,897 @@
+// C ABI of libpgbsc.so: context, graph upload (relabelling and degree
+// buckets), plans, the per-pass DP schedule and its memory plan.
+//
+// Reference counterparts: CountMat::initialization/compute/colorCounting
+// (CountMat.cpp:21-322), DataTableColMajor table lifetimes
+// (DataTableColMajor.cpp:63-249), CSCGraph (CSCGraph.hpp:96-217).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "pg_host.hpp"
+#include "pg_kernels.hpp"
+#include "pgbsc.h"
+
+using namespace pgbsc;
+
+namespace {
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    hipError_t alloc(size_t n)
+    {
+        if (n <= bytes && p) return hipSuccess;
+        release();
+        if (n == 0) return hipSuccess;
+        hipError_t e = hipMalloc(&p, n);
+        if (e == hipSuccess) bytes = n;
+        else p = nullptr;
+        return e;
+    }
+    template <typename T>
+    T* as() const { return static_cast<T*>(p); }
+};
+
+struct LaunchRec {
+    std::string name;
+    double bytes;
+    hipEvent_t beg, end;
+};
+
+// One non-leaf subtemplate as the device schedule sees it.
+struct DevSub {
+    int s, size, mi, ai, mm, aa;
+    int Cs, Ca, Cm, P;
+    bool root, scaled;
+    DevBuf split;  // [Cs][P] uint32 by output colex rank
+    DevBuf drop;   // [Cs][k] int16 (main is a leaf)
+};
+
+}  // namespace
+
+struct pg_plan {
+    pg_ctx* ctx = nullptr;
+    int device = 0;
+    int k = 0, nsub = 0, nonleaf = 0, scaled_sub = -1;
+    std::vector<int> sizes, mains, auxs;
+    std::vector<std::unique_ptr<DevSub>> subs;  // processing order (descending s)
+};
+
+struct pg_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    // options
+    int64_t hub_degree = 2048, hub_chunk = 1024, batch_tile = 0, mem_budget_mb = 0, timing = 0, autotune = 0;
+    // graph
+    bool has_graph = false, weighted = false;
+    int32_t n = 0, n_act = 0, max_deg = 0, n_hub = 0;
+    int64_t nnz = 0;
+    std::vector<int32_t> h_rowptr;
+    DevBuf d_rowptr, d_col, d_ent, d_perm, d_w32, d_w64, d_wsrc;
+    // hub work items for the current hub_degree/hub_chunk
+    int64_t items_for_deg = -1, items_for_chunk = -1;
+    int32_t n_hub_items = 0, n_slots = 0;
+    DevBuf d_hub_items, d_slot_begin;
+    // per-call workspace
+    DevBuf d_arena, d_colors_in, d_raw;
+    // timing of the last call
+    std::vector<LaunchRec> recs;
+    double last_total_ms = 0;
+    std::string dom_name;
+    double dom_ms = 0, dom_bytes = 0;
+    int64_t dom_launches = 0;
+};
+
+namespace {
+
+int set_err(pg_ctx* ctx, int code, const char* fmt, ...)
+{
+    if (!ctx) return code;
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    ctx->err = buf;
+    return code;
+}
+
+#define HIP_TRY(ctx, expr)                                                                          \
+    do {                                                                                            \
+        hipError_t e_ = (expr);                                                                     \
+        if (e_ != hipSuccess)                                                                       \
+            return set_err(ctx, e_ == hipErrorOutOfMemory ? PG_ERR_NOMEM : PG_ERR_HIP, "%s: %s (%s:%d)", \
+                           #expr, hipGetErrorString(e_), __FILE__, __LINE__);                      \
+    } while (0)
+
+int team_for(int cols)
+{
+    if (cols <= 8) return 8;
+    if (cols <= 16) return 16;
+    if (cols <= 32) return 32;
+    return 64;
+}
+
+// (Re)build the hub chunk items for the current options.
+int build_hub_items(pg_ctx* c)
+{
+    if (c->items_for_deg == c->hub_degree && c->items_for_chunk == c->hub_chunk) return PG_OK;
+    int32_t nh = 0;
+    while (nh < c->n_act && c->h_rowptr[nh + 1] - c->h_rowptr[nh] > c->hub_degree) ++nh;
+    std::vector<int4> items;
+    std::vector<int32_t> slot_begin(nh + 1, 0);
+    for (int32_t h = 0; h < nh; ++h) {
+        slot_begin[h] = (int32_t)items.size();
+        for (int32_t e = c->h_rowptr[h]; e < c->h_rowptr[h + 1]; e += (int32_t)c->hub_chunk) {
+            int32_t e1 = std::min<int64_t>(e + c->hub_chunk, c->h_rowptr[h + 1]);
+            items.push_back(make_int4(h, e, e1, (int)items.size()));
+        }
+    }
+    slot_begin[nh] = (int32_t)items.size();
+    c->n_hub = nh;
+    c->n_hub_items = (int32_t)items.size();
+    c->n_slots = (int32_t)items.size();
+    HIP_TRY(c, c->d_hub_items.alloc(std::max<size_t>(items.size(), 1) * sizeof(int4)));
+    HIP_TRY(c, c->d_slot_begin.alloc(slot_begin.size() * sizeof(int32_t)));
+    if (!items.empty())
+        HIP_TRY(c, hipMemcpy(c->d_hub_items.p, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(c->d_slot_begin.p, slot_begin.data(), slot_begin.size() * sizeof(int32_t),
+                         hipMemcpyHostToDevice));
+    c->items_for_deg = c->hub_degree;
+    c->items_for_chunk = c->hub_chunk;
+    return PG_OK;
+}
+
+// ---- memory plan for one pass: first-fit offsets over table lifetimes ----
+struct MemPlan {
+    size_t total = 0;
+    size_t off_H = 0, off_part = 0, off_root = 0, off_colors = 0, off_rawdev = 0;
+    std::vector<size_t> off_tab;  // per subtemplate index (tables of non-root non-leaf)
+};
+
+size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+int64_t blocks_for(int64_t items, int team) { return (items + (kBlock / team) - 1) / (kBlock / team); }
+
+MemPlan plan_memory(const pg_ctx* c, const pg_plan* p, int B, size_t tsz)
+{
+    MemPlan m;
+    const int S = p->nsub;
+    m.off_tab.assign(S, 0);
+    // live intervals: [start step, end step] in processing order
+    struct Iv { size_t bytes; int beg, end; size_t* off; };
+    std::vector<Iv> ivs;
+    const size_t na = (size_t)c->n_act;
+    int nsteps = (int)p->subs.size();
+    std::vector<int> step_of(S, -1);
+    for (int i = 0; i < nsteps; ++i) step_of[p->subs[i]->s] = i;
+    // H lives for the whole pass (step -1 .. last)
+    ivs.push_back({align_up(na * B * p->k * tsz, 256), -1, nsteps, &m.off_H});
+    // hub partial rows
+    int maxCa = p->k;
+    for (auto& d : p->subs) if (d->aa > 1) maxCa = std::max(maxCa, d->Ca);
+    ivs.push_back({align_up((size_t)std::max(c->n_slots, 1) * B * maxCa * tsz, 256), -1, nsteps, &m.off_part});
+    // per-vertex root values [B][n_act]
+    ivs.push_back({align_up(na * B * sizeof(double), 256), -1, nsteps, &m.off_root});
+    ivs.push_back({align_up(na * B, 256), -1, nsteps, &m.off_colors});
+    ivs.push_back({align_up((size_t)B * sizeof(double), 256), -1, nsteps, &m.off_rawdev});
+    for (int i = 0; i < nsteps; ++i) {
+        const DevSub* d = p->subs[i].get();
+        if (d->root) continue;
+        // consumed by the parent step
+        int parent_step = -1;
+        for (int j = i + 1; j < nsteps; ++j)
+            if (p->subs[j]->mi == d->s || p->subs[j]->ai == d->s) { parent_step = j; break; }
+        ivs.push_back({align_up(na * B * d->Cs * tsz, 256), i, parent_step, &m.off_tab[d->s]});
+    }
+    // first-fit in order of start step
+    struct Placed { size_t off, bytes; int beg, end; };
+    std::vector<Placed> placed;
+    for (auto& iv : ivs) {
+        std::vector<std::pair<size_t, size_t>> busy;
+        for (auto& pl : placed)
+            if (!(pl.end < iv.beg || iv.end < pl.beg)) busy.push_back({pl.off, pl.off + pl.bytes});
+        std::sort(busy.begin(), busy.end());
+        size_t off = 0;
+        for (auto& bz : busy) {
+            if (off + iv.bytes <= bz.first) break;
+            off = std::max(off, bz.second);
+        }
+        *iv.off = off;
+        placed.push_back({off, iv.bytes, iv.beg, iv.end});
+        m.total = std::max(m.total, off + iv.bytes);
+    }
+    return m;
+}
+
+size_t lds_bytes(int team, int B, int Ca, int Cm_lds, size_t tsz)
+{
+    const int teams = kBlock / team;
+    return (size_t)teams * B * (Ca + Cm_lds) * tsz;
+}
+
+size_t max_lds(const pg_plan* p, int B, size_t tsz)
+{
+    size_t mx = 0;
+    for (auto& d : p->subs) {
+        int team = d->aa > 1 ? team_for(B * d->Ca) : team_for(std::max(B * d->Ca, B * d->Cs));
+        mx = std::max(mx, lds_bytes(team, B, d->Ca, d->mm > 1 ? d->Cm : 0, tsz));
+    }
+    return mx;
+}
+
+constexpr size_t kLdsLimit = 64 * 1024;
+
+void record(pg_ctx* c, const std::string& name, double bytes, bool begin)
+{
+    if (!c->timing) return;
+    if (begin) {
+        LaunchRec r{name, bytes, nullptr, nullptr};
+        (void)hipEventCreate(&r.beg);
+        (void)hipEventCreate(&r.end);
+        (void)hipEventRecord(r.beg, c->stream);
+        c->recs.push_back(r);
+    } else {
+        (void)hipEventRecord(c->recs.back().end, c->stream);
+    }
+}
+
+template <typename T>
+std::string kname(int team, int aux, int main_mode)
+{
+    char buf[96];
+    snprintf(buf, sizeof buf, "k_sub<%s,%d,%d,%d>", sizeof(T) == 4 ? "float" : "double", team, aux, main_mode);
+    return buf;
+}
+
+// Runs one pass of B colorings (first..first+B-1) through the whole DP.
+template <typename T>
+int run_pass(pg_ctx* c, const pg_plan* p, int B, int64_t first, const uint8_t* d_colors_in, uint64_t seed,
+             const MemPlan& mp, uint32_t flags, double* raw_host)
+{
+    char* base = c->d_arena.as<char>();
+    uint8_t* colors = (uint8_t*)(base + mp.off_colors);
+    T* H = (T*)(base + mp.off_H);
+    T* part = (T*)(base + mp.off_part);
+    double* rootp = (double*)(base + mp.off_root);
+    double* rawd = (double*)(base + mp.off_rawdev);
+    const T* w = nullptr;
+    if (c->weighted) w = sizeof(T) == 4 ? (const T*)c->d_w32.p : (const T*)c->d_w64.p;
+    const hipStream_t st = c->stream;
+    const int k = p->k;
+    const double nb_csc = 4.0 * (c->n_act + 1) + 4.0 * c->nnz * (c->weighted ? 2 : 1);
+    const double na = (double)c->n_act;
+
+    if (d_colors_in)
+        HIP_TRY(c, launch_colors_gather(colors, d_colors_in + first * (int64_t)c->n, c->d_perm.as<int32_t>(),
+                                        c->n_act, B, c->n, st));
+    else
+        HIP_TRY(c, launch_colors_rng(colors, c->d_perm.as<int32_t>(), c->n_act, B, seed, first, k, st));
+
+    SubArgs<T> base_args{};
+    base_args.rowptr = c->d_rowptr.as<int32_t>();
+    base_args.col = c->d_col.as<int32_t>();
+    base_args.w = w;
+    base_args.colors = colors;
+    base_args.B = B;
+    base_args.k = k;
+    base_args.slot_begin = c->d_slot_begin.as<int32_t>();
+    base_args.scale = 1.0e-12f;
+
+    // ---- neighbour-color histogram H = A * leaf (shared by every leaf aux)
+    bool need_H = false;
+    for (auto& d : p->subs) need_H |= d->aa == 1;
+    if (need_H) {
+        SubArgs<T> a = base_args;
+        a.Ca = k;
+        a.out = H;
+        a.ldo = (int64_t)B * k;
+        a.out_mode = OUT_Y;
+        const int team = team_for(B * k);
+        a.n_items = c->n_act - c->n_hub;
+        a.item_base = c->n_hub;
+        record(c, kname<T>(team, AUX_COLORS, MAIN_LEAF), nb_csc + na * B * 1.0, true);
+        HIP_TRY(c, launch_sub<T>(team, AUX_COLORS, MAIN_LEAF, a, (int)blocks_for(a.n_items, team), 0, st));
+        record(c, "", 0, false);
+        if (c->n_hub > 0) {
+            SubArgs<T> h = base_args;
+            h.Ca = k;
+            h.out = part;
+            h.ldo = (int64_t)B * k;
+            h.out_mode = OUT_PARTIAL;
+            h.items = c->d_hub_items.as<int4>();
+            h.n_items = c->n_hub_items;
+            HIP_TRY(c, launch_sub<T>(team, AUX_COLORS, MAIN_LEAF, h, (int)blocks_for(h.n_items, team), 0, st));
+            SubArgs<T> f = base_args;
+            f.Ca = k;
+            f.aux = part;
+            f.lda = (int64_t)B * k;
+            f.out = H;
+            f.ldo = (int64_t)B * k;
+            f.out_mode = OUT_Y;
+            f.n_items = c->n_hub;
+            f.item_base = 0;
+            HIP_TRY(c, launch_sub<T>(team, AUX_PARTIALS, MAIN_LEAF, f, (int)blocks_for(f.n_items, team), 0, st));
+        }
+    }
+
+    // ---- subtemplates, children before parents
+    for (auto& dp : p->subs) {
+        const DevSub& d = *dp;
+        SubArgs<T> a = base_args;
+        a.Ca = d.Ca;
+        a.Cm = d.Cm;
+        a.Cs = d.Cs;
+        a.P = d.P;
+        a.split = d.split.as<uint32_t>();
+        a.drop = d.drop.as<int16_t>();
+        const int main_mode = d.mm == 1 ? MAIN_LEAF : MAIN_TABLE;
+        if (d.mm > 1) {
+            a.main = (const T*)(base + mp.off_tab[d.mi]);
+            a.ldm = (int64_t)B * d.Cm;
+        }
+        if (d.root) {
+            a.out_mode = OUT_ROOT;
+            a.root_vals = rootp;
+            a.n_root = c->n_act;
+        } else {
+            a.out = (T*)(base + mp.off_tab[d.s]);
+            a.ldo = (int64_t)B * d.Cs;
+            const bool scaled = d.scaled && sizeof(T) == 4;
+            a.out_mode = scaled ? OUT_EMA_SCALED : OUT_EMA;
+        }
+        const double tab_bytes = 4.0 * na * B * ((d.aa > 1 ? d.Ca : 0) + (d.mm > 1 ? d.Cm : 0) + (d.root ? 0 : d.Cs));
+        const int Cm_lds = d.mm > 1 ? d.Cm : 0;
+        if (d.aa == 1) {
+            const int team = team_for(std::max(B * d.Ca, B * d.Cs));
+            a.aux = H;
+            a.lda = (int64_t)B * k;
+            a.n_items = c->n_act;
+            a.item_base = 0;
+            const size_t lds = lds_bytes(team, B, d.Ca, Cm_lds, sizeof(T));
+            record(c, kname<T>(team, AUX_READ, main_mode), tab_bytes, true);
+            HIP_TRY(c, launch_sub<T>(team, AUX_READ, main_mode, a, (int)blocks_for(a.n_items, team), lds, st));
+            record(c, "", 0, false);
+        } else {
+            const int team = team_for(B * d.Ca);
+            const size_t lds = lds_bytes(team, B, d.Ca, Cm_lds, sizeof(T));
+            const T* Ta = (const T*)(base + mp.off_tab[d.ai]);
+            // hub chunks first: partial rows
+            if (c->n_hub > 0) {
+                SubArgs<T> h = a;
+                h.aux = Ta;
+                h.lda = (int64_t)B * d.Ca;
+                h.out = part;
+                h.ldo = (int64_t)B * d.Ca;
+                h.out_mode = OUT_PARTIAL;
+                h.items = c->d_hub_items.as<int4>();
+                h.n_items = c->n_hub_items;
+                HIP_TRY(c, launch_sub<T>(team, AUX_GATHER, main_mode, h, (int)blocks_for(h.n_items, team), 0, st));
+            }
+            a.aux = Ta;
+            a.lda = (int64_t)B * d.Ca;
+            a.n_items = c->n_act - c->n_hub;
+            a.item_base = c->n_hub;
+            record(c, kname<T>(team, AUX_GATHER, main_mode), nb_csc + tab_bytes, true);
+            HIP_TRY(c, launch_sub<T>(team, AUX_GATHER, main_mode, a, (int)blocks_for(a.n_items, team), lds, st));
+            record(c, "", 0, false);
+            if (c->n_hub > 0) {
+                SubArgs<T> f = a;
+                f.aux = part;
+                f.lda = (int64_t)B * d.Ca;
+                f.n_items = c->n_hub;
+                f.item_base = 0;
+                HIP_TRY(c, launch_sub<T>(team, AUX_PARTIALS, main_mode, f, (int)blocks_for(f.n_items, team), lds, st));
+            }
+        }
+        if (d.root) HIP_TRY(c, launch_root_reduce(rootp, c->n_act, B, rawd, st));
+    }
+    std::vector<double> raw(B);
+    HIP_TRY(c, hipMemcpyAsync(raw.data(), rawd, sizeof(double) * B, hipMemcpyDeviceToHost, st));
+    HIP_TRY(c, hipStreamSynchronize(st));
+    // CountMat.cpp:815-824: the root sum is multiplied by 1e12 unconditionally
+    const bool two_vertex = p->nonleaf == 1;
+    for (int b = 0; b < B; ++b) {
+        double r = raw[b];
+        if (sizeof(T) == 4) {
+            if (!(two_vertex && (flags & PG_FLAG_FIX_2VERTEX))) r *= 1.0e12;
+        } else if (two_vertex && !(flags & PG_FLAG_FIX_2VERTEX)) {
+            r *= 1.0e12;
+        }
+        raw_host[b] = r;
+    }
+    return PG_OK;
+}
+
+int choose_tile(pg_ctx* c, const pg_plan* p, int ncol, size_t tsz, int* tile, MemPlan* mp)
+{
+    size_t budget;
+    if (c->mem_budget_mb > 0) {
+        budget = (size_t)c->mem_budget_mb << 20;
+    } else {
+        size_t fr = 0, tot = 0;
+        HIP_TRY(c, hipMemGetInfo(&fr, &tot));
+        budget = (size_t)(0.9 * (double)(fr + c->d_arena.bytes));
+    }
+    int cap = c->batch_tile > 0 ? (int)std::min<int64_t>(c->batch_tile, ncol) : std::min(ncol, 64);
+    for (int B = cap; B >= 1; --B) {
+        if (max_lds(p, B, tsz) > kLdsLimit) continue;
+        MemPlan m = plan_memory(c, p, B, tsz);
+        if (m.total > budget) continue;
+        if (c->batch_tile > 0 && B != cap) break;
+        *tile = B;
+        *mp = m;
+        return PG_OK;
+    }
+    return set_err(c, PG_ERR_NOMEM, "tables of one coloring do not fit the device memory budget (%.1f GB)",
+                   budget / 1e9);
+}
+
+void finish_timing(pg_ctx* c)
+{
+    c->dom_name.clear();
+    c->dom_ms = c->dom_bytes = 0;
+    c->dom_launches = 0;
+    if (!c->timing) return;
+    std::map<std::string, std::pair<double, std::pair<double, int64_t>>> agg;
+    for (auto& r : c->recs) {
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, r.beg, r.end);
+        auto& a = agg[r.name];
+        a.first += ms;
+        a.second.first += r.bytes;
+        a.second.second += 1;
+        (void)hipEventDestroy(r.beg);
+        (void)hipEventDestroy(r.end);
+    }
+    c->recs.clear();
+    for (auto& kv : agg) {
+        if (kv.second.first > c->dom_ms) {
+            c->dom_name = kv.first;
+            c->dom_ms = kv.second.first;
+            c->dom_bytes = kv.second.second.first;
+            c->dom_launches = kv.second.second.second;
+        }
+    }
+}
+
+template <typename T>
+int count_impl(pg_ctx* c, const pg_plan* p, int ncol, const uint8_t* colors, uint64_t seed, uint32_t flags,
+               double* raw_out)
+{
+    int rc = build_hub_items(c);
+    if (rc) return rc;
+    int tile = 1;
+    MemPlan mp;
+    rc = choose_tile(c, p, ncol, sizeof(T), &tile, &mp);
+    if (rc) return rc;
+    HIP_TRY(c, c->d_arena.alloc(mp.total));
+    const uint8_t* d_in = nullptr;
+    if (colors) {
+        HIP_TRY(c, c->d_colors_in.alloc((size_t)ncol * c->n));
+        HIP_TRY(c, hipMemcpyAsync(c->d_colors_in.p, colors, (size_t)ncol * c->n, hipMemcpyHostToDevice, c->stream));
+        d_in = c->d_colors_in.as<uint8_t>();
+    }
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    HIP_TRY(c, hipEventCreate(&t0));
+    HIP_TRY(c, hipEventCreate(&t1));
+    HIP_TRY(c, hipEventRecord(t0, c->stream));
+    for (int first = 0; first < ncol; first += tile) {
+        int B = std::min(tile, ncol - first);
+        MemPlan m = B == tile ? mp : plan_memory(c, p, B, sizeof(T));
+        rc = run_pass<T>(c, p, B, first, d_in, seed, m, flags, raw_out + first);
+        if (rc) break;
+    }
+    (void)hipEventRecord(t1, c->stream);
+    (void)hipEventSynchronize(t1);
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, t0, t1);
+    c->last_total_ms = ms;
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    finish_timing(c);
+    return rc;
+}
+
+}  // namespace
+
+// =========================================================================
+// C ABI
+// =========================================================================
+
+extern "C" int pg_create(pg_ctx** out, int device)
+{
+    if (!out) return PG_ERR_INVALID;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return PG_ERR_HIP;
+    if (device < 0 || device >= ndev) return PG_ERR_INVALID;
+    if (hipSetDevice(device) != hipSuccess) return PG_ERR_HIP;
+    pg_ctx* c = new (std::nothrow) pg_ctx();
+    if (!c) return PG_ERR_NOMEM;
+    c->device = device;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return PG_ERR_HIP;
+    }
+    *out = c;
+    return PG_OK;
+}
+
+extern "C" void pg_destroy(pg_ctx* c)
+{
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    for (auto& r : c->recs) { (void)hipEventDestroy(r.beg); (void)hipEventDestroy(r.end); }
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+extern "C" const char* pg_last_error(const pg_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+extern "C" int pg_set_option(pg_ctx* c, const char* key, int64_t v)
+{
+    if (!c || !key) return PG_ERR_INVALID;
+    std::string k(key);
+    if (k == "hub_degree") {
+        if (v < 16) return set_err(c, PG_ERR_INVALID, "hub_degree must be >= 16");
+        c->hub_degree = v;
+    } else if (k == "hub_chunk") {
+        if (v < 16 || v > (1 << 24)) return set_err(c, PG_ERR_INVALID, "hub_chunk must be in [16, 2^24]");
+        c->hub_chunk = v;
+    } else if (k == "batch_tile") {
+        if (v < 0 || v > 4096) return set_err(c, PG_ERR_INVALID, "batch_tile must be in [0, 4096]");
+        c->batch_tile = v;
+    } else if (k == "mem_budget_mb") {
+        if (v < 0) return set_err(c, PG_ERR_INVALID, "mem_budget_mb must be >= 0");
+        c->mem_budget_mb = v;
+    } else if (k == "timing") {
+        c->timing = v ? 1 : 0;
+    } else if (k == "autotune") {
+        c->autotune = v ? 1 : 0;
+    } else {
+        return set_err(c, PG_ERR_INVALID, "unknown option %s", key);
+    }
+    return PG_OK;
+}
+
+extern "C" int pg_get_option(const pg_ctx* c, const char* key, int64_t* v)
+{
+    if (!c || !key || !v) return PG_ERR_INVALID;
+    std::string k(key);
+    if (k == "hub_degree") *v = c->hub_degree;
+    else if (k == "hub_chunk") *v = c->hub_chunk;
+    else if (k == "batch_tile") *v = c->batch_tile;
+    else if (k == "mem_budget_mb") *v = c->mem_budget_mb;
+    else if (k == "timing") *v = c->timing;
+    else if (k == "autotune") *v = c->autotune;
+    else return PG_ERR_INVALID;
+    return PG_OK;
+}
+
+// Internal order: vertices that appear in any entry, by descending row
+// degree (ties by caller id), so hubs form a prefix and isolated vertices --
+// whose tables are all zero above the leaves -- are dropped.  Row v lists the
+// entries (row v, column u) in ascending u, the order in which the reference
+// accumulates row v (CSCGraph.hpp:199-213, 254-307).
+extern "C" int pg_graph_upload(pg_ctx* c, int32_t n, int64_t nnz, const int32_t* colptr, const int32_t* rowidx,
+                               const float* vals)
+{
+    if (!c) return PG_ERR_INVALID;
+    if (n <= 0 || nnz < 0 || nnz > INT32_MAX || !colptr || (nnz > 0 && !rowidx))
+        return set_err(c, PG_ERR_INVALID, "invalid graph arguments (n=%d nnz=%lld)", n, (long long)nnz);
+    if (colptr[0] != 0 || colptr[n] != nnz) return set_err(c, PG_ERR_INVALID, "colptr must run from 0 to nnz");
+    for (int32_t u = 0; u < n; ++u)
+        if (colptr[u + 1] < colptr[u]) return set_err(c, PG_ERR_INVALID, "colptr not monotone at %d", u);
+    HIP_TRY(c, hipSetDevice(c->device));
+    std::vector<int32_t> rdeg(n, 0);
+    std::vector<char> seen(n, 0);
+    for (int32_t u = 0; u < n; ++u) {
+        if (colptr[u + 1] > colptr[u]) seen[u] = 1;
+        for (int64_t e = colptr[u]; e < colptr[u + 1]; ++e) {
+            int32_t v = rowidx[e];
+            if (v < 0 || v >= n) return set_err(c, PG_ERR_INVALID, "row index %d out of range at entry %lld", v, (long long)e);
+            rdeg[v]++;
+            seen[v] = 1;
+        }
+    }
+    std::vector<int32_t> perm;
+    perm.reserve(n);
+    for (int32_t v = 0; v < n; ++v) if (seen[v]) perm.push_back(v);
+    std::stable_sort(perm.begin(), perm.end(), [&](int32_t a, int32_t b) { return rdeg[a] > rdeg[b]; });
+    const int32_t na = (int32_t)perm.size();
+    std::vector<int32_t> inv(n, -1);
+    for (int32_t i = 0; i < na; ++i) inv[perm[i]] = i;
+    // row-major view in caller ids (ascending column within each row)
+    std::vector<int64_t> rptr((size_t)n + 1, 0);
+    for (int32_t v = 0; v < n; ++v) rptr[v + 1] = rptr[v] + rdeg[v];
+    std::vector<int32_t> rcol((size_t)nnz), rent((size_t)nnz);
+    {
+        std::vector<int64_t> pos(rptr.begin(), rptr.end() - 1);
+        for (int32_t u = 0; u < n; ++u)
+            for (int64_t e = colptr[u]; e < colptr[u + 1]; ++e) {
+                int32_t v = rowidx[e];
+                rcol[pos[v]] = u;
+                rent[pos[v]] = (int32_t)e;
+                pos[v]++;
+            }
+    }
+    std::vector<int32_t> hrow((size_t)na + 1, 0), hcol((size_t)nnz), hent((size_t)nnz);
+    for (int32_t i = 0; i < na; ++i) hrow[i + 1] = hrow[i] + rdeg[perm[i]];
+#pragma omp parallel for schedule(dynamic, 4096)
+    for (int32_t i = 0; i < na; ++i) {
+        int32_t v = perm[i];
+        int64_t o = hrow[i];
+        for (int64_t e = rptr[v]; e < rptr[v + 1]; ++e, ++o) {
+            hcol[o] = inv[rcol[e]];
+            hent[o] = rent[e];
+        }
+    }
+    c->has_graph = false;
+    c->n = n;
+    c->nnz = nnz;
+    c->n_act = na;
+    c->max_deg = na > 0 ? rdeg[perm[0]] : 0;
+    c->h_rowptr = hrow;
+    HIP_TRY(c, c->d_rowptr.alloc(hrow.size() * 4));
+    HIP_TRY(c, c->d_col.alloc(std::max<size_t>(hcol.size(), 1) * 4));
+    HIP_TRY(c, c->d_ent.alloc(std::max<size_t>(hent.size(), 1) * 4));
+    HIP_TRY(c, c->d_perm.alloc(std::max<size_t>(perm.size(), 1) * 4));
+    HIP_TRY(c, hipMemcpy(c->d_rowptr.p, hrow.data(), hrow.size() * 4, hipMemcpyHostToDevice));
+    if (nnz > 0) {
+        HIP_TRY(c, hipMemcpy(c->d_col.p, hcol.data(), hcol.size() * 4, hipMemcpyHostToDevice));
+        HIP_TRY(c, hipMemcpy(c->d_ent.p, hent.data(), hent.size() * 4, hipMemcpyHostToDevice));
+    }
+    if (na > 0) HIP_TRY(c, hipMemcpy(c->d_perm.p, perm.data(), perm.size() * 4, hipMemcpyHostToDevice));
+    c->items_for_deg = c->items_for_chunk = -1;
+    c->weighted = false;
+    c->has_graph = true;
+    int rc = build_hub_items(c);
+    if (rc) return rc;
+    if (vals) return pg_graph_set_weights(c, vals, 0);
+    return PG_OK;
+}
+
+extern "C" int pg_graph_set_weights(pg_ctx* c, const float* vals, int on_device)
+{
+    if (!c) return PG_ERR_INVALID;
+    if (!c->has_graph) return set_err(c, PG_ERR_STATE, "no graph uploaded");
+    HIP_TRY(c, hipSetDevice(c->device));
+    if (!vals) {
+        c->weighted = false;
+        return PG_OK;
+    }
+    const float* src = vals;
+    if (!on_device) {
+        HIP_TRY(c, c->d_wsrc.alloc(std::max<int64_t>(c->nnz, 1) * 4));
+        HIP_TRY(c, hipMemcpyAsync(c->d_wsrc.p, vals, c->nnz * 4, hipMemcpyHostToDevice, c->stream));
+        src = c->d_wsrc.as<float>();
+    }
+    HIP_TRY(c, c->d_w32.alloc(std::max<int64_t>(c->nnz, 1) * 4));
+    HIP_TRY(c, c->d_w64.alloc(std::max<int64_t>(c->nnz, 1) * 8));
+    HIP_TRY(c, launch_gather_weights_f32(c->d_w32.as<float>(), src, c->d_ent.as<int32_t>(), c->nnz, c->stream));
+    HIP_TRY(c, launch_gather_weights_f64(c->d_w64.as<double>(), src, c->d_ent.as<int32_t>(), c->nnz, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->weighted = true;
+    return PG_OK;
+}
+
+extern "C" int pg_graph_stats(const pg_ctx* c, int64_t* n, int64_t* nnz, int64_t* n_active, int64_t* max_degree)
+{
+    if (!c) return PG_ERR_INVALID;
+    if (!c->has_graph) return PG_ERR_STATE;
+    if (n) *n = c->n;
+    if (nnz) *nnz = c->nnz;
+    if (n_active) *n_active = c->n_act;
+    if (max_degree) *max_degree = c->max_deg;
+    return PG_OK;
+}
+
+extern "C" int pg_plan_create(pg_ctx* c, int k, int nsub, const int32_t* sub_size, const int32_t* main_idx,
+                              const int32_t* aux_idx, const int32_t* comb_to_count, const int32_t* split_main,
+                              const int32_t* split_aux, pg_plan** out)
+{
+    if (!c || !out) return PG_ERR_INVALID;
+    *out = nullptr;
+    if (k < 2 || k > kMaxTemplate || nsub != 2 * k - 1 || !sub_size || !main_idx || !aux_idx || !comb_to_count ||
+        !split_main || !split_aux)
+        return set_err(c, PG_ERR_INVALID, "plan needs 2 <= k <= %d and nsub = 2k-1", kMaxTemplate);
+    if (sub_size[0] != k) return set_err(c, PG_ERR_INVALID, "subtemplate 0 must be the whole template");
+    int nonleaf = 0;
+    for (int s = 0; s < nsub; ++s) {
+        int sz = sub_size[s];
+        if (sz < 1 || sz > k) return set_err(c, PG_ERR_INVALID, "subtemplate %d has size %d", s, sz);
+        if (sz == 1) continue;
+        ++nonleaf;
+        int mi = main_idx[s], ai = aux_idx[s];
+        if (mi <= s || ai <= s || mi >= nsub || ai >= nsub || mi == ai)
+            return set_err(c, PG_ERR_INVALID, "subtemplate %d children must follow it", s);
+        if (sub_size[mi] + sub_size[ai] != sz)
+            return set_err(c, PG_ERR_INVALID, "subtemplate %d: child sizes do not add up", s);
+    }
+    HIP_TRY(c, hipSetDevice(c->device));
+    auto plan = std::make_unique<pg_plan>();
+    plan->ctx = c;
+    plan->device = c->device;
+    plan->k = k;
+    plan->nsub = nsub;
+    plan->nonleaf = nonleaf;
+    plan->sizes.assign(sub_size, sub_size + nsub);
+    plan->mains.assign(main_idx, main_idx + nsub);
+    plan->auxs.assign(aux_idx, aux_idx + nsub);
+    std::vector<int64_t> c2c_off(nsub + 1, 0), split_off(nsub + 1, 0);
+    for (int s = 0; s < nsub; ++s) {
+        c2c_off[s + 1] = c2c_off[s] + IndexSys::comb_calc(k, sub_size[s]);
+        int64_t ns = 0;
+        if (sub_size[s] > 1) ns = (int64_t)IndexSys::comb_calc(k, sub_size[s]) * IndexSys::comb_calc(sub_size[s], sub_size[main_idx[s]]);
+        split_off[s + 1] = split_off[s] + ns;
+    }
+    // the reference scales the first non-leaf it processes (highest index)
+    // unless that is the root (CountMat.cpp:783-790, 811)
+    int first_nonleaf = -1;
+    for (int s = nsub - 1; s >= 0; --s) if (sub_size[s] > 1) { first_nonleaf = s; break; }
+    plan->scaled_sub = first_nonleaf > 0 ? first_nonleaf : -1;
+    for (int s = nsub - 1; s >= 0; --s) {
+        if (sub_size[s] == 1) continue;
+        auto d = std::make_unique<DevSub>();
+        d->s = s;
+        d->size = sub_size[s];
+        d->mi = main_idx[s];
+        d->ai = aux_idx[s];
+        d->mm = sub_size[d->mi];
+        d->aa = sub_size[d->ai];
+        d->Cs = IndexSys::comb_calc(k, d->size);
+        d->Ca = IndexSys::comb_calc(k, d->aa);
+        d->Cm = IndexSys::comb_calc(k, d->mm);
+        d->P = IndexSys::comb_calc(d->size, d->mm);
+        d->root = s == 0;
+        d->scaled = s == plan->scaled_sub;
+        const int32_t* c2c = comb_to_count + c2c_off[s];
+        const int32_t* sm = split_main + split_off[s];
+        const int32_t* sa = split_aux + split_off[s];
+        std::vector<uint32_t> by_rank((size_t)d->Cs * d->P, 0);
+        std::vector<char> hit(d->Cs, 0);
+        std::vector<int16_t> drop;
+        if (d->mm == 1) drop.assign((size_t)d->Cs * k, (int16_t)-1);
+        for (int i = 0; i < d->Cs; ++i) {
+            int r = c2c[i];
+            if (r < 0 || r >= d->Cs || hit[r]) return set_err(c, PG_ERR_INVALID, "comb_to_count of %d is not a permutation", s);
+            hit[r] = 1;
+            for (int j = 0; j < d->P; ++j) {
+                int m = sm[(int64_t)i * d->P + j], a = sa[(int64_t)i * d->P + j];
+                if (m < 0 || m >= d->Cm || a < 0 || a >= d->Ca)
+                    return set_err(c, PG_ERR_INVALID, "split index out of range in subtemplate %d", s);
+                by_rank[(size_t)r * d->P + j] = ((uint32_t)m << 16) | (uint32_t)a;
+                if (d->mm == 1) drop[(size_t)r * k + m] = (int16_t)a;  // colex rank of {c} is c
+            }
+        }
+        HIP_TRY(c, d->split.alloc(by_rank.size() * 4));
+        HIP_TRY(c, hipMemcpy(d->split.p, by_rank.data(), by_rank.size() * 4, hipMemcpyHostToDevice));
+        if (d->mm == 1) {
+            HIP_TRY(c, d->drop.alloc(drop.size() * 2));
+            HIP_TRY(c, hipMemcpy(d->drop.p, drop.data(), drop.size() * 2, hipMemcpyHostToDevice));
+        }
+        plan->subs.push_back(std::move(d));
+    }
+    *out = plan.release();
+    return PG_OK;
+}
+
+extern "C" int pg_plan_from_edges(pg_ctx* c, int nv, int ne, const int32_t* src, const int32_t* dst, pg_plan** out)
+{
+    if (!c || !out || !src || !dst) return PG_ERR_INVALID;
+    if (nv < 2 || nv > kMaxTemplate || ne != nv - 1)
+        return set_err(c, PG_ERR_INVALID, "template must be a tree with 2..%d vertices", kMaxTemplate);
+    for (int j = 0; j < ne; ++j)
+        if (src[j] < 0 || src[j] >= nv || dst[j] < 0 || dst[j] >= nv)
+            return set_err(c, PG_ERR_INVALID, "template vertex id out of range");
+    Graph g;
+    std::vector<int> s(src, src + ne), d(dst, dst + ne);
+    g.build_graph(nv, ne, s.data(), d.data());
+    FlatPlan fp;
+    std::string err;
+    int rc = flatten_plan(g, &fp, &err);
+    if (rc) return set_err(c, rc, "%s", err.c_str());
+    return pg_plan_create(c, fp.k, fp.nsub, fp.sub_size.data(), fp.main_idx.data(), fp.aux_idx.data(),
+                          fp.comb_to_count.data(), fp.split_main.data(), fp.split_aux.data(), out);
+}
+
+extern "C" void pg_plan_destroy(pg_plan* p)
+{
+    if (!p) return;
+    (void)hipSetDevice(p->device);
+    delete p;
+}
+
+extern "C" int pg_plan_info(const pg_plan* p, int* k, int* nsub, int* n_nonleaf, int* scaled_sub)
+{
+    if (!p) return PG_ERR_INVALID;
+    if (k) *k = p->k;
+    if (nsub) *nsub = p->nsub;
+    if (n_nonleaf) *n_nonleaf = p->nonleaf;
+    if (scaled_sub) *scaled_sub = p->scaled_sub;
+    return PG_OK;
+}
+
+extern "C" int pg_count(pg_ctx* c, const pg_plan* p, int ncol, const uint8_t* colors, uint64_t seed, int precision,
+                        uint32_t flags, double* raw_out)
+{
+    if (!c) return PG_ERR_INVALID;
+    if (!p || p->ctx != c || ncol <= 0 || !raw_out) return set_err(c, PG_ERR_INVALID, "invalid count arguments");
+    if (precision != PG_FP32 && precision != PG_FP64) return set_err(c, PG_ERR_INVALID, "precision must be PG_FP32 or PG_FP64");
+    if (!c->has_graph) return set_err(c, PG_ERR_STATE, "no graph uploaded");
+    if (colors) {
+        const size_t total = (size_t)ncol * c->n;
+        for (size_t i = 0; i < total; ++i)
+            if (colors[i] >= p->k) return set_err(c, PG_ERR_INVALID, "color %d >= k at %zu", colors[i], i);
+    }
+    HIP_TRY(c, hipSetDevice(c->device));
+    if (c->n_act == 0) {
+        for (int i = 0; i < ncol; ++i) raw_out[i] = 0.0;
+        return PG_OK;
+    }
+    return precision == PG_FP32 ? count_impl<float>(c, p, ncol, colors, seed, flags, raw_out)
+                                : count_impl<double>(c, p, ncol, colors, seed, flags, raw_out);
+}
+
+extern "C" int pg_count_multi(pg_ctx* c, int nplans, const pg_plan* const* plans, const int32_t* ncolorings,
+                              const uint64_t* seeds, int precision, uint32_t flags, double* raw_out)
+{
+    if (!c) return PG_ERR_INVALID;
+    if (nplans <= 0 || !plans || !ncolorings || !seeds || !raw_out) return set_err(c, PG_ERR_INVALID, "invalid multi-count arguments");
+    int64_t off = 0;
+    double total = 0;
+    for (int i = 0; i < nplans; ++i) {
+        int rc = pg_count(c, plans[i], ncolorings[i], nullptr, seeds[i], precision, flags, raw_out + off);
+        if (rc) return rc;
+        total += c->last_total_ms;
+        off += ncolorings[i];
+    }
+    c->last_total_ms = total;
+    return PG_OK;
+}
+
+extern "C" int pg_count_grad(pg_ctx* c, const pg_plan*, int, const uint8_t*, uint64_t, int, uint32_t, double*, void*, int)
+{
+    return set_err(c, PG_ERR_STATE, "pg_count_grad: not built yet");
+}
+
+extern "C" int pg_last_timing(const pg_ctx* c, double* total_ms, double* spmm_ms, int64_t* spmm_launches,
+                              double* spmm_bytes)
+{
+    if (!c) return PG_ERR_INVALID;
+    if (total_ms) *total_ms = c->last_total_ms;
+    if (spmm_ms) *spmm_ms = c->dom_ms;
+    if (spmm_launches) *spmm_launches = c->dom_launches;
+    if (spmm_bytes) *spmm_bytes = c->dom_bytes;
+    return PG_OK;
+}
+
+extern "C" int pg_last_timing_kernel(const pg_ctx* c, char* buf, int len)
+{
+    if (!c || !buf || len <= 0) return PG_ERR_INVALID;
+    snprintf(buf, (size_t)len, "%s", c->dom_name.c_str());
+    return PG_OK;
+}

@@ -1,0 +1,230 @@
+"""Parity of the HIP path (libpgbsc.so through its C ABI) with the oracle and
+with golden vectors of the compiled reference.
+
+Bar: FP32 tables within 1e-5 relative of the reference's raw count per
+coloring; FP64 tables exactly equal to the oracle's FP64 DP (integer counts
+below 2^53) and to brute force.
+"""
+import numpy as np
+import pytest
+
+import oracle
+import subgraph_amd as sg
+from conftest import golden_colors, golden_graph
+
+pytestmark = pytest.mark.gpu
+
+TOL32 = 1e-5
+
+TREE7 = [(0, 1), (1, 2), (2, 3), (1, 4), (4, 5), (4, 6)]
+
+
+def _rel(a, b):
+    return abs(a - b) / max(abs(b), 1e-300) if b != 0 else abs(a)
+
+
+def _simple_graph(rng, n, m):
+    s = rng.integers(0, n, m)
+    d = rng.integers(0, n, m)
+    keep = s != d
+    pairs = {(int(min(a, b)), int(max(a, b))) for a, b in zip(s[keep], d[keep])}
+    e = np.array(sorted(pairs), np.int32).reshape(-1, 2)
+    return e[:, 0].copy(), e[:, 1].copy()
+
+
+def _random_tree(rng, k):
+    e = [(int(rng.integers(0, j)), j) for j in range(1, k)]
+    perm = rng.permutation(k)
+    e = [(int(perm[a]), int(perm[b])) for a, b in e]
+    rng.shuffle(e)
+    return e
+
+
+def test_golden_vectors(golden, gpu_ctx):
+    """Every reference golden vector, both precisions (incl. s16/ef16 u3-1 and
+    s20/ef8 7-tree at full size)."""
+    worst = 0.0
+    for c in golden["counts"]:
+        t = golden["templates"][c["template"]]
+        n, colptr, rowidx = golden_graph(c["graph"])
+        print(f"[golden] {c['name']} n={n} nnz={len(rowidx)}", flush=True)
+        cols = golden_colors(c["colors"], n, t["nv"])
+        gpu_ctx.upload_graph(colptr, rowidx)
+        plan = gpu_ctx.plan(t["nv"], t["edges"])
+        got32 = gpu_ctx.count(plan, len(cols), colors=cols, precision="fp32")
+        want = np.array([float.fromhex(r) for r in c["raw"]])
+        for i in range(len(want)):
+            r = _rel(got32[i], want[i])
+            worst = max(worst, r)
+            assert r <= TOL32, (c["name"], i, got32[i], want[i], r)
+        if c["graph"]["kind"] == "edges" or c["graph"]["scale"] <= 12:
+            op = oracle.Plan(t["nv"], t["edges"])
+            got64 = gpu_ctx.count(plan, len(cols), colors=cols, precision="fp64")
+            for i in range(len(want)):
+                o64 = oracle.count_f64(op, colptr, rowidx, cols[i])
+                if abs(o64) < 2.0 ** 53:
+                    assert got64[i] == o64, (c["name"], i, got64[i], o64)
+                else:
+                    assert _rel(got64[i], o64) <= 1e-12
+    print(f"[golden] worst FP32 relative error vs reference: {worst:.3e}", flush=True)
+
+
+@pytest.mark.parametrize("hub", [None, 16])
+def test_random_graphs_and_trees(gpu_ctx, hub):
+    """Random simple graphs, random trees k=2..12; hub=16 forces the split
+    (partial-row) path for every vertex of degree > 16."""
+    rng = np.random.default_rng(1234 if hub is None else 99)
+    if hub is not None:
+        gpu_ctx.set_option("hub_degree", hub)
+        gpu_ctx.set_option("hub_chunk", 16)
+    try:
+        for trial in range(12):
+            n = int(rng.integers(30, 400))
+            s, d = _simple_graph(rng, n, int(rng.integers(2 * n, 10 * n)))
+            colptr, rowidx = sg.csc_from_edges(n, s, d)
+            gpu_ctx.upload_graph(colptr, rowidx)
+            k = int(rng.integers(2, 13))
+            te = _random_tree(rng, k)
+            print(f"[random] trial={trial} n={n} nnz={len(rowidx)} k={k} hub={hub}", flush=True)
+            plan = gpu_ctx.plan(k, te)
+            op = oracle.Plan(k, te)
+            cols = rng.integers(0, k, size=(3, n)).astype(np.uint8)
+            g32 = gpu_ctx.count(plan, 3, colors=cols, precision="fp32")
+            g64 = gpu_ctx.count(plan, 3, colors=cols, precision="fp64")
+            for i in range(3):
+                o32 = oracle.count_f32(op, colptr, rowidx, cols[i])
+                o64 = oracle.count_f64(op, colptr, rowidx, cols[i])
+                assert _rel(g32[i], o32) <= TOL32, (trial, i, g32[i], o32)
+                assert g64[i] == o64, (trial, i, g64[i], o64)
+    finally:
+        gpu_ctx.set_option("hub_degree", 2048)
+        gpu_ctx.set_option("hub_chunk", 1024)
+
+
+def test_fp64_equals_brute_force(gpu_ctx):
+    rng = np.random.default_rng(8)
+    trees = [(3, [(0, 1), (1, 2)]), (5, [(0, 1), (0, 2), (0, 3), (0, 4)]), (5, [(0, 1), (1, 2), (2, 3), (3, 4)]),
+             (6, [(0, 1), (1, 2), (1, 3), (3, 4), (3, 5)])]
+    for tv, te in trees:
+        n = 40
+        s, d = _simple_graph(rng, n, 160)
+        colptr, rowidx = sg.csc_from_edges(n, s, d)
+        gpu_ctx.upload_graph(colptr, rowidx)
+        plan = gpu_ctx.plan(tv, te)
+        cols = rng.integers(0, tv, size=(2, n)).astype(np.uint8)
+        got = gpu_ctx.count(plan, 2, colors=cols, precision="fp64")
+        for i in range(2):
+            assert got[i] == oracle.brute_force(tv, te, colptr, rowidx, cols[i])
+
+
+def test_batch_tiles_agree(gpu_ctx):
+    """One call with 37 colorings equals any tiling of them (and per-coloring calls)."""
+    n = 1 << 12
+    s, d = sg.rmat_edges(12, 8, seed=4)
+    colptr, rowidx = sg.csc_from_edges(n, s, d)
+    gpu_ctx.upload_graph(colptr, rowidx)
+    plan = gpu_ctx.plan(7, TREE7)
+    cols = sg.colors_fill(5, 0, 37, n, 7)
+    ref64 = None
+    for tile in (0, 1, 5, 16):
+        gpu_ctx.set_option("batch_tile", tile)
+        print(f"[tiles] batch_tile={tile}", flush=True)
+        g64 = gpu_ctx.count(plan, 37, colors=cols, precision="fp64")
+        g32 = gpu_ctx.count(plan, 37, colors=cols, precision="fp32")
+        if ref64 is None:
+            ref64, ref32 = g64, g32
+        assert np.array_equal(g64, ref64)
+        assert np.array_equal(g32, ref32)  # same per-coloring arithmetic whatever the tile
+    gpu_ctx.set_option("batch_tile", 0)
+    one = gpu_ctx.count(plan, 1, colors=cols[3:4], precision="fp32")
+    assert one[0] == ref32[3]
+
+
+def test_device_colors_equal_host_stream(gpu_ctx):
+    n = 1 << 11
+    s, d = sg.rmat_edges(11, 8, seed=9)
+    colptr, rowidx = sg.csc_from_edges(n, s, d)
+    gpu_ctx.upload_graph(colptr, rowidx)
+    plan = gpu_ctx.plan(5, [(0, 1), (1, 2), (2, 3), (3, 4)])
+    dev = gpu_ctx.count(plan, 6, colors=None, seed=77, precision="fp64")
+    host = gpu_ctx.count(plan, 6, colors=sg.colors_fill(77, 0, 6, n, 5), precision="fp64")
+    assert np.array_equal(dev, host)
+
+
+def test_two_vertex_template_quirk(gpu_ctx):
+    """u2: the reference reports 1e12 x the count (CountMat.cpp:811-824);
+    PG_FLAG_FIX_2VERTEX gives the count itself."""
+    colptr, rowidx = sg.csc_from_edges(4, np.array([0, 1, 2, 2], np.int32), np.array([1, 2, 0, 3], np.int32))
+    gpu_ctx.upload_graph(colptr, rowidx)
+    plan = gpu_ctx.plan(2, [(0, 1)])
+    cols = np.array([[0, 1, 0, 1]], np.uint8)
+    assert gpu_ctx.count(plan, 1, colors=cols, precision="fp32")[0] == 6.0e12
+    assert gpu_ctx.count(plan, 1, colors=cols, precision="fp64")[0] == 6.0e12
+    assert gpu_ctx.count(plan, 1, colors=cols, precision="fp32", fix_2vertex=True)[0] == 6.0
+    assert gpu_ctx.count(plan, 1, colors=cols, precision="fp64", fix_2vertex=True)[0] == 6.0
+
+
+def test_errors_are_reported(gpu_ctx):
+    colptr, rowidx = sg.csc_from_edges(4, np.array([0, 1, 2], np.int32), np.array([1, 2, 3], np.int32))
+    fresh = sg.Context(0)
+    plan = fresh.plan(3, [(0, 1), (1, 2)])
+    with pytest.raises(sg.PGError):
+        fresh.count(plan, 1)  # no graph yet
+    fresh.upload_graph(colptr, rowidx)
+    with pytest.raises(sg.PGError):
+        fresh.count(plan, 1, colors=np.array([[0, 1, 3, 0]], np.uint8))  # color >= k
+    with pytest.raises(sg.PGError):
+        fresh.upload_graph(np.array([0, 2, 1], np.int32), np.array([0, 1], np.int32))  # colptr not monotone
+    with pytest.raises(sg.PGError):
+        fresh.plan(3, [(0, 1), (0, 1)])  # not a tree
+    with pytest.raises(sg.PGError):
+        fresh.set_option("hub_degree", 1)
+    bad = sg.template_tables(3, [(0, 1), (1, 2)])
+    bad["split_aux"] = bad["split_aux"].copy()
+    bad["split_aux"][0] = 99
+    with pytest.raises(sg.PGError):
+        fresh.plan_from_tables(3, bad["sub_size"], bad["main_idx"], bad["aux_idx"], bad["comb_to_count"],
+                               bad["split_main"], bad["split_aux"])
+
+
+def test_isolated_vertices_and_empty_graph(gpu_ctx):
+    # vertices 5..9 isolated: same counts as without them
+    s = np.array([0, 1, 2, 3, 0], np.int32)
+    d = np.array([1, 2, 3, 4, 2], np.int32)
+    c5 = sg.csc_from_edges(5, s, d)
+    c10 = sg.csc_from_edges(10, s, d)
+    rng = np.random.default_rng(2)
+    cols = rng.integers(0, 3, 10).astype(np.uint8)
+    plan = gpu_ctx.plan(3, [(0, 1), (1, 2)])
+    gpu_ctx.upload_graph(*c5)
+    a = gpu_ctx.count(plan, 1, colors=cols[None, :5], precision="fp64")
+    gpu_ctx.upload_graph(*c10)
+    b = gpu_ctx.count(plan, 1, colors=cols[None, :], precision="fp64")
+    assert a[0] == b[0]
+    # a graph with no edges counts zero
+    gpu_ctx.upload_graph(np.zeros(8, np.int32), np.zeros(0, np.int32))
+    assert gpu_ctx.count(plan, 2, precision="fp32").tolist() == [0.0, 0.0]
+
+
+def test_full_size_properties_s20(gpu_ctx):
+    """s20/ef8 7-tree (the drop-in workload): color renaming leaves every count
+    unchanged (FP64, exact), FP32 within 1e-5 of FP64, hubs split or not."""
+    n = 1 << 20
+    s, d = sg.rmat_edges(20, 8, seed=1)
+    colptr, rowidx = sg.csc_from_edges(n, s, d)
+    gpu_ctx.upload_graph(colptr, rowidx)
+    plan = gpu_ctx.plan(7, TREE7)
+    cols = sg.colors_fill(3, 0, 2, n, 7)
+    perm = np.array([3, 6, 0, 1, 5, 2, 4], np.uint8)
+    g64 = gpu_ctx.count(plan, 2, colors=cols, precision="fp64")
+    p64 = gpu_ctx.count(plan, 2, colors=perm[cols], precision="fp64")
+    g32 = gpu_ctx.count(plan, 2, colors=cols, precision="fp32")
+    assert np.array_equal(g64, p64)
+    for i in range(2):
+        assert _rel(g32[i], g64[i]) <= TOL32
+    gpu_ctx.set_option("hub_degree", 1 << 30)
+    try:
+        nohub = gpu_ctx.count(plan, 2, colors=cols, precision="fp64")
+    finally:
+        gpu_ctx.set_option("hub_degree", 2048)
+    assert np.array_equal(nohub, g64)
