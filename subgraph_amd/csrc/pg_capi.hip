@@ -1,9 +1,11 @@
 // C ABI of libpgbsc.so: context, graph upload (relabelling and degree
-// buckets), plans, the per-pass DP schedule and its memory plan.
+// buckets), plans (color-relative index tables), the per-pass DP schedule and
+// its memory plan.
 //
 // Reference counterparts: CountMat::initialization/compute/colorCounting
 // (CountMat.cpp:21-322), DataTableColMajor table lifetimes
-// (DataTableColMajor.cpp:63-249), CSCGraph (CSCGraph.hpp:96-217).
+// (DataTableColMajor.cpp:63-249), CSCGraph (CSCGraph.hpp:96-217),
+// IndexSys (IndexSys.cpp:315-436) for the split order.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -58,13 +60,19 @@ struct LaunchRec {
     hipEvent_t beg, end;
 };
 
-// One non-leaf subtemplate as the device schedule sees it.
+// One non-leaf subtemplate as the device schedule sees it.  Widths are per
+// coloring in color-relative rows (K = k - 1):
+//   Wo = C(K, size-1) output values, Wy = C(K, aa) aux sums,
+//   Wa = C(K, aa-1) values per row of the aux child, Wm = C(K, mm-1) of the main child,
+//   P  = C(size-1, mm-1) non-zero splits per output set.
+// Full (reference) widths Cs, Ca, Cm are kept for the algorithmic byte count.
 struct DevSub {
     int s, size, mi, ai, mm, aa;
-    int Cs, Ca, Cm, P;
+    int Wo, Wy, Wa, Wm, P;
+    int Cs, Ca, Cm;
     bool root, scaled;
-    DevBuf split;  // [Cs][P] uint32 by output colex rank
-    DevBuf drop;   // [Cs][k] int16 (main is a leaf)
+    DevBuf split;  // [Wo][P] uint32 (main_rank << 16 | y_rank)
+    DevBuf ymask;  // [Wy] uint16
 };
 
 }  // namespace
@@ -72,29 +80,32 @@ struct DevSub {
 struct pg_plan {
     pg_ctx* ctx = nullptr;
     int device = 0;
-    int k = 0, nsub = 0, nonleaf = 0, scaled_sub = -1;
+    int k = 0, K = 0, nsub = 0, nonleaf = 0, scaled_sub = -1;
     std::vector<int> sizes, mains, auxs;
     std::vector<std::unique_ptr<DevSub>> subs;  // processing order (descending s)
+    DevBuf rank_tab;                            // [2^K] int16
 };
 
 struct pg_ctx {
-    int device = 0;
+    int device = 0, num_cus = 0;
     hipStream_t stream = nullptr;
     std::string err;
     // options
-    int64_t hub_degree = 2048, hub_chunk = 1024, batch_tile = 0, mem_budget_mb = 0, timing = 0, autotune = 0;
+    int64_t hub_degree = 256, hub_chunk = 256, batch_tile = 0, mem_budget_mb = 0, timing = 0, autotune = 0;
+    int64_t bucket_high = 32, bucket_mid = 8;
     // graph
     bool has_graph = false, weighted = false;
-    int32_t n = 0, n_act = 0, max_deg = 0, n_hub = 0;
+    int32_t n = 0, n_act = 0, max_deg = 0;
     int64_t nnz = 0;
     std::vector<int32_t> h_rowptr;
     DevBuf d_rowptr, d_col, d_ent, d_perm, d_w32, d_w64, d_wsrc;
-    // hub work items for the current hub_degree/hub_chunk
-    int64_t items_for_deg = -1, items_for_chunk = -1;
+    // degree buckets and hub work items for the current options
+    int64_t built_deg = -1, built_chunk = -1, built_high = -1, built_mid = -1;
+    int32_t n_hub = 0, hi_end = 0, mid_end = 0;  // [0,n_hub) hubs, [n_hub,hi_end) high, [hi_end,mid_end) mid
     int32_t n_hub_items = 0, n_slots = 0;
     DevBuf d_hub_items, d_slot_begin;
     // per-call workspace
-    DevBuf d_arena, d_colors_in, d_raw;
+    DevBuf d_arena, d_colors_in;
     // timing of the last call
     std::vector<LaunchRec> recs;
     double last_total_ms = 0;
@@ -117,39 +128,54 @@ int set_err(pg_ctx* ctx, int code, const char* fmt, ...)
     return code;
 }
 
-#define HIP_TRY(ctx, expr)                                                                          \
-    do {                                                                                            \
-        hipError_t e_ = (expr);                                                                     \
-        if (e_ != hipSuccess)                                                                       \
+#define HIP_TRY(ctx, expr)                                                                              \
+    do {                                                                                                \
+        hipError_t e_ = (expr);                                                                         \
+        if (e_ != hipSuccess)                                                                           \
             return set_err(ctx, e_ == hipErrorOutOfMemory ? PG_ERR_NOMEM : PG_ERR_HIP, "%s: %s (%s:%d)", \
-                           #expr, hipGetErrorString(e_), __FILE__, __LINE__);                      \
+                           #expr, hipGetErrorString(e_), __FILE__, __LINE__);                          \
     } while (0)
 
-int team_for(int cols)
+int pow2_at_least(int x)
 {
-    if (cols <= 8) return 8;
-    if (cols <= 16) return 16;
-    if (cols <= 32) return 32;
-    return 64;
+    int p = 1;
+    while (p < x) p <<= 1;
+    return p;
 }
 
-// (Re)build the hub chunk items for the current options.
-int build_hub_items(pg_ctx* c)
+// column lanes for `cols` value slots (1..64)
+int cl_for(int cols) { return std::min(64, pow2_at_least(std::max(cols, 1))); }
+// value slots of a neighbour-summing launch: kRegCols values of one coloring each
+int slots_for(int B, int W) { return B * ((W + kRegCols - 1) / kRegCols); }
+// lanes of a team that only streams rows (no neighbour loop)
+int team_for(int cols) { return std::max(8, cl_for(cols)); }
+
+// (Re)build degree buckets and hub chunk items for the current options.
+int build_buckets(pg_ctx* c)
 {
-    if (c->items_for_deg == c->hub_degree && c->items_for_chunk == c->hub_chunk) return PG_OK;
+    if (c->built_deg == c->hub_degree && c->built_chunk == c->hub_chunk && c->built_high == c->bucket_high &&
+        c->built_mid == c->bucket_mid)
+        return PG_OK;
+    auto deg = [&](int32_t v) { return (int64_t)c->h_rowptr[v + 1] - c->h_rowptr[v]; };
     int32_t nh = 0;
-    while (nh < c->n_act && c->h_rowptr[nh + 1] - c->h_rowptr[nh] > c->hub_degree) ++nh;
+    while (nh < c->n_act && deg(nh) > c->hub_degree) ++nh;
+    int32_t hi = nh;
+    while (hi < c->n_act && deg(hi) >= c->bucket_high) ++hi;
+    int32_t mid = hi;
+    while (mid < c->n_act && deg(mid) >= c->bucket_mid) ++mid;
     std::vector<int4> items;
     std::vector<int32_t> slot_begin(nh + 1, 0);
     for (int32_t h = 0; h < nh; ++h) {
         slot_begin[h] = (int32_t)items.size();
         for (int32_t e = c->h_rowptr[h]; e < c->h_rowptr[h + 1]; e += (int32_t)c->hub_chunk) {
-            int32_t e1 = std::min<int64_t>(e + c->hub_chunk, c->h_rowptr[h + 1]);
+            int32_t e1 = (int32_t)std::min<int64_t>((int64_t)e + c->hub_chunk, c->h_rowptr[h + 1]);
             items.push_back(make_int4(h, e, e1, (int)items.size()));
         }
     }
     slot_begin[nh] = (int32_t)items.size();
     c->n_hub = nh;
+    c->hi_end = hi;
+    c->mid_end = mid;
     c->n_hub_items = (int32_t)items.size();
     c->n_slots = (int32_t)items.size();
     HIP_TRY(c, c->d_hub_items.alloc(std::max<size_t>(items.size(), 1) * sizeof(int4)));
@@ -158,54 +184,50 @@ int build_hub_items(pg_ctx* c)
         HIP_TRY(c, hipMemcpy(c->d_hub_items.p, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice));
     HIP_TRY(c, hipMemcpy(c->d_slot_begin.p, slot_begin.data(), slot_begin.size() * sizeof(int32_t),
                          hipMemcpyHostToDevice));
-    c->items_for_deg = c->hub_degree;
-    c->items_for_chunk = c->hub_chunk;
+    c->built_deg = c->hub_degree;
+    c->built_chunk = c->hub_chunk;
+    c->built_high = c->bucket_high;
+    c->built_mid = c->bucket_mid;
     return PG_OK;
 }
 
 // ---- memory plan for one pass: first-fit offsets over table lifetimes ----
 struct MemPlan {
     size_t total = 0;
-    size_t off_H = 0, off_part = 0, off_root = 0, off_colors = 0, off_rawdev = 0;
+    size_t off_H = 0, off_part = 0, off_root = 0, off_colors = 0, off_ecol = 0, off_rawdev = 0, off_tmp = 0;
     std::vector<size_t> off_tab;  // per subtemplate index (tables of non-root non-leaf)
 };
 
 size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
-
-int64_t blocks_for(int64_t items, int team) { return (items + (kBlock / team) - 1) / (kBlock / team); }
 
 MemPlan plan_memory(const pg_ctx* c, const pg_plan* p, int B, size_t tsz)
 {
     MemPlan m;
     const int S = p->nsub;
     m.off_tab.assign(S, 0);
-    // live intervals: [start step, end step] in processing order
     struct Iv { size_t bytes; int beg, end; size_t* off; };
     std::vector<Iv> ivs;
     const size_t na = (size_t)c->n_act;
-    int nsteps = (int)p->subs.size();
-    std::vector<int> step_of(S, -1);
-    for (int i = 0; i < nsteps; ++i) step_of[p->subs[i]->s] = i;
-    // H lives for the whole pass (step -1 .. last)
+    const int nsteps = (int)p->subs.size();
+    // pass-long buffers: H, hub partial rows, root values, colors, results
     ivs.push_back({align_up(na * B * p->k * tsz, 256), -1, nsteps, &m.off_H});
-    // hub partial rows
-    int maxCa = p->k;
-    for (auto& d : p->subs) if (d->aa > 1) maxCa = std::max(maxCa, d->Ca);
-    ivs.push_back({align_up((size_t)std::max(c->n_slots, 1) * B * maxCa * tsz, 256), -1, nsteps, &m.off_part});
-    // per-vertex root values [B][n_act]
+    int maxY = p->k;
+    for (auto& d : p->subs) if (d->aa > 1) maxY = std::max(maxY, d->Wy);
+    ivs.push_back({align_up((size_t)std::max(c->n_slots, 1) * B * maxY * tsz, 256), -1, nsteps, &m.off_part});
     ivs.push_back({align_up(na * B * sizeof(double), 256), -1, nsteps, &m.off_root});
     ivs.push_back({align_up(na * B, 256), -1, nsteps, &m.off_colors});
+    ivs.push_back({align_up((size_t)std::max<int64_t>(c->nnz, 1) * B, 256), -1, nsteps, &m.off_ecol});
     ivs.push_back({align_up((size_t)B * sizeof(double), 256), -1, nsteps, &m.off_rawdev});
+    ivs.push_back({align_up((size_t)B * kReduceBlocks * sizeof(double), 256), -1, nsteps, &m.off_tmp});
+    // table of step i lives until its parent's step consumes it
     for (int i = 0; i < nsteps; ++i) {
         const DevSub* d = p->subs[i].get();
         if (d->root) continue;
-        // consumed by the parent step
-        int parent_step = -1;
+        int parent_step = nsteps;
         for (int j = i + 1; j < nsteps; ++j)
             if (p->subs[j]->mi == d->s || p->subs[j]->ai == d->s) { parent_step = j; break; }
-        ivs.push_back({align_up(na * B * d->Cs * tsz, 256), i, parent_step, &m.off_tab[d->s]});
+        ivs.push_back({align_up(na * B * d->Wo * tsz, 256), i, parent_step, &m.off_tab[d->s]});
     }
-    // first-fit in order of start step
     struct Placed { size_t off, bytes; int beg, end; };
     std::vector<Placed> placed;
     for (auto& iv : ivs) {
@@ -225,20 +247,15 @@ MemPlan plan_memory(const pg_ctx* c, const pg_plan* p, int B, size_t tsz)
     return m;
 }
 
-size_t lds_bytes(int team, int B, int Ca, int Cm_lds, size_t tsz)
+// LDS the largest launch of a sub needs for B colorings: the one with the
+// smallest team (most teams per workgroup)
+size_t sub_lds_max(const pg_plan* p, const DevSub& d, int B, size_t tsz)
 {
-    const int teams = kBlock / team;
-    return (size_t)teams * B * (Ca + Cm_lds) * tsz;
-}
-
-size_t max_lds(const pg_plan* p, int B, size_t tsz)
-{
-    size_t mx = 0;
-    for (auto& d : p->subs) {
-        int team = d->aa > 1 ? team_for(B * d->Ca) : team_for(std::max(B * d->Ca, B * d->Cs));
-        mx = std::max(mx, lds_bytes(team, B, d->Ca, d->mm > 1 ? d->Cm : 0, tsz));
-    }
-    return mx;
+    const size_t rank = d.aa > 1 ? ((((size_t)2 << p->K) + 15) & ~(size_t)15) : 0;
+    const int team = d.aa > 1 ? std::max(8, cl_for(slots_for(B, d.Wy))) : team_for(std::max(B * d.Wy, B * d.Wo));
+    const int team_fin = team_for(std::max(B * d.Wy, B * d.Wo));  // hub rows from partials
+    const int teams = kBlock / std::min(team, team_fin);
+    return rank + (size_t)teams * B * (d.Wy + (d.mm > 1 ? d.Wm : 0)) * tsz;
 }
 
 constexpr size_t kLdsLimit = 64 * 1024;
@@ -265,6 +282,64 @@ std::string kname(int team, int aux, int main_mode)
     return buf;
 }
 
+// The neighbour-summing launches of one subtemplate (or of H): explicit hub
+// chunks first (partial rows), then the degree buckets of the remaining
+// vertices with team shapes fitted to their degree, then the hub rows from
+// their partials.  `a` carries everything but the item range and team shape.
+// The step's algorithmic bytes are shared among its launches by the work
+// each covers (entries + vertices), so per-kernel sums stay exact.
+template <typename T>
+int launch_gathering(pg_ctx* c, int aux, int main_mode, SubArgs<T> a, T* part, int y_cols, double bytes)
+{
+    const int cl = cl_for(slots_for(a.B, aux == AUX_GATHER ? a.Wy : a.k));
+    const hipStream_t st = c->stream;
+    const int cus = c->num_cus;
+    const double units = (double)c->nnz + (double)c->n_act;
+    auto share = [&](double ents, double verts) { return units > 0 ? bytes * (ents + verts) / units : 0.0; };
+    if (c->n_hub > 0) {
+        SubArgs<T> h = a;
+        h.out = part;
+        h.ldo = y_cols;
+        h.out_mode = OUT_PARTIAL;
+        h.items = c->d_hub_items.as<int4>();
+        h.n_items = c->n_hub_items;
+        h.item_base = 0;
+        h.ng = 64 / cl;
+        record(c, kname<T>(64, aux, main_mode), share(c->h_rowptr[c->n_hub], 0), true);
+        HIP_TRY(c, launch_sub<T>(64, aux, main_mode, h, cus, st));
+        record(c, "", 0, false);
+    }
+    struct Bucket { int32_t lo, hi; int lanes; };
+    const Bucket buckets[3] = {{c->n_hub, c->hi_end, 64}, {c->hi_end, c->mid_end, 16}, {c->mid_end, c->n_act, 8}};
+    for (const Bucket& bk : buckets) {
+        if (bk.hi <= bk.lo) continue;
+        const int team = std::max(bk.lanes, cl);
+        SubArgs<T> r = a;
+        r.items = nullptr;
+        r.item_base = bk.lo;
+        r.n_items = bk.hi - bk.lo;
+        r.ng = team / cl;
+        record(c, kname<T>(team, aux, main_mode),
+               share((double)c->h_rowptr[bk.hi] - c->h_rowptr[bk.lo], bk.hi - bk.lo), true);
+        HIP_TRY(c, launch_sub<T>(team, aux, main_mode, r, cus, st));
+        record(c, "", 0, false);
+    }
+    if (c->n_hub > 0) {
+        SubArgs<T> f = a;
+        f.aux = part;
+        f.lda = y_cols;
+        f.items = nullptr;
+        f.item_base = 0;
+        f.n_items = c->n_hub;
+        f.ng = 1;
+        const int team = team_for(std::max(y_cols, a.B * a.Wo));
+        record(c, kname<T>(team, AUX_PARTIALS, main_mode), share(0, c->n_hub), true);
+        HIP_TRY(c, launch_sub<T>(team, AUX_PARTIALS, main_mode, f, cus, st));
+        record(c, "", 0, false);
+    }
+    return PG_OK;
+}
+
 // Runs one pass of B colorings (first..first+B-1) through the whole DP.
 template <typename T>
 int run_pass(pg_ctx* c, const pg_plan* p, int B, int64_t first, const uint8_t* d_colors_in, uint64_t seed,
@@ -274,12 +349,14 @@ int run_pass(pg_ctx* c, const pg_plan* p, int B, int64_t first, const uint8_t* d
     uint8_t* colors = (uint8_t*)(base + mp.off_colors);
     T* H = (T*)(base + mp.off_H);
     T* part = (T*)(base + mp.off_part);
-    double* rootp = (double*)(base + mp.off_root);
+    double* rootv = (double*)(base + mp.off_root);
     double* rawd = (double*)(base + mp.off_rawdev);
+    double* tmp = (double*)(base + mp.off_tmp);
     const T* w = nullptr;
     if (c->weighted) w = sizeof(T) == 4 ? (const T*)c->d_w32.p : (const T*)c->d_w64.p;
     const hipStream_t st = c->stream;
     const int k = p->k;
+    // survey byte model: CSC streamed once per gathering launch, full-width tables
     const double nb_csc = 4.0 * (c->n_act + 1) + 4.0 * c->nnz * (c->weighted ? 2 : 1);
     const double na = (double)c->n_act;
 
@@ -288,124 +365,81 @@ int run_pass(pg_ctx* c, const pg_plan* p, int B, int64_t first, const uint8_t* d
                                         c->n_act, B, c->n, st));
     else
         HIP_TRY(c, launch_colors_rng(colors, c->d_perm.as<int32_t>(), c->n_act, B, seed, first, k, st));
+    uint8_t* ecol = (uint8_t*)(base + mp.off_ecol);
+    HIP_TRY(c, launch_edge_colors(ecol, c->d_col.as<int32_t>(), colors, c->nnz, B, st));
 
     SubArgs<T> base_args{};
     base_args.rowptr = c->d_rowptr.as<int32_t>();
     base_args.col = c->d_col.as<int32_t>();
     base_args.w = w;
     base_args.colors = colors;
+    base_args.ecol = ecol;
     base_args.B = B;
     base_args.k = k;
+    base_args.K = p->K;
+    base_args.ng = 1;
+    base_args.rank_tab = p->rank_tab.as<int16_t>();
     base_args.slot_begin = c->d_slot_begin.as<int32_t>();
     base_args.scale = 1.0e-12f;
 
     // ---- neighbour-color histogram H = A * leaf (shared by every leaf aux)
     bool need_H = false;
-    for (auto& d : p->subs) need_H |= d->aa == 1;
+    for (auto& d : p->subs) need_H |= d.get()->aa == 1;
     if (need_H) {
         SubArgs<T> a = base_args;
-        a.Ca = k;
+        a.Wy = k;
+        a.Wo = k;
         a.out = H;
         a.ldo = (int64_t)B * k;
         a.out_mode = OUT_Y;
-        const int team = team_for(B * k);
-        a.n_items = c->n_act - c->n_hub;
-        a.item_base = c->n_hub;
-        record(c, kname<T>(team, AUX_COLORS, MAIN_LEAF), nb_csc + na * B * 1.0, true);
-        HIP_TRY(c, launch_sub<T>(team, AUX_COLORS, MAIN_LEAF, a, (int)blocks_for(a.n_items, team), 0, st));
-        record(c, "", 0, false);
-        if (c->n_hub > 0) {
-            SubArgs<T> h = base_args;
-            h.Ca = k;
-            h.out = part;
-            h.ldo = (int64_t)B * k;
-            h.out_mode = OUT_PARTIAL;
-            h.items = c->d_hub_items.as<int4>();
-            h.n_items = c->n_hub_items;
-            HIP_TRY(c, launch_sub<T>(team, AUX_COLORS, MAIN_LEAF, h, (int)blocks_for(h.n_items, team), 0, st));
-            SubArgs<T> f = base_args;
-            f.Ca = k;
-            f.aux = part;
-            f.lda = (int64_t)B * k;
-            f.out = H;
-            f.ldo = (int64_t)B * k;
-            f.out_mode = OUT_Y;
-            f.n_items = c->n_hub;
-            f.item_base = 0;
-            HIP_TRY(c, launch_sub<T>(team, AUX_PARTIALS, MAIN_LEAF, f, (int)blocks_for(f.n_items, team), 0, st));
-        }
+        int rc = launch_gathering<T>(c, AUX_COLORS, MAIN_LEAF, a, part, B * k, nb_csc + na * B);
+        if (rc) return rc;
     }
 
     // ---- subtemplates, children before parents
     for (auto& dp : p->subs) {
         const DevSub& d = *dp;
         SubArgs<T> a = base_args;
-        a.Ca = d.Ca;
-        a.Cm = d.Cm;
-        a.Cs = d.Cs;
+        a.Wy = d.Wy;
+        a.Wm = d.Wm;
+        a.Wo = d.Wo;
         a.P = d.P;
         a.split = d.split.as<uint32_t>();
-        a.drop = d.drop.as<int16_t>();
+        a.ymask = d.ymask.as<uint16_t>();
         const int main_mode = d.mm == 1 ? MAIN_LEAF : MAIN_TABLE;
         if (d.mm > 1) {
             a.main = (const T*)(base + mp.off_tab[d.mi]);
-            a.ldm = (int64_t)B * d.Cm;
+            a.ldm = (int64_t)B * d.Wm;
         }
         if (d.root) {
             a.out_mode = OUT_ROOT;
-            a.root_vals = rootp;
+            a.root_vals = rootv;
             a.n_root = c->n_act;
         } else {
             a.out = (T*)(base + mp.off_tab[d.s]);
-            a.ldo = (int64_t)B * d.Cs;
-            const bool scaled = d.scaled && sizeof(T) == 4;
-            a.out_mode = scaled ? OUT_EMA_SCALED : OUT_EMA;
+            a.ldo = (int64_t)B * d.Wo;
+            a.out_mode = (d.scaled && sizeof(T) == 4) ? OUT_EMA_SCALED : OUT_EMA;
         }
-        const double tab_bytes = 4.0 * na * B * ((d.aa > 1 ? d.Ca : 0) + (d.mm > 1 ? d.Cm : 0) + (d.root ? 0 : d.Cs));
-        const int Cm_lds = d.mm > 1 ? d.Cm : 0;
+        const double tab_bytes =
+            4.0 * na * B * ((d.aa > 1 ? d.Ca : 0) + (d.mm > 1 ? d.Cm : 0) + (d.root ? 0 : d.Cs));
         if (d.aa == 1) {
-            const int team = team_for(std::max(B * d.Ca, B * d.Cs));
+            const int team = team_for(std::max(B * d.Wy, B * d.Wo));
             a.aux = H;
             a.lda = (int64_t)B * k;
+            a.Wa = k;
             a.n_items = c->n_act;
             a.item_base = 0;
-            const size_t lds = lds_bytes(team, B, d.Ca, Cm_lds, sizeof(T));
             record(c, kname<T>(team, AUX_READ, main_mode), tab_bytes, true);
-            HIP_TRY(c, launch_sub<T>(team, AUX_READ, main_mode, a, (int)blocks_for(a.n_items, team), lds, st));
+            HIP_TRY(c, launch_sub<T>(team, AUX_READ, main_mode, a, c->num_cus, st));
             record(c, "", 0, false);
         } else {
-            const int team = team_for(B * d.Ca);
-            const size_t lds = lds_bytes(team, B, d.Ca, Cm_lds, sizeof(T));
-            const T* Ta = (const T*)(base + mp.off_tab[d.ai]);
-            // hub chunks first: partial rows
-            if (c->n_hub > 0) {
-                SubArgs<T> h = a;
-                h.aux = Ta;
-                h.lda = (int64_t)B * d.Ca;
-                h.out = part;
-                h.ldo = (int64_t)B * d.Ca;
-                h.out_mode = OUT_PARTIAL;
-                h.items = c->d_hub_items.as<int4>();
-                h.n_items = c->n_hub_items;
-                HIP_TRY(c, launch_sub<T>(team, AUX_GATHER, main_mode, h, (int)blocks_for(h.n_items, team), 0, st));
-            }
-            a.aux = Ta;
-            a.lda = (int64_t)B * d.Ca;
-            a.n_items = c->n_act - c->n_hub;
-            a.item_base = c->n_hub;
-            record(c, kname<T>(team, AUX_GATHER, main_mode), nb_csc + tab_bytes, true);
-            HIP_TRY(c, launch_sub<T>(team, AUX_GATHER, main_mode, a, (int)blocks_for(a.n_items, team), lds, st));
-            record(c, "", 0, false);
-            if (c->n_hub > 0) {
-                SubArgs<T> f = a;
-                f.aux = part;
-                f.lda = (int64_t)B * d.Ca;
-                f.n_items = c->n_hub;
-                f.item_base = 0;
-                HIP_TRY(c, launch_sub<T>(team, AUX_PARTIALS, main_mode, f, (int)blocks_for(f.n_items, team), lds, st));
-            }
+            a.aux = (const T*)(base + mp.off_tab[d.ai]);
+            a.Wa = d.Wa;
+            a.lda = (int64_t)B * d.Wa;
+            int rc = launch_gathering<T>(c, AUX_GATHER, main_mode, a, part, B * d.Wy, nb_csc + tab_bytes);
+            if (rc) return rc;
         }
-        if (d.root) HIP_TRY(c, launch_root_reduce(rootp, c->n_act, B, rawd, st));
+        if (d.root) HIP_TRY(c, launch_root_reduce(rootv, c->n_act, B, tmp, rawd, st));
     }
     std::vector<double> raw(B);
     HIP_TRY(c, hipMemcpyAsync(raw.data(), rawd, sizeof(double) * B, hipMemcpyDeviceToHost, st));
@@ -434,18 +468,25 @@ int choose_tile(pg_ctx* c, const pg_plan* p, int ncol, size_t tsz, int* tile, Me
         HIP_TRY(c, hipMemGetInfo(&fr, &tot));
         budget = (size_t)(0.9 * (double)(fr + c->d_arena.bytes));
     }
-    int cap = c->batch_tile > 0 ? (int)std::min<int64_t>(c->batch_tile, ncol) : std::min(ncol, 64);
+    const int cap = c->batch_tile > 0 ? (int)std::min<int64_t>(c->batch_tile, ncol) : std::min(ncol, 64);
     for (int B = cap; B >= 1; --B) {
-        if (max_lds(p, B, tsz) > kLdsLimit) continue;
+        size_t lds = 0;
+        for (auto& d : p->subs) lds = std::max(lds, sub_lds_max(p, *d, B, tsz));
+        if (lds > kLdsLimit) {
+            if (c->batch_tile > 0) break;
+            continue;
+        }
         MemPlan m = plan_memory(c, p, B, tsz);
-        if (m.total > budget) continue;
-        if (c->batch_tile > 0 && B != cap) break;
+        if (m.total > budget) {
+            if (c->batch_tile > 0) break;
+            continue;
+        }
         *tile = B;
         *mp = m;
         return PG_OK;
     }
-    return set_err(c, PG_ERR_NOMEM, "tables of one coloring do not fit the device memory budget (%.1f GB)",
-                   budget / 1e9);
+    return set_err(c, PG_ERR_NOMEM, "no batch tile fits (LDS %zu KB per block, device budget %.1f GB)",
+                   kLdsLimit >> 10, budget / 1e9);
 }
 
 void finish_timing(pg_ctx* c)
@@ -480,7 +521,7 @@ template <typename T>
 int count_impl(pg_ctx* c, const pg_plan* p, int ncol, const uint8_t* colors, uint64_t seed, uint32_t flags,
                double* raw_out)
 {
-    int rc = build_hub_items(c);
+    int rc = build_buckets(c);
     if (rc) return rc;
     int tile = 1;
     MemPlan mp;
@@ -498,8 +539,8 @@ int count_impl(pg_ctx* c, const pg_plan* p, int ncol, const uint8_t* colors, uin
     HIP_TRY(c, hipEventCreate(&t1));
     HIP_TRY(c, hipEventRecord(t0, c->stream));
     for (int first = 0; first < ncol; first += tile) {
-        int B = std::min(tile, ncol - first);
-        MemPlan m = B == tile ? mp : plan_memory(c, p, B, sizeof(T));
+        const int B = std::min(tile, ncol - first);
+        const MemPlan m = B == tile ? mp : plan_memory(c, p, B, sizeof(T));
         rc = run_pass<T>(c, p, B, first, d_in, seed, m, flags, raw_out + first);
         if (rc) break;
     }
@@ -512,6 +553,24 @@ int count_impl(pg_ctx* c, const pg_plan* p, int ncol, const uint8_t* colors, uin
     (void)hipEventDestroy(t1);
     finish_timing(c);
     return rc;
+}
+
+// colex rank of every K-bit mask among the masks of its popcount: colex
+// order of sets is the numeric order of their bit masks
+std::vector<int16_t> colex_rank_table(int K)
+{
+    std::vector<int16_t> rank((size_t)1 << K);
+    std::vector<int> next(K + 1, 0);
+    for (uint32_t m = 0; m < (1u << K); ++m) rank[m] = (int16_t)next[__builtin_popcount(m)]++;
+    return rank;
+}
+
+std::vector<uint16_t> masks_of_size(int K, int size)
+{
+    std::vector<uint16_t> out;
+    for (uint32_t m = 0; m < (1u << K); ++m)
+        if (__builtin_popcount(m) == size) out.push_back((uint16_t)m);
+    return out;
 }
 
 }  // namespace
@@ -531,6 +590,9 @@ extern "C" int pg_create(pg_ctx** out, int device)
     pg_ctx* c = new (std::nothrow) pg_ctx();
     if (!c) return PG_ERR_NOMEM;
     c->device = device;
+    if (hipDeviceGetAttribute(&c->num_cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
+        c->num_cus <= 0)
+        c->num_cus = 256;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;
         return PG_ERR_HIP;
@@ -561,6 +623,12 @@ extern "C" int pg_set_option(pg_ctx* c, const char* key, int64_t v)
     } else if (k == "hub_chunk") {
         if (v < 16 || v > (1 << 24)) return set_err(c, PG_ERR_INVALID, "hub_chunk must be in [16, 2^24]");
         c->hub_chunk = v;
+    } else if (k == "bucket_high") {
+        if (v < 1) return set_err(c, PG_ERR_INVALID, "bucket_high must be >= 1");
+        c->bucket_high = v;
+    } else if (k == "bucket_mid") {
+        if (v < 1) return set_err(c, PG_ERR_INVALID, "bucket_mid must be >= 1");
+        c->bucket_mid = v;
     } else if (k == "batch_tile") {
         if (v < 0 || v > 4096) return set_err(c, PG_ERR_INVALID, "batch_tile must be in [0, 4096]");
         c->batch_tile = v;
@@ -583,6 +651,8 @@ extern "C" int pg_get_option(const pg_ctx* c, const char* key, int64_t* v)
     std::string k(key);
     if (k == "hub_degree") *v = c->hub_degree;
     else if (k == "hub_chunk") *v = c->hub_chunk;
+    else if (k == "bucket_high") *v = c->bucket_high;
+    else if (k == "bucket_mid") *v = c->bucket_mid;
     else if (k == "batch_tile") *v = c->batch_tile;
     else if (k == "mem_budget_mb") *v = c->mem_budget_mb;
     else if (k == "timing") *v = c->timing;
@@ -592,10 +662,11 @@ extern "C" int pg_get_option(const pg_ctx* c, const char* key, int64_t* v)
 }
 
 // Internal order: vertices that appear in any entry, by descending row
-// degree (ties by caller id), so hubs form a prefix and isolated vertices --
-// whose tables are all zero above the leaves -- are dropped.  Row v lists the
-// entries (row v, column u) in ascending u, the order in which the reference
-// accumulates row v (CSCGraph.hpp:199-213, 254-307).
+// degree (ties by caller id), so hubs form a prefix, degree buckets are
+// ranges, and isolated vertices -- whose tables are all zero above the
+// leaves -- are dropped.  Row v lists the entries (row v, column u) in
+// ascending u, the order in which the reference accumulates row v
+// (CSCGraph.hpp:199-213, 254-307).
 extern "C" int pg_graph_upload(pg_ctx* c, int32_t n, int64_t nnz, const int32_t* colptr, const int32_t* rowidx,
                                const float* vals)
 {
@@ -612,14 +683,16 @@ extern "C" int pg_graph_upload(pg_ctx* c, int32_t n, int64_t nnz, const int32_t*
         if (colptr[u + 1] > colptr[u]) seen[u] = 1;
         for (int64_t e = colptr[u]; e < colptr[u + 1]; ++e) {
             int32_t v = rowidx[e];
-            if (v < 0 || v >= n) return set_err(c, PG_ERR_INVALID, "row index %d out of range at entry %lld", v, (long long)e);
+            if (v < 0 || v >= n)
+                return set_err(c, PG_ERR_INVALID, "row index %d out of range at entry %lld", v, (long long)e);
             rdeg[v]++;
             seen[v] = 1;
         }
     }
     std::vector<int32_t> perm;
     perm.reserve(n);
-    for (int32_t v = 0; v < n; ++v) if (seen[v]) perm.push_back(v);
+    for (int32_t v = 0; v < n; ++v)
+        if (seen[v]) perm.push_back(v);
     std::stable_sort(perm.begin(), perm.end(), [&](int32_t a, int32_t b) { return rdeg[a] > rdeg[b]; });
     const int32_t na = (int32_t)perm.size();
     std::vector<int32_t> inv(n, -1);
@@ -665,10 +738,10 @@ extern "C" int pg_graph_upload(pg_ctx* c, int32_t n, int64_t nnz, const int32_t*
         HIP_TRY(c, hipMemcpy(c->d_ent.p, hent.data(), hent.size() * 4, hipMemcpyHostToDevice));
     }
     if (na > 0) HIP_TRY(c, hipMemcpy(c->d_perm.p, perm.data(), perm.size() * 4, hipMemcpyHostToDevice));
-    c->items_for_deg = c->items_for_chunk = -1;
+    c->built_deg = -1;
     c->weighted = false;
     c->has_graph = true;
-    int rc = build_hub_items(c);
+    int rc = build_buckets(c);
     if (rc) return rc;
     if (vals) return pg_graph_set_weights(c, vals, 0);
     return PG_OK;
@@ -709,49 +782,112 @@ extern "C" int pg_graph_stats(const pg_ctx* c, int64_t* n, int64_t* nnz, int64_t
     return PG_OK;
 }
 
+// The reference tables (IndexSys) fix the column of every color set (colex
+// rank) and the order of the splits (lexicographic position subsets, aux =
+// complement).  The kernels rely on exactly that convention, re-expressed
+// per vertex color: the plan checks the caller's tables against it, then
+// builds the color-relative tables.  Restricting the reference's split list
+// of a set S to the splits whose main part holds col(v), and dropping col(v),
+// leaves the lexicographic position-subset order of S \ {col(v)}: removing a
+// common element preserves lexicographic order.
 extern "C" int pg_plan_create(pg_ctx* c, int k, int nsub, const int32_t* sub_size, const int32_t* main_idx,
                               const int32_t* aux_idx, const int32_t* comb_to_count, const int32_t* split_main,
                               const int32_t* split_aux, pg_plan** out)
 {
     if (!c || !out) return PG_ERR_INVALID;
     *out = nullptr;
-    if (k < 2 || k > kMaxTemplate || nsub != 2 * k - 1 || !sub_size || !main_idx || !aux_idx || !comb_to_count ||
+    if (k < 2 || k > kMaxK || nsub != 2 * k - 1 || !sub_size || !main_idx || !aux_idx || !comb_to_count ||
         !split_main || !split_aux)
-        return set_err(c, PG_ERR_INVALID, "plan needs 2 <= k <= %d and nsub = 2k-1", kMaxTemplate);
+        return set_err(c, PG_ERR_INVALID, "plan needs 2 <= k <= %d and nsub = 2k-1", kMaxK);
     if (sub_size[0] != k) return set_err(c, PG_ERR_INVALID, "subtemplate 0 must be the whole template");
     int nonleaf = 0;
     for (int s = 0; s < nsub; ++s) {
-        int sz = sub_size[s];
+        const int sz = sub_size[s];
         if (sz < 1 || sz > k) return set_err(c, PG_ERR_INVALID, "subtemplate %d has size %d", s, sz);
         if (sz == 1) continue;
         ++nonleaf;
-        int mi = main_idx[s], ai = aux_idx[s];
+        const int mi = main_idx[s], ai = aux_idx[s];
         if (mi <= s || ai <= s || mi >= nsub || ai >= nsub || mi == ai)
             return set_err(c, PG_ERR_INVALID, "subtemplate %d children must follow it", s);
         if (sub_size[mi] + sub_size[ai] != sz)
             return set_err(c, PG_ERR_INVALID, "subtemplate %d: child sizes do not add up", s);
+    }
+    // the caller's index tables must be the IndexSys convention
+    {
+        std::vector<int64_t> c2c_off(nsub + 1, 0), split_off(nsub + 1, 0);
+        for (int s = 0; s < nsub; ++s) {
+            c2c_off[s + 1] = c2c_off[s] + IndexSys::comb_calc(k, sub_size[s]);
+            int64_t ns = 0;
+            if (sub_size[s] > 1)
+                ns = (int64_t)IndexSys::comb_calc(k, sub_size[s]) *
+                     IndexSys::comb_calc(sub_size[s], sub_size[main_idx[s]]);
+            split_off[s + 1] = split_off[s] + ns;
+        }
+        for (int s = 0; s < nsub; ++s) {
+            const int sz = sub_size[s];
+            const int nc = IndexSys::comb_calc(k, sz);
+            // i-th lexicographic set of size sz
+            std::vector<int> set(sz);
+            std::iota(set.begin(), set.end(), 0);
+            const int mm = sz > 1 ? sub_size[main_idx[s]] : 0;
+            const int P = sz > 1 ? IndexSys::comb_calc(sz, mm) : 0;
+            for (int i = 0; i < nc; ++i) {
+                uint32_t mask = 0;
+                for (int x : set) mask |= 1u << x;
+                int rank = 0;
+                for (int q = 0; q < sz; ++q) rank += IndexSys::comb_calc(set[q], q + 1);
+                if (comb_to_count[c2c_off[s] + i] != rank)
+                    return set_err(c, PG_ERR_INVALID, "comb_to_count of subtemplate %d is not IndexSys's", s);
+                if (sz > 1) {
+                    std::vector<int> pos(mm);
+                    std::iota(pos.begin(), pos.end(), 0);
+                    for (int j = 0; j < P; ++j) {
+                        uint32_t mmask = 0;
+                        for (int q : pos) mmask |= 1u << set[q];
+                        const uint32_t amask = mask & ~mmask;
+                        int rm = 0, ra = 0, qm = 0, qa = 0;
+                        for (int bit = 0; bit < k; ++bit) {
+                            if (mmask >> bit & 1u) rm += IndexSys::comb_calc(bit, ++qm);
+                            if (amask >> bit & 1u) ra += IndexSys::comb_calc(bit, ++qa);
+                        }
+                        const int64_t at = split_off[s] + (int64_t)i * P + j;
+                        if (split_main[at] != rm || split_aux[at] != ra)
+                            return set_err(c, PG_ERR_INVALID, "split tables of subtemplate %d are not IndexSys's", s);
+                        int q = mm - 1;
+                        while (q >= 0 && pos[q] == sz - mm + q) --q;
+                        if (q < 0) break;
+                        ++pos[q];
+                        for (int r = q + 1; r < mm; ++r) pos[r] = pos[r - 1] + 1;
+                    }
+                }
+                int q = sz - 1;
+                while (q >= 0 && set[q] == k - sz + q) --q;
+                if (q < 0) break;
+                ++set[q];
+                for (int r = q + 1; r < sz; ++r) set[r] = set[r - 1] + 1;
+            }
+        }
     }
     HIP_TRY(c, hipSetDevice(c->device));
     auto plan = std::make_unique<pg_plan>();
     plan->ctx = c;
     plan->device = c->device;
     plan->k = k;
+    plan->K = k - 1;
     plan->nsub = nsub;
     plan->nonleaf = nonleaf;
     plan->sizes.assign(sub_size, sub_size + nsub);
     plan->mains.assign(main_idx, main_idx + nsub);
     plan->auxs.assign(aux_idx, aux_idx + nsub);
-    std::vector<int64_t> c2c_off(nsub + 1, 0), split_off(nsub + 1, 0);
-    for (int s = 0; s < nsub; ++s) {
-        c2c_off[s + 1] = c2c_off[s] + IndexSys::comb_calc(k, sub_size[s]);
-        int64_t ns = 0;
-        if (sub_size[s] > 1) ns = (int64_t)IndexSys::comb_calc(k, sub_size[s]) * IndexSys::comb_calc(sub_size[s], sub_size[main_idx[s]]);
-        split_off[s + 1] = split_off[s] + ns;
-    }
+    const int K = k - 1;
+    const std::vector<int16_t> rank = colex_rank_table(K);
+    HIP_TRY(c, plan->rank_tab.alloc(rank.size() * 2));
+    HIP_TRY(c, hipMemcpy(plan->rank_tab.p, rank.data(), rank.size() * 2, hipMemcpyHostToDevice));
     // the reference scales the first non-leaf it processes (highest index)
     // unless that is the root (CountMat.cpp:783-790, 811)
     int first_nonleaf = -1;
-    for (int s = nsub - 1; s >= 0; --s) if (sub_size[s] > 1) { first_nonleaf = s; break; }
+    for (int s = nsub - 1; s >= 0; --s)
+        if (sub_size[s] > 1) { first_nonleaf = s; break; }
     plan->scaled_sub = first_nonleaf > 0 ? first_nonleaf : -1;
     for (int s = nsub - 1; s >= 0; --s) {
         if (sub_size[s] == 1) continue;
@@ -762,36 +898,48 @@ extern "C" int pg_plan_create(pg_ctx* c, int k, int nsub, const int32_t* sub_siz
         d->ai = aux_idx[s];
         d->mm = sub_size[d->mi];
         d->aa = sub_size[d->ai];
+        d->Wo = IndexSys::comb_calc(K, d->size - 1);
+        d->Wy = IndexSys::comb_calc(K, d->aa);
+        d->Wa = d->aa > 1 ? IndexSys::comb_calc(K, d->aa - 1) : k;
+        d->Wm = d->mm > 1 ? IndexSys::comb_calc(K, d->mm - 1) : 0;
+        d->P = IndexSys::comb_calc(d->size - 1, d->mm - 1);
         d->Cs = IndexSys::comb_calc(k, d->size);
         d->Ca = IndexSys::comb_calc(k, d->aa);
         d->Cm = IndexSys::comb_calc(k, d->mm);
-        d->P = IndexSys::comb_calc(d->size, d->mm);
         d->root = s == 0;
         d->scaled = s == plan->scaled_sub;
-        const int32_t* c2c = comb_to_count + c2c_off[s];
-        const int32_t* sm = split_main + split_off[s];
-        const int32_t* sa = split_aux + split_off[s];
-        std::vector<uint32_t> by_rank((size_t)d->Cs * d->P, 0);
-        std::vector<char> hit(d->Cs, 0);
-        std::vector<int16_t> drop;
-        if (d->mm == 1) drop.assign((size_t)d->Cs * k, (int16_t)-1);
-        for (int i = 0; i < d->Cs; ++i) {
-            int r = c2c[i];
-            if (r < 0 || r >= d->Cs || hit[r]) return set_err(c, PG_ERR_INVALID, "comb_to_count of %d is not a permutation", s);
-            hit[r] = 1;
-            for (int j = 0; j < d->P; ++j) {
-                int m = sm[(int64_t)i * d->P + j], a = sa[(int64_t)i * d->P + j];
-                if (m < 0 || m >= d->Cm || a < 0 || a >= d->Ca)
-                    return set_err(c, PG_ERR_INVALID, "split index out of range in subtemplate %d", s);
-                by_rank[(size_t)r * d->P + j] = ((uint32_t)m << 16) | (uint32_t)a;
-                if (d->mm == 1) drop[(size_t)r * k + m] = (int16_t)a;  // colex rank of {c} is c
+        // split table over output sets U (size-1 colors of the K others):
+        // j-th lexicographic (mm-1)-subset of U's positions is the main part
+        if (d->mm > 1) {
+            const std::vector<uint16_t> outs = masks_of_size(K, d->size - 1);
+            std::vector<uint32_t> tab;
+            tab.reserve((size_t)d->Wo * d->P);
+            const int nm = d->mm - 1, n1 = d->size - 1;
+            for (uint16_t U : outs) {
+                int el[kMaxK], ne = 0;
+                for (int bit = 0; bit < K; ++bit)
+                    if (U >> bit & 1) el[ne++] = bit;
+                std::vector<int> pos(nm);
+                std::iota(pos.begin(), pos.end(), 0);
+                for (int j = 0; j < d->P; ++j) {
+                    uint32_t mmask = 0;
+                    for (int q : pos) mmask |= 1u << el[q];
+                    const uint32_t amask = U & ~mmask;
+                    tab.push_back(((uint32_t)rank[mmask] << 16) | (uint32_t)rank[amask]);
+                    int q = nm - 1;
+                    while (q >= 0 && pos[q] == n1 - nm + q) --q;
+                    if (q < 0) break;
+                    ++pos[q];
+                    for (int r = q + 1; r < nm; ++r) pos[r] = pos[r - 1] + 1;
+                }
             }
+            HIP_TRY(c, d->split.alloc(tab.size() * 4));
+            HIP_TRY(c, hipMemcpy(d->split.p, tab.data(), tab.size() * 4, hipMemcpyHostToDevice));
         }
-        HIP_TRY(c, d->split.alloc(by_rank.size() * 4));
-        HIP_TRY(c, hipMemcpy(d->split.p, by_rank.data(), by_rank.size() * 4, hipMemcpyHostToDevice));
-        if (d->mm == 1) {
-            HIP_TRY(c, d->drop.alloc(drop.size() * 2));
-            HIP_TRY(c, hipMemcpy(d->drop.p, drop.data(), drop.size() * 2, hipMemcpyHostToDevice));
+        if (d->aa > 1) {
+            const std::vector<uint16_t> ym = masks_of_size(K, d->aa);
+            HIP_TRY(c, d->ymask.alloc(ym.size() * 2));
+            HIP_TRY(c, hipMemcpy(d->ymask.p, ym.data(), ym.size() * 2, hipMemcpyHostToDevice));
         }
         plan->subs.push_back(std::move(d));
     }
@@ -802,8 +950,8 @@ extern "C" int pg_plan_create(pg_ctx* c, int k, int nsub, const int32_t* sub_siz
 extern "C" int pg_plan_from_edges(pg_ctx* c, int nv, int ne, const int32_t* src, const int32_t* dst, pg_plan** out)
 {
     if (!c || !out || !src || !dst) return PG_ERR_INVALID;
-    if (nv < 2 || nv > kMaxTemplate || ne != nv - 1)
-        return set_err(c, PG_ERR_INVALID, "template must be a tree with 2..%d vertices", kMaxTemplate);
+    if (nv < 2 || nv > kMaxK || ne != nv - 1)
+        return set_err(c, PG_ERR_INVALID, "template must be a tree with 2..%d vertices", kMaxK);
     for (int j = 0; j < ne; ++j)
         if (src[j] < 0 || src[j] >= nv || dst[j] < 0 || dst[j] >= nv)
             return set_err(c, PG_ERR_INVALID, "template vertex id out of range");
@@ -840,7 +988,8 @@ extern "C" int pg_count(pg_ctx* c, const pg_plan* p, int ncol, const uint8_t* co
 {
     if (!c) return PG_ERR_INVALID;
     if (!p || p->ctx != c || ncol <= 0 || !raw_out) return set_err(c, PG_ERR_INVALID, "invalid count arguments");
-    if (precision != PG_FP32 && precision != PG_FP64) return set_err(c, PG_ERR_INVALID, "precision must be PG_FP32 or PG_FP64");
+    if (precision != PG_FP32 && precision != PG_FP64)
+        return set_err(c, PG_ERR_INVALID, "precision must be PG_FP32 or PG_FP64");
     if (!c->has_graph) return set_err(c, PG_ERR_STATE, "no graph uploaded");
     if (colors) {
         const size_t total = (size_t)ncol * c->n;
@@ -850,6 +999,7 @@ extern "C" int pg_count(pg_ctx* c, const pg_plan* p, int ncol, const uint8_t* co
     HIP_TRY(c, hipSetDevice(c->device));
     if (c->n_act == 0) {
         for (int i = 0; i < ncol; ++i) raw_out[i] = 0.0;
+        c->last_total_ms = 0;
         return PG_OK;
     }
     return precision == PG_FP32 ? count_impl<float>(c, p, ncol, colors, seed, flags, raw_out)
@@ -860,7 +1010,8 @@ extern "C" int pg_count_multi(pg_ctx* c, int nplans, const pg_plan* const* plans
                               const uint64_t* seeds, int precision, uint32_t flags, double* raw_out)
 {
     if (!c) return PG_ERR_INVALID;
-    if (nplans <= 0 || !plans || !ncolorings || !seeds || !raw_out) return set_err(c, PG_ERR_INVALID, "invalid multi-count arguments");
+    if (nplans <= 0 || !plans || !ncolorings || !seeds || !raw_out)
+        return set_err(c, PG_ERR_INVALID, "invalid multi-count arguments");
     int64_t off = 0;
     double total = 0;
     for (int i = 0; i < nplans; ++i) {
@@ -873,7 +1024,8 @@ extern "C" int pg_count_multi(pg_ctx* c, int nplans, const pg_plan* const* plans
     return PG_OK;
 }
 
-extern "C" int pg_count_grad(pg_ctx* c, const pg_plan*, int, const uint8_t*, uint64_t, int, uint32_t, double*, void*, int)
+extern "C" int pg_count_grad(pg_ctx* c, const pg_plan*, int, const uint8_t*, uint64_t, int, uint32_t, double*, void*,
+                             int)
 {
     return set_err(c, PG_ERR_STATE, "pg_count_grad: not built yet");
 }

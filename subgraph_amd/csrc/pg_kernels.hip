@@ -7,19 +7,23 @@
 // (DataTableColMajor::arrayWiseFMA*, DataTableColMajor.cpp:346-500), fused so
 // the SpMM result never leaves the workgroup.
 //
-// Layout: tables are vertex-major, row v holds B colorings x C color-set
-// columns (column = colex rank), so a neighbour gather reads one contiguous
-// row segment.  A "team" of TEAM lanes (8..64) owns one work item; its lanes
-// split the row's columns.  Teams of a workgroup move through the phases in
-// lockstep: (1) gather Y into registers, then LDS; (2) stage the main row in
-// LDS; (3) every lane produces output sets, reading the split table and LDS.
+// Work decomposition: a team of TEAM lanes owns one vertex (or one hub
+// chunk).  The team's lanes are `ng` neighbour groups x (TEAM/ng) column
+// lanes: group g sums neighbours g, g+ng, ... in row order, column lanes
+// split the Y values, and the groups are folded with an xor butterfly, so
+// the result is fixed by the launch shape.  Workgroups are persistent: they
+// stride over items, and stage the colex rank table in LDS once.
 //
-// FP32 numerics follow the reference: neighbours are summed in the order of
-// the row (ascending caller id), splits in the reference's j order, with
-// separate multiply and add (this file is compiled with -ffp-contract=off);
-// the scaled subtemplate and the root use FP64 exactly as
+// Phases per item: (1) Y into registers, then LDS; (2) the main row into
+// LDS; (3) each lane produces output values from the split table and LDS.
+//
+// FP32 numerics follow the reference: separate multiply and add (this file
+// is compiled with -ffp-contract=off), splits in the reference's j order; the
+// scaled subtemplate and the root use FP64 exactly as
 // DataTableColMajor.cpp:367 and :396 do.
 #include "pg_kernels.hpp"
+
+#include <map>
 
 namespace pgbsc {
 
@@ -33,19 +37,51 @@ __device__ __forceinline__ uint64_t d_splitmix64(uint64_t x)
     return x ^ (x >> 31);
 }
 
-template <typename T, int AUX>
-__device__ __forceinline__ T aux_value(const SubArgs<T>& a, int u, int c, int64_t lda)
+// insert a zero bit at position c (K-bit reduced mask -> k-bit color mask)
+__device__ __forceinline__ uint32_t expand_bit(uint32_t m, uint32_t c)
 {
-    if (AUX == AUX_GATHER) {
-        return a.aux[(int64_t)u * lda + c];
-    } else {
-        // one-hot leaf: column c = (coloring b, color c % k)
-        const int b = c / a.k;
-        return a.colors[(int64_t)u * a.B + b] == (uint8_t)(c - b * a.k) ? T(1) : T(0);
-    }
+    return (m & ((1u << c) - 1u)) | ((m >> c) << (c + 1u));
 }
 
+// remove bit position c (k-bit color mask -> K-bit mask relative to color c)
+__device__ __forceinline__ uint32_t squeeze_bit(uint32_t m, uint32_t c)
+{
+    return (m & ((1u << c) - 1u)) | ((m >> (c + 1u)) << c);
+}
+
+// one neighbour's contribution to a Y value: GATHER reads the neighbour's
+// color-relative row at the rank of (Y set minus the neighbour's color);
+// COLORS is the one-hot leaf (amask holds the value's color)
+template <typename T, int AUX>
+__device__ __forceinline__ T aux_term(const SubArgs<T>& a, const int16_t* s_rank, uint32_t amask, int aoff, int u,
+                                      uint32_t cu)
+{
+    if (AUX == AUX_GATHER) {
+        if ((amask >> cu) & 1u) {
+            const int idx = s_rank[squeeze_bit(amask & ~(1u << cu), cu)];
+            return a.aux[(int64_t)u * a.lda + aoff + idx];
+        }
+        return T(0);
+    }
+    return cu == amask ? T(1) : T(0);
+}
+
+__host__ __device__ constexpr size_t rank_lds_bytes(int K) { return (((size_t)2 << K) + 15) & ~(size_t)15; }
+
 }  // namespace
+
+template <typename T>
+size_t sub_lds_bytes(int team, int aux, int main_mode, const SubArgs<T>& a)
+{
+    const int teams = kBlock / team;
+    size_t bytes = aux == AUX_GATHER ? rank_lds_bytes(a.K) : 0;
+    if (a.out_mode == OUT_Y || a.out_mode == OUT_PARTIAL) return bytes;
+    const size_t per = (size_t)a.B * a.Wy + (main_mode == MAIN_TABLE ? (size_t)a.B * a.Wm : 0);
+    return bytes + (size_t)teams * per * sizeof(T);
+}
+
+template size_t sub_lds_bytes<float>(int, int, int, const SubArgs<float>&);
+template size_t sub_lds_bytes<double>(int, int, int, const SubArgs<double>&);
 
 template <typename T, int TEAM, int AUX, int MAIN>
 __global__ __launch_bounds__(kBlock) void k_sub(const SubArgs<T> a)
@@ -54,209 +90,276 @@ __global__ __launch_bounds__(kBlock) void k_sub(const SubArgs<T> a)
     constexpr int TEAMS = kBlock / TEAM;
     const int lane = threadIdx.x % TEAM;
     const int team = threadIdx.x / TEAM;
-    const int item = blockIdx.x * TEAMS + team;
-    const bool active = item < a.n_items;
     const int B = a.B;
-    const int nY = B * a.Ca;
-    const int nM = (MAIN == MAIN_TABLE) ? B * a.Cm : 0;
-    const int per_team = nY + nM;
-    T* sY = reinterpret_cast<T*>(smem_raw) + (size_t)team * per_team;
+    const int mode = a.out_mode;
+    const int nY = B * a.Wy;
+    const int nM = (MAIN == MAIN_TABLE) ? B * a.Wm : 0;
+    const size_t rank_bytes = (AUX == AUX_GATHER) ? rank_lds_bytes(a.K) : 0;
+    int16_t* s_rank = reinterpret_cast<int16_t*>(smem_raw);
+    T* sY = reinterpret_cast<T*>(smem_raw + rank_bytes) + (size_t)team * (nY + nM);
     T* sM = sY + nY;
 
-    int v = 0, e0 = 0, e1 = 0, slot = 0;
-    if (active) {
-        if (a.items) {
-            const int4 it = a.items[item];
-            v = it.x; e0 = it.y; e1 = it.z; slot = it.w;
-        } else {
-            v = a.item_base + item;
-            if (AUX == AUX_GATHER || AUX == AUX_COLORS) { e0 = a.rowptr[v]; e1 = a.rowptr[v + 1]; }
-        }
+    if (AUX == AUX_GATHER) {
+        for (int i = threadIdx.x; i < (1 << a.K); i += kBlock) s_rank[i] = a.rank_tab[i];
+        __syncthreads();
     }
-    const int mode = a.out_mode;
 
-    // ---------------------------------------------------------- phase 1: Y
-    if (AUX == AUX_GATHER || AUX == AUX_COLORS) {
-        for (int c0 = 0; c0 < nY; c0 += TEAM * kRegCols) {
-            T acc[kRegCols];
-            int cc[kRegCols];
-#pragma unroll
-            for (int r = 0; r < kRegCols; ++r) {
-                acc[r] = T(0);
-                cc[r] = c0 + lane + r * TEAM;
+    const int ng = a.ng;
+    const int cl = TEAM / ng;   // column lanes per group
+    const int grp = lane / cl;
+    const int cln = lane - grp * cl;
+
+    for (int64_t base = (int64_t)blockIdx.x * TEAMS; base < a.n_items; base += (int64_t)gridDim.x * TEAMS) {
+        const int64_t item = base + team;
+        const bool active = item < a.n_items;
+        int v = 0, e0 = 0, e1 = 0, slot = 0;
+        if (active) {
+            if (a.items) {
+                const int4 it = a.items[item];
+                v = it.x; e0 = it.y; e1 = it.z; slot = it.w;
+            } else {
+                v = a.item_base + (int)item;
+                if (AUX == AUX_GATHER || AUX == AUX_COLORS) { e0 = a.rowptr[v]; e1 = a.rowptr[v + 1]; }
             }
-            if (active) {
-                int e = e0;
-                for (; e + 4 <= e1; e += 4) {
-                    int u[4];
-                    T wv[4];
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) {
-                        u[q] = a.col[e + q];
-                        wv[q] = a.w ? a.w[e + q] : T(1);
-                    }
-                    T x[4][kRegCols];
-#pragma unroll
-                    for (int q = 0; q < 4; ++q)
-#pragma unroll
-                        for (int r = 0; r < kRegCols; ++r)
-                            x[q][r] = cc[r] < nY ? aux_value<T, AUX>(a, u[q], cc[r], a.lda) : T(0);
-#pragma unroll
-                    for (int q = 0; q < 4; ++q)
-#pragma unroll
-                        for (int r = 0; r < kRegCols; ++r)
-                            acc[r] = a.w ? acc[r] + wv[q] * x[q][r] : acc[r] + x[q][r];
-                }
-                for (; e < e1; ++e) {
-                    const int u = a.col[e];
-                    const T wv = a.w ? a.w[e] : T(1);
-#pragma unroll
-                    for (int r = 0; r < kRegCols; ++r) {
-                        const T x = cc[r] < nY ? aux_value<T, AUX>(a, u, cc[r], a.lda) : T(0);
-                        acc[r] = a.w ? acc[r] + wv * x : acc[r] + x;
-                    }
-                }
-            }
-            if (active) {
+        }
+
+        // ------------------------------------------------------ phase 1: Y
+        if (AUX == AUX_GATHER || AUX == AUX_COLORS) {
+            // a slot is kRegCols consecutive Y values of one coloring; column
+            // lane cln of every group owns slot s0 + cln of each chunk
+            const int W = (AUX == AUX_GATHER) ? a.Wy : a.k;
+            const int spb = (W + kRegCols - 1) / kRegCols;
+            const int nslots = B * spb;
+            for (int s0 = 0; s0 < nslots; s0 += cl) {
+                const int sid = s0 + cln;
+                const bool sv = active && sid < nslots;
+                const int b = sv ? sid / spb : 0;
+                const int i0 = (sid - b * spb) * kRegCols;
+                T acc[kRegCols];
+                uint32_t amask[kRegCols];  // GATHER: k-bit color set of the value (0 = unused); COLORS: its color
+                const uint32_t cv = sv && AUX == AUX_GATHER ? a.colors[(int64_t)v * B + b] : 0u;
 #pragma unroll
                 for (int r = 0; r < kRegCols; ++r) {
-                    if (cc[r] >= nY) continue;
-                    if (mode == OUT_Y) a.out[(int64_t)v * a.ldo + cc[r]] = acc[r];
-                    else if (mode == OUT_PARTIAL) a.out[(int64_t)slot * a.ldo + cc[r]] = acc[r];
-                    else sY[cc[r]] = acc[r];
+                    acc[r] = T(0);
+                    const int i = i0 + r;
+                    const bool ok = sv && i < W;
+                    if (AUX == AUX_GATHER) amask[r] = ok ? expand_bit(a.ymask[i], cv) : 0u;
+                    else amask[r] = ok ? (uint32_t)i : 0xffu;
                 }
-            }
-        }
-    } else if (AUX == AUX_READ) {
-        if (active)
-            for (int c = lane; c < nY; c += TEAM) sY[c] = a.aux[(int64_t)v * a.lda + c];
-    } else {  // AUX_PARTIALS: item = hub index, v = hub vertex
-        if (active) {
-            const int s0 = a.slot_begin[item], s1 = a.slot_begin[item + 1];
-            for (int c = lane; c < nY; c += TEAM) {
-                T acc = T(0);
-                for (int sl = s0; sl < s1; ++sl) acc = acc + a.aux[(int64_t)sl * a.lda + c];
-                if (mode == OUT_Y) a.out[(int64_t)v * a.ldo + c] = acc;
-                else sY[c] = acc;
-            }
-        }
-    }
-    if (mode == OUT_Y || mode == OUT_PARTIAL) return;  // uniform over the grid
-
-    // ------------------------------------------------------ phase 2: main
-    if (MAIN == MAIN_TABLE && active)
-        for (int c = lane; c < nM; c += TEAM) sM[c] = a.main[(int64_t)v * a.ldm + c];
-    __syncthreads();
-
-    // --------------------------------------------------- phase 3: combine
-    const int Cs = a.Cs, P = a.P, Ca = a.Ca, Cm = a.Cm;
-    const int nO = B * Cs;
-    if (mode == OUT_ROOT) {
-        // Cs == 1: lane b owns coloring b of this vertex; the per-vertex FP64
-        // value goes to root_vals[b][v] and is summed later in a fixed order
-        if (active) {
-            for (int b = lane; b < B; b += TEAM) {
-                double val = 0.0;
-                const T* y = sY + b * Ca;
-                if (MAIN == MAIN_LEAF) {
-                    const int c = a.colors[(int64_t)v * B + b];
-                    const int ai = a.drop[c];
-                    // f32 product y * 1, widened (DataTableColMajor.cpp:396)
-                    val = ai >= 0 ? (double)y[ai] : 0.0;
-                } else {
-                    const T* m = sM + b * Cm;
-                    for (int j = 0; j < P; ++j) {
-                        const uint32_t p = a.split[j];
-                        const T prod = y[p & 0xffffu] * m[p >> 16];
-                        val = val + (double)prod;
+                const int aoff = b * a.Wa;
+                // a neighbour past the end reads as the vertex itself with a
+                // color no value uses (GATHER: col(v) is never in a Y set)
+                const uint32_t dead = AUX == AUX_GATHER ? cv : 0xffu;
+                if (active) {
+                    const int step = ng;
+                    int e = e0 + grp;
+                    // software pipeline: the ids, colors and weights of the
+                    // next 4 neighbours load while the current rows are read
+                    int un[4];
+                    uint32_t cn[4];
+                    T wn[4];
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const int ee = e + q * step;
+                        const bool ok = ee < e1;
+                        un[q] = ok ? a.col[ee] : v;
+                        cn[q] = ok ? (uint32_t)a.ecol[(int64_t)ee * B + b] : dead;
+                        wn[q] = ok && a.w ? a.w[ee] : T(1);
+                    }
+                    for (; e < e1; e += 4 * step) {
+                        int u[4];
+                        uint32_t cu[4];
+                        T wv[4];
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) {
+                            u[q] = un[q];
+                            cu[q] = cn[q];
+                            wv[q] = wn[q];
+                            const int ee = e + (4 + q) * step;
+                            const bool ok = ee < e1;
+                            un[q] = ok ? a.col[ee] : v;
+                            cn[q] = ok ? (uint32_t)a.ecol[(int64_t)ee * B + b] : dead;
+                            wn[q] = ok && a.w ? a.w[ee] : T(1);
+                        }
+                        T x[4][kRegCols];
+#pragma unroll
+                        for (int q = 0; q < 4; ++q)
+#pragma unroll
+                            for (int r = 0; r < kRegCols; ++r)
+                                x[q][r] = aux_term<T, AUX>(a, s_rank, amask[r], aoff, u[q], cu[q]);
+#pragma unroll
+                        for (int q = 0; q < 4; ++q)
+#pragma unroll
+                            for (int r = 0; r < kRegCols; ++r)
+                                acc[r] = a.w ? acc[r] + wv[q] * x[q][r] : acc[r] + x[q][r];
                     }
                 }
-                a.root_vals[(int64_t)b * a.n_root + v] = val;
-            }
-        }
-        return;
-    }
-    if (active) {
-        for (int o = lane; o < nO; o += TEAM) {
-            const int b = o / Cs;
-            const int r = o - b * Cs;
-            const T* y = sY + b * Ca;
-            T res;
-            if (MAIN == MAIN_LEAF) {
-                // only the split whose main set is {col(v)} has a non-zero
-                // main value (1.0): res = 0 + 1 * y exactly
-                const int c = a.colors[(int64_t)v * B + b];
-                const int ai = a.drop[r * a.k + c];
-                const T yv = ai >= 0 ? y[ai] : T(0);
-                if (mode == OUT_EMA_SCALED) res = (T)((double)yv * (double)a.scale);
-                else res = yv;
-            } else {
-                const T* m = sM + b * Cm;
-                const uint32_t* sp = a.split + (size_t)r * P;
-                if (mode == OUT_EMA_SCALED) {
-                    // DataTableColMajor.cpp:367: dst + (a * (double)b) * scale
-                    T dst = T(0);
-                    for (int j = 0; j < P; ++j) {
-                        const uint32_t p = sp[j];
-                        const double t = ((double)y[p & 0xffffu] * (double)m[p >> 16]) * (double)a.scale;
-                        dst = (T)((double)dst + t);
+                // fold the neighbour groups: fixed xor tree, identical in every lane
+                for (int st = cl; st < TEAM; st <<= 1)
+#pragma unroll
+                    for (int r = 0; r < kRegCols; ++r) acc[r] = acc[r] + __shfl_xor(acc[r], st, TEAM);
+                if (sv && grp == 0) {
+#pragma unroll
+                    for (int r = 0; r < kRegCols; ++r) {
+                        const int i = i0 + r;
+                        if (i >= W) continue;
+                        const int cc = b * W + i;
+                        if (mode == OUT_Y) a.out[(int64_t)v * a.ldo + cc] = acc[r];
+                        else if (mode == OUT_PARTIAL) a.out[(int64_t)slot * a.ldo + cc] = acc[r];
+                        else sY[cc] = acc[r];
                     }
-                    res = dst;
-                } else {
+                }
+            }
+        } else if (AUX == AUX_READ) {
+            // aux child is a leaf: Y value i is H at the i-th color other than col(v)
+            if (active)
+                for (int c = lane; c < nY; c += TEAM) {
+                    const int b = c / a.Wy;
+                    const int i = c - b * a.Wy;
+                    const int cv = a.colors[(int64_t)v * B + b];
+                    sY[c] = a.aux[(int64_t)v * a.lda + b * a.k + i + (i >= cv ? 1 : 0)];
+                }
+        } else {  // AUX_PARTIALS: item = hub index, v = hub vertex
+            if (active) {
+                const int s0 = a.slot_begin[item], s1 = a.slot_begin[item + 1];
+                for (int c = lane; c < nY; c += TEAM) {
                     T acc = T(0);
-                    for (int j = 0; j < P; ++j) {
-                        const uint32_t p = sp[j];
-                        const T prod = y[p & 0xffffu] * m[p >> 16];
-                        acc = acc + prod;
-                    }
-                    res = acc;
+                    for (int sl = s0; sl < s1; ++sl) acc = acc + a.aux[(int64_t)sl * a.lda + c];
+                    if (mode == OUT_Y) a.out[(int64_t)v * a.ldo + c] = acc;
+                    else sY[c] = acc;
                 }
             }
-            a.out[(int64_t)v * a.ldo + o] = res;
         }
+        if (mode == OUT_Y || mode == OUT_PARTIAL) continue;  // uniform over the grid
+
+        // -------------------------------------------------- phase 2: main
+        if (MAIN == MAIN_TABLE && active)
+            for (int c = lane; c < nM; c += TEAM) sM[c] = a.main[(int64_t)v * a.ldm + c];
+        __syncthreads();
+
+        // ----------------------------------------------- phase 3: combine
+        const int Wy = a.Wy, Wm = a.Wm, Wo = a.Wo, P = a.P;
+        if (mode == OUT_ROOT) {
+            if (active) {
+                for (int b = lane; b < B; b += TEAM) {
+                    double val = 0.0;
+                    const T* y = sY + b * Wy;
+                    if (MAIN == MAIN_LEAF) {
+                        val = (double)y[0];  // f32 product y * 1, widened (DataTableColMajor.cpp:396)
+                    } else {
+                        const T* m = sM + b * Wm;
+                        for (int j = 0; j < P; ++j) {
+                            const uint32_t p = a.split[j];
+                            const T prod = y[p & 0xffffu] * m[p >> 16];
+                            val = val + (double)prod;
+                        }
+                    }
+                    a.root_vals[(int64_t)b * a.n_root + v] = val;
+                }
+            }
+        } else if (active) {
+            const int nO = B * Wo;
+            for (int o = lane; o < nO; o += TEAM) {
+                const int b = o / Wo;
+                const int r = o - b * Wo;
+                const T* y = sY + b * Wy;
+                T res;
+                if (MAIN == MAIN_LEAF) {
+                    // main = {col(v)}: the set's other colors are the Y set itself
+                    if (mode == OUT_EMA_SCALED) res = (T)((double)y[r] * (double)a.scale);
+                    else res = y[r];
+                } else {
+                    const T* m = sM + b * Wm;
+                    const uint32_t* sp = a.split + (size_t)r * P;
+                    if (mode == OUT_EMA_SCALED) {
+                        // DataTableColMajor.cpp:367: dst + (a * (double)b) * scale
+                        T dst = T(0);
+                        for (int j = 0; j < P; ++j) {
+                            const uint32_t p = sp[j];
+                            const double t = ((double)y[p & 0xffffu] * (double)m[p >> 16]) * (double)a.scale;
+                            dst = (T)((double)dst + t);
+                        }
+                        res = dst;
+                    } else {
+                        T acc = T(0);
+                        for (int j = 0; j < P; ++j) {
+                            const uint32_t p = sp[j];
+                            const T prod = y[p & 0xffffu] * m[p >> 16];
+                            acc = acc + prod;
+                        }
+                        res = acc;
+                    }
+                }
+                a.out[(int64_t)v * a.ldo + o] = res;
+            }
+        }
+        __syncthreads();  // LDS rows are rewritten by the next item
     }
 }
 
-template <typename T, int TEAM>
-static hipError_t launch_team(int aux, int main_mode, const SubArgs<T>& a, int nblocks, size_t lds, hipStream_t st)
+// Persistent grid: as many workgroups as can be resident at once (occupancy
+// for this instantiation and LDS size, times the CU count), never more than
+// there are items.
+template <typename T, int TEAM, int AUX, int MAIN>
+static hipError_t launch_one(const SubArgs<T>& a, int num_cus, hipStream_t st)
 {
-#define PG_LAUNCH(AUXV, MAINV) \
-    hipLaunchKernelGGL((k_sub<T, TEAM, AUXV, MAINV>), dim3(nblocks), dim3(kBlock), lds, st, a)
-    if (main_mode == MAIN_LEAF) {
-        switch (aux) {
-        case AUX_GATHER: PG_LAUNCH(AUX_GATHER, MAIN_LEAF); break;
-        case AUX_COLORS: PG_LAUNCH(AUX_COLORS, MAIN_LEAF); break;
-        case AUX_READ: PG_LAUNCH(AUX_READ, MAIN_LEAF); break;
-        case AUX_PARTIALS: PG_LAUNCH(AUX_PARTIALS, MAIN_LEAF); break;
-        default: return hipErrorInvalidValue;
-        }
+    const size_t lds = sub_lds_bytes<T>(TEAM, AUX, MAIN, a);
+    const int64_t need = (a.n_items + (kBlock / TEAM) - 1) / (kBlock / TEAM);
+    if (need <= 0) return hipSuccess;
+    static thread_local std::map<size_t, int> per_cu_cache;
+    int per_cu = 0;
+    auto it = per_cu_cache.find(lds);
+    if (it != per_cu_cache.end()) {
+        per_cu = it->second;
     } else {
-        switch (aux) {
-        case AUX_GATHER: PG_LAUNCH(AUX_GATHER, MAIN_TABLE); break;
-        case AUX_READ: PG_LAUNCH(AUX_READ, MAIN_TABLE); break;
-        case AUX_PARTIALS: PG_LAUNCH(AUX_PARTIALS, MAIN_TABLE); break;
-        default: return hipErrorInvalidValue;
-        }
+        hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sub<T, TEAM, AUX, MAIN>, kBlock, lds);
+        if (e != hipSuccess) return e;
+        if (per_cu < 1) return hipErrorInvalidConfiguration;
+        per_cu_cache[lds] = per_cu;
     }
-#undef PG_LAUNCH
+    const int64_t cap = (int64_t)per_cu * num_cus;
+    const int grid = (int)(need < cap ? need : cap);
+    hipLaunchKernelGGL((k_sub<T, TEAM, AUX, MAIN>), dim3(grid), dim3(kBlock), lds, st, a);
     return hipGetLastError();
 }
 
-template <typename T>
-hipError_t launch_sub(int team, int aux, int main_mode, const SubArgs<T>& a, int nblocks, size_t lds, hipStream_t st)
+template <typename T, int TEAM>
+static hipError_t launch_team(int aux, int main_mode, const SubArgs<T>& a, int max_blocks, hipStream_t st)
 {
-    if (nblocks <= 0) return hipSuccess;
-    switch (team) {
-    case 8: return launch_team<T, 8>(aux, main_mode, a, nblocks, lds, st);
-    case 16: return launch_team<T, 16>(aux, main_mode, a, nblocks, lds, st);
-    case 32: return launch_team<T, 32>(aux, main_mode, a, nblocks, lds, st);
-    case 64: return launch_team<T, 64>(aux, main_mode, a, nblocks, lds, st);
+    if (main_mode == MAIN_LEAF) {
+        switch (aux) {
+        case AUX_GATHER: return launch_one<T, TEAM, AUX_GATHER, MAIN_LEAF>(a, max_blocks, st);
+        case AUX_COLORS: return launch_one<T, TEAM, AUX_COLORS, MAIN_LEAF>(a, max_blocks, st);
+        case AUX_READ: return launch_one<T, TEAM, AUX_READ, MAIN_LEAF>(a, max_blocks, st);
+        case AUX_PARTIALS: return launch_one<T, TEAM, AUX_PARTIALS, MAIN_LEAF>(a, max_blocks, st);
+        default: return hipErrorInvalidValue;
+        }
+    }
+    switch (aux) {
+    case AUX_GATHER: return launch_one<T, TEAM, AUX_GATHER, MAIN_TABLE>(a, max_blocks, st);
+    case AUX_READ: return launch_one<T, TEAM, AUX_READ, MAIN_TABLE>(a, max_blocks, st);
+    case AUX_PARTIALS: return launch_one<T, TEAM, AUX_PARTIALS, MAIN_TABLE>(a, max_blocks, st);
     default: return hipErrorInvalidValue;
     }
 }
 
-template hipError_t launch_sub<float>(int, int, int, const SubArgs<float>&, int, size_t, hipStream_t);
-template hipError_t launch_sub<double>(int, int, int, const SubArgs<double>&, int, size_t, hipStream_t);
+template <typename T>
+hipError_t launch_sub(int team, int aux, int main_mode, const SubArgs<T>& a, int max_blocks, hipStream_t st)
+{
+    if (a.n_items <= 0) return hipSuccess;
+    if (a.ng < 1 || team % a.ng != 0 || (a.ng & (a.ng - 1)) != 0) return hipErrorInvalidValue;
+    if (aux == AUX_GATHER && (a.K < 1 || a.K > kMaxK - 1)) return hipErrorInvalidValue;
+    switch (team) {
+    case 8: return launch_team<T, 8>(aux, main_mode, a, max_blocks, st);
+    case 16: return launch_team<T, 16>(aux, main_mode, a, max_blocks, st);
+    case 32: return launch_team<T, 32>(aux, main_mode, a, max_blocks, st);
+    case 64: return launch_team<T, 64>(aux, main_mode, a, max_blocks, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template hipError_t launch_sub<float>(int, int, int, const SubArgs<float>&, int, hipStream_t);
+template hipError_t launch_sub<double>(int, int, int, const SubArgs<double>&, int, hipStream_t);
 
 // ------------------------------------------------------------ utilities ---
 
@@ -302,28 +405,69 @@ hipError_t launch_colors_gather(uint8_t* out, const uint8_t* in, const int32_t* 
     return hipGetLastError();
 }
 
-// One workgroup per coloring; thread t sums vertices t, t+1024, ... in order,
-// then a fixed-shape tree: the result depends neither on timing nor on how
-// the root values were produced (batch tile, team size, hub split).
-__global__ __launch_bounds__(1024) void k_root_reduce(const double* vals, int64_t n, int B, double* out)
+__global__ void k_edge_colors(uint8_t* ecol, const int32_t* col, const uint8_t* colors, int64_t nnz, int B)
 {
-    __shared__ double s[1024];
-    const int b = blockIdx.x;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nnz * B) return;
+    const int64_t e = i / B;
+    const int b = (int)(i - e * B);
+    ecol[i] = colors[(int64_t)col[e] * B + b];
+}
+
+hipError_t launch_edge_colors(uint8_t* ecol, const int32_t* col, const uint8_t* colors, int64_t nnz, int B,
+                              hipStream_t st)
+{
+    const int64_t total = nnz * B;
+    if (total == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_edge_colors, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, ecol, col, colors,
+                       nnz, B);
+    return hipGetLastError();
+}
+
+// Two fixed-shape passes: block j of coloring b sums the contiguous slice j
+// of vals[b] (thread t takes every 256th element of it, then a tree), and one
+// block per coloring folds the kReduceBlocks slice sums with the same tree.
+// The result depends neither on timing nor on how the values were produced.
+__global__ __launch_bounds__(256) void k_reduce_slices(const double* vals, int64_t n, double* tmp)
+{
+    __shared__ double s[256];
+    const int b = blockIdx.y;
+    const int64_t per = (n + kReduceBlocks - 1) / kReduceBlocks;
+    const int64_t lo = (int64_t)blockIdx.x * per;
+    const int64_t hi = lo + per < n ? lo + per : n;
     const double* row = vals + (int64_t)b * n;
     double acc = 0.0;
-    for (int64_t i = threadIdx.x; i < n; i += 1024) acc += row[i];
+    for (int64_t i = lo + threadIdx.x; i < hi; i += 256) acc += row[i];
     s[threadIdx.x] = acc;
     __syncthreads();
-    for (int w = 512; w > 0; w >>= 1) {
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) tmp[(int64_t)b * kReduceBlocks + blockIdx.x] = s[0];
+}
+
+__global__ __launch_bounds__(256) void k_reduce_final(const double* tmp, double* out)
+{
+    __shared__ double s[256];
+    const int b = blockIdx.x;
+    s[threadIdx.x] = tmp[(int64_t)b * kReduceBlocks + threadIdx.x];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
         if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
         __syncthreads();
     }
     if (threadIdx.x == 0) out[b] = s[0];
 }
 
-hipError_t launch_root_reduce(const double* vals, int64_t n, int B, double* out, hipStream_t st)
+static_assert(kReduceBlocks == 256, "k_reduce_final folds exactly 256 slices");
+
+hipError_t launch_root_reduce(const double* vals, int64_t n, int B, double* tmp, double* out, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_root_reduce, dim3(B), dim3(1024), 0, st, vals, n, B, out);
+    hipLaunchKernelGGL(k_reduce_slices, dim3(kReduceBlocks, B), dim3(256), 0, st, vals, n, tmp);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_reduce_final, dim3(B), dim3(256), 0, st, tmp, out);
     return hipGetLastError();
 }
 
@@ -347,20 +491,6 @@ hipError_t launch_gather_weights_f64(double* dst, const float* src, const int32_
     if (nnz == 0) return hipSuccess;
     hipLaunchKernelGGL(k_gather_weights<double>, dim3((unsigned)((nnz + 255) / 256)), dim3(256), 0, st, dst, src,
                        ent, nnz);
-    return hipGetLastError();
-}
-
-__global__ void k_fill_zero(uint32_t* dst, int64_t words)
-{
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < words) dst[i] = 0u;
-}
-
-hipError_t launch_fill_zero(void* dst, size_t bytes, hipStream_t st)
-{
-    const int64_t words = (int64_t)(bytes / 4);
-    if (words == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_fill_zero, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, st, (uint32_t*)dst, words);
     return hipGetLastError();
 }
 

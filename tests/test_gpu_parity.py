@@ -74,6 +74,7 @@ def test_random_graphs_and_trees(gpu_ctx, hub):
     """Random simple graphs, random trees k=2..12; hub=16 forces the split
     (partial-row) path for every vertex of degree > 16."""
     rng = np.random.default_rng(1234 if hub is None else 99)
+    saved = gpu_ctx.get_option("hub_degree"), gpu_ctx.get_option("hub_chunk")
     if hub is not None:
         gpu_ctx.set_option("hub_degree", hub)
         gpu_ctx.set_option("hub_chunk", 16)
@@ -97,8 +98,8 @@ def test_random_graphs_and_trees(gpu_ctx, hub):
                 assert _rel(g32[i], o32) <= TOL32, (trial, i, g32[i], o32)
                 assert g64[i] == o64, (trial, i, g64[i], o64)
     finally:
-        gpu_ctx.set_option("hub_degree", 2048)
-        gpu_ctx.set_option("hub_chunk", 1024)
+        gpu_ctx.set_option("hub_degree", saved[0])
+        gpu_ctx.set_option("hub_chunk", saved[1])
 
 
 def test_fp64_equals_brute_force(gpu_ctx):
@@ -118,7 +119,10 @@ def test_fp64_equals_brute_force(gpu_ctx):
 
 
 def test_batch_tiles_agree(gpu_ctx):
-    """One call with 37 colorings equals any tiling of them (and per-coloring calls)."""
+    """One call with 37 colorings equals any tiling of them (and per-coloring
+    calls): FP64 exactly; FP32 within 1e-6, since the neighbour-group shape of
+    a team (and so the order of a vertex's FP32 neighbour sum) follows the
+    number of values per row, i.e. the tile."""
     n = 1 << 12
     s, d = sg.rmat_edges(12, 8, seed=4)
     colptr, rowidx = sg.csc_from_edges(n, s, d)
@@ -134,10 +138,13 @@ def test_batch_tiles_agree(gpu_ctx):
         if ref64 is None:
             ref64, ref32 = g64, g32
         assert np.array_equal(g64, ref64)
-        assert np.array_equal(g32, ref32)  # same per-coloring arithmetic whatever the tile
+        assert np.all(np.abs(g32 - ref32) <= 1e-6 * np.abs(ref64))
+        assert np.all(np.abs(g32 - g64) <= TOL32 * np.abs(g64))
     gpu_ctx.set_option("batch_tile", 0)
-    one = gpu_ctx.count(plan, 1, colors=cols[3:4], precision="fp32")
-    assert one[0] == ref32[3]
+    one = gpu_ctx.count(plan, 1, colors=cols[3:4], precision="fp64")
+    assert one[0] == ref64[3]
+    again = gpu_ctx.count(plan, 37, colors=cols, precision="fp32")
+    assert np.array_equal(again, ref32)  # run to run: bit-identical
 
 
 def test_device_colors_equal_host_stream(gpu_ctx):
@@ -222,9 +229,10 @@ def test_full_size_properties_s20(gpu_ctx):
     assert np.array_equal(g64, p64)
     for i in range(2):
         assert _rel(g32[i], g64[i]) <= TOL32
+    saved = gpu_ctx.get_option("hub_degree")
     gpu_ctx.set_option("hub_degree", 1 << 30)
     try:
         nohub = gpu_ctx.count(plan, 2, colors=cols, precision="fp64")
     finally:
-        gpu_ctx.set_option("hub_degree", 2048)
+        gpu_ctx.set_option("hub_degree", saved)
     assert np.array_equal(nohub, g64)
