@@ -133,12 +133,14 @@ int pg_count_multi(pg_ctx* ctx, int nplans, const pg_plan* const* plans,
 /* ---- differentiation (no reference counterpart: the reference ignores edge
  *      values, CSCGraph.hpp:283) ----
  * Forward as pg_count with the current weights, then the adjoint DP:
- * grad[e] = sum over the colorings of d raw / d A-entry e (caller CSC entry
- * order, nnz values).  grad_on_device = 1: grad is a device pointer (float
- * for PG_FP32, double for PG_FP64), else a host double array. */
+ * grad[e] = sum over colorings c of cotangent[c] * d raw_c / d A-entry e
+ * (caller CSC entry order, nnz values; cotangent NULL = all ones).  At unit
+ * weights this is the number of colorful maps using entry e.
+ * grad_on_device = 1: grad is a device pointer (float for PG_FP32, double
+ * for PG_FP64), else a host double array.  Needs a symmetric pattern. */
 int pg_count_grad(pg_ctx* ctx, const pg_plan* plan, int ncolorings, const uint8_t* colors,
-                  uint64_t seed, int precision, uint32_t flags, double* raw_out,
-                  void* grad, int grad_on_device);
+                  uint64_t seed, int precision, uint32_t flags, const double* cotangent,
+                  double* raw_out, void* grad, int grad_on_device);
 
 /* ---- timing of the last pg_count / pg_count_grad call ----
  * total device milliseconds, and the dominant kernel's launches and summed

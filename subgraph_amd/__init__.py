@@ -195,6 +195,39 @@ class Context:
                     "pg_count")
         return out
 
+    def count_grad(self, plan: Plan, ncolorings: int, colors=None, seed: int = 0, precision="fp64",
+                   fix_2vertex: bool = False, cotangent=None):
+        """Raw counts and d(sum_c cot_c * raw_c)/d(entry weight), caller entry order."""
+        raw = np.zeros(ncolorings, np.float64)
+        grad = np.zeros(max(self.nnz, 1), np.float64)
+        cp = None
+        if colors is not None:
+            colors = np.ascontiguousarray(colors, np.uint8)
+            if colors.size != ncolorings * self.n:
+                raise ValueError("colors must be [ncolorings][n]")
+            cp = colors.ctypes.data
+        cot = None
+        if cotangent is not None:
+            cotangent = np.ascontiguousarray(cotangent, np.float64)
+            cot = cotangent.ctypes.data
+        flags = PG_FLAG_FIX_2VERTEX if fix_2vertex else 0
+        self._check(lib().pg_count_grad(self.ptr, plan.ptr, ncolorings, cp, seed, _PREC[precision], flags, cot,
+                                        raw, grad.ctypes.data, 0), "pg_count_grad")
+        return raw, grad[: self.nnz]
+
+    def count_grad_device(self, plan: Plan, ncolorings: int, grad_ptr: int, seed: int = 0, precision="fp32",
+                          cotangent=None, fix_2vertex: bool = False) -> np.ndarray:
+        """As count_grad, writing the gradient to a device buffer (float for fp32, double for fp64)."""
+        raw = np.zeros(ncolorings, np.float64)
+        cot = None
+        if cotangent is not None:
+            cotangent = np.ascontiguousarray(cotangent, np.float64)
+            cot = cotangent.ctypes.data
+        flags = PG_FLAG_FIX_2VERTEX if fix_2vertex else 0
+        self._check(lib().pg_count_grad(self.ptr, plan.ptr, ncolorings, None, seed, _PREC[precision], flags, cot,
+                                        raw, grad_ptr, 1), "pg_count_grad")
+        return raw
+
     def count_multi(self, plans, ncolorings, seeds, precision="fp32") -> list[np.ndarray]:
         arr = (C.c_void_p * len(plans))(*[p.ptr for p in plans])
         nc = _i32(ncolorings)

@@ -61,7 +61,7 @@ def lib() -> C.CDLL:
         "pg_colors_fill": (None, [C.c_uint64, C.c_int64, C.c_int, C.c_int32, C.c_int, _u8p]),
         "pg_count": (C.c_int, [_vp, _vp, C.c_int, _vp, C.c_uint64, C.c_int, C.c_uint32, _f64p]),
         "pg_count_multi": (C.c_int, [_vp, C.c_int, _vp, _i32p, _vp, C.c_int, C.c_uint32, _f64p]),
-        "pg_count_grad": (C.c_int, [_vp, _vp, C.c_int, _vp, C.c_uint64, C.c_int, C.c_uint32, _f64p,
+        "pg_count_grad": (C.c_int, [_vp, _vp, C.c_int, _vp, C.c_uint64, C.c_int, C.c_uint32, _vp, _f64p,
                                     _vp, C.c_int]),
         "pg_last_timing": (C.c_int, [_vp, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                      C.POINTER(C.c_int64), C.POINTER(C.c_double)]),

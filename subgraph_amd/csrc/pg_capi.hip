@@ -17,6 +17,7 @@
 #include <memory>
 #include <numeric>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "pg_host.hpp"
@@ -60,6 +61,30 @@ struct LaunchRec {
     hipEvent_t beg, end;
 };
 
+struct KernelTotals {
+    double ms = 0, bytes = 0;
+    int64_t launches = 0;
+};
+
+struct GraphKey {
+    const pg_plan* plan;
+    int B, tsize;
+    const void* colors_in;  // caller colors buffer, nullptr = device RNG
+    const void* arena;
+    uint64_t generation;    // graph / weights / bucket version
+    bool timed;
+    bool operator<(const GraphKey& o) const
+    {
+        return std::tie(plan, B, tsize, colors_in, arena, generation, timed) <
+               std::tie(o.plan, o.B, o.tsize, o.colors_in, o.arena, o.generation, o.timed);
+    }
+};
+
+struct PassGraph {
+    hipGraphExec_t exec = nullptr;
+    std::vector<LaunchRec> recs;  // event pairs recorded by the graph's nodes
+};
+
 // One non-leaf subtemplate as the device schedule sees it.  Widths are per
 // coloring in color-relative rows (K = k - 1):
 //   Wo = C(K, size-1) output values, Wy = C(K, aa) aux sums,
@@ -73,6 +98,12 @@ struct DevSub {
     bool root, scaled;
     DevBuf split;  // [Wo][P] uint32 (main_rank << 16 | y_rank)
     DevBuf ymask;  // [Wy] uint16
+    // adjoint: the split list transposed by aux set and by main set, and the
+    // row masks of the aux child (sets of size aa-1)
+    int Pa = 0, Pm = 0;
+    DevBuf by_aux;   // [Wy][Pa] uint32 (o << 16 | main rank)
+    DevBuf by_main;  // [Wm][Pm] uint32 (o << 16 | y rank)
+    DevBuf amask;    // [Wa] uint16
 };
 
 }  // namespace
@@ -97,7 +128,11 @@ struct pg_ctx {
     bool has_graph = false, weighted = false;
     int32_t n = 0, n_act = 0, max_deg = 0;
     int64_t nnz = 0;
-    std::vector<int32_t> h_rowptr;
+    std::vector<int32_t> h_rowptr, h_col, h_perm;
+    // adjoint support, built on first use: row of each entry, the entry of
+    // the transposed pair, and weights in transposed order
+    bool has_transpose = false, symmetric = false;
+    DevBuf d_erow, d_tr, d_wT32, d_wT64, d_grad, d_grad_out;
     DevBuf d_rowptr, d_col, d_ent, d_perm, d_w32, d_w64, d_wsrc;
     // degree buckets and hub work items for the current options
     int64_t built_deg = -1, built_chunk = -1, built_high = -1, built_mid = -1;
@@ -106,8 +141,13 @@ struct pg_ctx {
     DevBuf d_hub_items, d_slot_begin;
     // per-call workspace
     DevBuf d_arena, d_colors_in;
+    // captured passes, and what invalidates them
+    bool use_graphs = true;
+    uint64_t generation = 0;
+    std::map<GraphKey, PassGraph> graphs;
     // timing of the last call
-    std::vector<LaunchRec> recs;
+    std::vector<LaunchRec>* cur_recs = nullptr;
+    std::map<std::string, KernelTotals> agg;
     double last_total_ms = 0;
     std::string dom_name;
     double dom_ms = 0, dom_bytes = 0;
@@ -184,6 +224,7 @@ int build_buckets(pg_ctx* c)
         HIP_TRY(c, hipMemcpy(c->d_hub_items.p, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice));
     HIP_TRY(c, hipMemcpy(c->d_slot_begin.p, slot_begin.data(), slot_begin.size() * sizeof(int32_t),
                          hipMemcpyHostToDevice));
+    c->generation++;  // captured passes hold the old bucket ranges
     c->built_deg = c->hub_degree;
     c->built_chunk = c->hub_chunk;
     c->built_high = c->bucket_high;
@@ -195,6 +236,7 @@ int build_buckets(pg_ctx* c)
 struct MemPlan {
     size_t total = 0;
     size_t off_H = 0, off_part = 0, off_root = 0, off_colors = 0, off_ecol = 0, off_rawdev = 0, off_tmp = 0;
+    size_t off_params = 0;
     std::vector<size_t> off_tab;  // per subtemplate index (tables of non-root non-leaf)
 };
 
@@ -212,13 +254,15 @@ MemPlan plan_memory(const pg_ctx* c, const pg_plan* p, int B, size_t tsz)
     // pass-long buffers: H, hub partial rows, root values, colors, results
     ivs.push_back({align_up(na * B * p->k * tsz, 256), -1, nsteps, &m.off_H});
     int maxY = p->k;
-    for (auto& d : p->subs) if (d->aa > 1) maxY = std::max(maxY, d->Wy);
+    for (auto& d : p->subs)
+        if (d->aa > 1) maxY = std::max(maxY, std::max(d->Wy, d->Wa));  // Wa: adjoint gathers
     ivs.push_back({align_up((size_t)std::max(c->n_slots, 1) * B * maxY * tsz, 256), -1, nsteps, &m.off_part});
     ivs.push_back({align_up(na * B * sizeof(double), 256), -1, nsteps, &m.off_root});
     ivs.push_back({align_up(na * B, 256), -1, nsteps, &m.off_colors});
     ivs.push_back({align_up((size_t)std::max<int64_t>(c->nnz, 1) * B, 256), -1, nsteps, &m.off_ecol});
     ivs.push_back({align_up((size_t)B * sizeof(double), 256), -1, nsteps, &m.off_rawdev});
     ivs.push_back({align_up((size_t)B * kReduceBlocks * sizeof(double), 256), -1, nsteps, &m.off_tmp});
+    ivs.push_back({align_up(sizeof(PassParams), 256), -1, nsteps, &m.off_params});
     // table of step i lives until its parent's step consumes it
     for (int i = 0; i < nsteps; ++i) {
         const DevSub* d = p->subs[i].get();
@@ -260,18 +304,43 @@ size_t sub_lds_max(const pg_plan* p, const DevSub& d, int B, size_t tsz)
 
 constexpr size_t kLdsLimit = 64 * 1024;
 
+// Timing: an event pair around each launch, appended to c->cur_recs (the
+// context's eager list, or the list owned by the graph being captured, whose
+// event-record nodes re-record on every replay).
 void record(pg_ctx* c, const std::string& name, double bytes, bool begin)
 {
-    if (!c->timing) return;
+    if (!c->timing || !c->cur_recs) return;
     if (begin) {
         LaunchRec r{name, bytes, nullptr, nullptr};
         (void)hipEventCreate(&r.beg);
         (void)hipEventCreate(&r.end);
         (void)hipEventRecord(r.beg, c->stream);
-        c->recs.push_back(r);
+        c->cur_recs->push_back(r);
     } else {
-        (void)hipEventRecord(c->recs.back().end, c->stream);
+        (void)hipEventRecord(c->cur_recs->back().end, c->stream);
     }
+}
+
+// adds the elapsed times of `recs` (all complete) to the per-kernel totals
+void accumulate_timing(pg_ctx* c, const std::vector<LaunchRec>& recs)
+{
+    for (auto& r : recs) {
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, r.beg, r.end) != hipSuccess) continue;
+        auto& a = c->agg[r.name];
+        a.ms += ms;
+        a.bytes += r.bytes;
+        a.launches += 1;
+    }
+}
+
+void destroy_recs(std::vector<LaunchRec>& recs)
+{
+    for (auto& r : recs) {
+        (void)hipEventDestroy(r.beg);
+        (void)hipEventDestroy(r.end);
+    }
+    recs.clear();
 }
 
 template <typename T>
@@ -291,7 +360,7 @@ std::string kname(int team, int aux, int main_mode)
 template <typename T>
 int launch_gathering(pg_ctx* c, int aux, int main_mode, SubArgs<T> a, T* part, int y_cols, double bytes)
 {
-    const int cl = cl_for(slots_for(a.B, aux == AUX_GATHER ? a.Wy : a.k));
+    const int cl = cl_for(slots_for(a.B, aux == AUX_COLORS ? a.k : a.Wy));
     const hipStream_t st = c->stream;
     const int cus = c->num_cus;
     const double units = (double)c->nnz + (double)c->n_act;
@@ -340,12 +409,15 @@ int launch_gathering(pg_ctx* c, int aux, int main_mode, SubArgs<T> a, T* part, i
     return PG_OK;
 }
 
-// Runs one pass of B colorings (first..first+B-1) through the whole DP.
+// Enqueues one pass of B colorings through the whole DP on c->stream.  The
+// pass's seed and first coloring are read on the device from mp.off_params,
+// so the same launches (captured once as a graph) serve every pass of this
+// shape.  Results land in mp.off_rawdev ([B] doubles, before the 1e12 factor).
 template <typename T>
-int run_pass(pg_ctx* c, const pg_plan* p, int B, int64_t first, const uint8_t* d_colors_in, uint64_t seed,
-             const MemPlan& mp, uint32_t flags, double* raw_host)
+int enqueue_pass(pg_ctx* c, const pg_plan* p, int B, const uint8_t* d_colors_in, const MemPlan& mp)
 {
     char* base = c->d_arena.as<char>();
+    const PassParams* pp = (const PassParams*)(base + mp.off_params);
     uint8_t* colors = (uint8_t*)(base + mp.off_colors);
     T* H = (T*)(base + mp.off_H);
     T* part = (T*)(base + mp.off_part);
@@ -361,10 +433,9 @@ int run_pass(pg_ctx* c, const pg_plan* p, int B, int64_t first, const uint8_t* d
     const double na = (double)c->n_act;
 
     if (d_colors_in)
-        HIP_TRY(c, launch_colors_gather(colors, d_colors_in + first * (int64_t)c->n, c->d_perm.as<int32_t>(),
-                                        c->n_act, B, c->n, st));
+        HIP_TRY(c, launch_colors_gather(colors, d_colors_in, c->d_perm.as<int32_t>(), c->n_act, B, c->n, pp, st));
     else
-        HIP_TRY(c, launch_colors_rng(colors, c->d_perm.as<int32_t>(), c->n_act, B, seed, first, k, st));
+        HIP_TRY(c, launch_colors_rng(colors, c->d_perm.as<int32_t>(), c->n_act, B, pp, k, st));
     uint8_t* ecol = (uint8_t*)(base + mp.off_ecol);
     HIP_TRY(c, launch_edge_colors(ecol, c->d_col.as<int32_t>(), colors, c->nnz, B, st));
 
@@ -441,21 +512,19 @@ int run_pass(pg_ctx* c, const pg_plan* p, int B, int64_t first, const uint8_t* d
         }
         if (d.root) HIP_TRY(c, launch_root_reduce(rootv, c->n_act, B, tmp, rawd, st));
     }
-    std::vector<double> raw(B);
-    HIP_TRY(c, hipMemcpyAsync(raw.data(), rawd, sizeof(double) * B, hipMemcpyDeviceToHost, st));
-    HIP_TRY(c, hipStreamSynchronize(st));
-    // CountMat.cpp:815-824: the root sum is multiplied by 1e12 unconditionally
-    const bool two_vertex = p->nonleaf == 1;
-    for (int b = 0; b < B; ++b) {
-        double r = raw[b];
-        if (sizeof(T) == 4) {
-            if (!(two_vertex && (flags & PG_FLAG_FIX_2VERTEX))) r *= 1.0e12;
-        } else if (two_vertex && !(flags & PG_FLAG_FIX_2VERTEX)) {
-            r *= 1.0e12;
-        }
-        raw_host[b] = r;
-    }
     return PG_OK;
+}
+
+// CountMat.cpp:815-824: the root sum is multiplied by 1e12 unconditionally
+// (it undoes the float(1e-12) of the scaled subtemplate; a template whose
+// only non-leaf is the root was never scaled, and the reference still
+// multiplies).  FP64 tables are unscaled: only that quirk is reproduced.
+double finish_raw(const pg_plan* p, bool fp32, uint32_t flags, double r)
+{
+    const bool two_vertex = p->nonleaf == 1;
+    const bool fix = (flags & PG_FLAG_FIX_2VERTEX) != 0;
+    if (fp32) return (two_vertex && fix) ? r : r * 1.0e12;
+    return (two_vertex && !fix) ? r * 1.0e12 : r;
 }
 
 int choose_tile(pg_ctx* c, const pg_plan* p, int ncol, size_t tsz, int* tile, MemPlan* mp)
@@ -494,27 +563,83 @@ void finish_timing(pg_ctx* c)
     c->dom_name.clear();
     c->dom_ms = c->dom_bytes = 0;
     c->dom_launches = 0;
-    if (!c->timing) return;
-    std::map<std::string, std::pair<double, std::pair<double, int64_t>>> agg;
-    for (auto& r : c->recs) {
-        float ms = 0;
-        (void)hipEventElapsedTime(&ms, r.beg, r.end);
-        auto& a = agg[r.name];
-        a.first += ms;
-        a.second.first += r.bytes;
-        a.second.second += 1;
-        (void)hipEventDestroy(r.beg);
-        (void)hipEventDestroy(r.end);
-    }
-    c->recs.clear();
-    for (auto& kv : agg) {
-        if (kv.second.first > c->dom_ms) {
+    for (auto& kv : c->agg) {
+        if (kv.second.ms > c->dom_ms) {
             c->dom_name = kv.first;
-            c->dom_ms = kv.second.first;
-            c->dom_bytes = kv.second.second.first;
-            c->dom_launches = kv.second.second.second;
+            c->dom_ms = kv.second.ms;
+            c->dom_bytes = kv.second.bytes;
+            c->dom_launches = kv.second.launches;
         }
     }
+}
+
+void drop_graphs(pg_ctx* c, const pg_plan* only = nullptr)
+{
+    for (auto it = c->graphs.begin(); it != c->graphs.end();) {
+        if (only && it->first.plan != only) {
+            ++it;
+            continue;
+        }
+        (void)hipGraphExecDestroy(it->second.exec);
+        destroy_recs(it->second.recs);
+        it = c->graphs.erase(it);
+    }
+}
+
+// One pass: through its cached graph when graphs are on (capturing it on
+// first use), else eagerly.  Writes B raw counts (before finish_raw).
+template <typename T>
+int run_pass(pg_ctx* c, const pg_plan* p, int B, int64_t first, const uint8_t* d_in, uint64_t seed,
+             const MemPlan& mp, double* raw)
+{
+    char* base = c->d_arena.as<char>();
+    const PassParams hp{seed, first};
+    // pageable source: the copy completes before the call returns
+    HIP_TRY(c, hipMemcpyAsync(base + mp.off_params, &hp, sizeof hp, hipMemcpyHostToDevice, c->stream));
+    if (!c->use_graphs) {
+        std::vector<LaunchRec> recs;
+        c->cur_recs = &recs;
+        int rc = enqueue_pass<T>(c, p, B, d_in, mp);
+        c->cur_recs = nullptr;
+        if (rc) {
+            destroy_recs(recs);
+            return rc;
+        }
+        HIP_TRY(c, hipMemcpyAsync(raw, base + mp.off_rawdev, sizeof(double) * B, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        accumulate_timing(c, recs);
+        destroy_recs(recs);
+        return PG_OK;
+    }
+    const GraphKey key{p, B, (int)sizeof(T), d_in, c->d_arena.p, c->generation, c->timing != 0};
+    auto it = c->graphs.find(key);
+    if (it == c->graphs.end()) {
+        PassGraph g;
+        c->cur_recs = &g.recs;
+        HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        int rc = enqueue_pass<T>(c, p, B, d_in, mp);
+        hipGraph_t graph = nullptr;
+        hipError_t e = hipStreamEndCapture(c->stream, &graph);
+        c->cur_recs = nullptr;
+        if (rc || e != hipSuccess) {
+            if (graph) (void)hipGraphDestroy(graph);
+            destroy_recs(g.recs);
+            if (rc) return rc;
+            return set_err(c, PG_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+        }
+        e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) {
+            destroy_recs(g.recs);
+            return set_err(c, PG_ERR_HIP, "graph instantiation failed: %s", hipGetErrorString(e));
+        }
+        it = c->graphs.emplace(key, std::move(g)).first;
+    }
+    HIP_TRY(c, hipGraphLaunch(it->second.exec, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(raw, base + mp.off_rawdev, sizeof(double) * B, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    accumulate_timing(c, it->second.recs);
+    return PG_OK;
 }
 
 template <typename T>
@@ -527,22 +652,333 @@ int count_impl(pg_ctx* c, const pg_plan* p, int ncol, const uint8_t* colors, uin
     MemPlan mp;
     rc = choose_tile(c, p, ncol, sizeof(T), &tile, &mp);
     if (rc) return rc;
-    HIP_TRY(c, c->d_arena.alloc(mp.total));
+    if (mp.total > c->d_arena.bytes) {
+        drop_graphs(c);  // graphs hold arena addresses
+        HIP_TRY(c, c->d_arena.alloc(mp.total));
+    }
     const uint8_t* d_in = nullptr;
     if (colors) {
-        HIP_TRY(c, c->d_colors_in.alloc((size_t)ncol * c->n));
+        if ((size_t)ncol * c->n > c->d_colors_in.bytes) {
+            drop_graphs(c);
+            HIP_TRY(c, c->d_colors_in.alloc((size_t)ncol * c->n));
+        }
         HIP_TRY(c, hipMemcpyAsync(c->d_colors_in.p, colors, (size_t)ncol * c->n, hipMemcpyHostToDevice, c->stream));
         d_in = c->d_colors_in.as<uint8_t>();
     }
+    c->agg.clear();
     hipEvent_t t0 = nullptr, t1 = nullptr;
     HIP_TRY(c, hipEventCreate(&t0));
     HIP_TRY(c, hipEventCreate(&t1));
     HIP_TRY(c, hipEventRecord(t0, c->stream));
+    std::vector<double> raw;
     for (int first = 0; first < ncol; first += tile) {
         const int B = std::min(tile, ncol - first);
         const MemPlan m = B == tile ? mp : plan_memory(c, p, B, sizeof(T));
-        rc = run_pass<T>(c, p, B, first, d_in, seed, m, flags, raw_out + first);
+        raw.assign(B, 0.0);
+        rc = run_pass<T>(c, p, B, first, d_in, seed, m, raw.data());
         if (rc) break;
+        for (int b = 0; b < B; ++b) raw_out[first + b] = finish_raw(p, sizeof(T) == 4, flags, raw[b]);
+    }
+    (void)hipEventRecord(t1, c->stream);
+    (void)hipEventSynchronize(t1);
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, t0, t1);
+    c->last_total_ms = ms;
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    finish_timing(c);
+    return rc;
+}
+
+// ---------------------------------------------------------------- adjoint
+
+// Row of every entry and, for entry (row v, column u), the index of its
+// transposed entry (row u, column v): the k-th duplicate pairs with the k-th.
+// Rows list columns in ascending caller id, so the partner is a binary search.
+int build_transpose(pg_ctx* c)
+{
+    if (c->has_transpose) return PG_OK;
+    const int64_t nnz = c->nnz;
+    std::vector<int32_t> erow(std::max<int64_t>(nnz, 1)), tr(std::max<int64_t>(nnz, 1), -1);
+    const auto& rp = c->h_rowptr;
+    const auto& col = c->h_col;
+    const auto& perm = c->h_perm;
+    int bad = 0;
+#pragma omp parallel for schedule(dynamic, 1024) reduction(+ : bad)
+    for (int32_t v = 0; v < c->n_act; ++v) {
+        const int32_t t = perm[v];
+        for (int32_t e = rp[v]; e < rp[v + 1]; ++e) {
+            erow[e] = v;
+            const int32_t u = col[e];
+            const int32_t cu = perm[u];
+            int32_t dup = 0;
+            while (e - dup - 1 >= rp[v] && perm[col[e - dup - 1]] == cu) ++dup;
+            int32_t lo = rp[u], hi = rp[u + 1];
+            while (lo < hi) {
+                const int32_t mid = lo + (hi - lo) / 2;
+                if (perm[col[mid]] < t) lo = mid + 1;
+                else hi = mid;
+            }
+            const int32_t p = lo + dup;
+            if (p < rp[u + 1] && perm[col[p]] == t) tr[e] = p;
+            else ++bad;
+        }
+    }
+    c->symmetric = bad == 0;
+    HIP_TRY(c, c->d_erow.alloc(erow.size() * 4));
+    HIP_TRY(c, c->d_tr.alloc(tr.size() * 4));
+    HIP_TRY(c, hipMemcpy(c->d_erow.p, erow.data(), erow.size() * 4, hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(c->d_tr.p, tr.data(), tr.size() * 4, hipMemcpyHostToDevice));
+    c->has_transpose = true;
+    return PG_OK;
+}
+
+// Tables kept for the adjoint: every forward table lives through the whole
+// pass, plus the adjoint tables G_s, recomputed Y and dY.
+struct GradPlan {
+    MemPlan fwd;
+    std::vector<size_t> off_G;
+    size_t off_Y = 0, off_dY = 0, off_cot = 0, total = 0;
+};
+
+GradPlan plan_memory_grad(const pg_ctx* c, const pg_plan* p, int B, size_t tsz)
+{
+    GradPlan g;
+    g.fwd = plan_memory(c, p, B, tsz);
+    // forward plan reuses table space; give every table its own slot instead
+    const size_t na = (size_t)c->n_act;
+    size_t off = g.fwd.total;
+    for (auto& d : p->subs) {
+        if (d->root) continue;
+        g.fwd.off_tab[d->s] = off;
+        off += align_up(na * B * d->Wo * tsz, 256);
+    }
+    g.off_G.assign(p->nsub, 0);
+    for (auto& d : p->subs) {
+        if (d->root) continue;
+        g.off_G[d->s] = off;
+        off += align_up(na * B * d->Wo * tsz, 256);
+    }
+    int maxY = 1;
+    for (auto& d : p->subs) maxY = std::max(maxY, d->Wy);
+    g.off_Y = off;
+    off += align_up(na * B * maxY * tsz, 256);
+    g.off_dY = off;
+    off += align_up(na * B * maxY * tsz, 256);
+    g.off_cot = off;
+    off += align_up((size_t)B * sizeof(double), 256);
+    g.total = off;
+    g.fwd.total = off;
+    return g;
+}
+
+// Reverse sweep of one pass (root first): for each non-leaf s, recompute Y,
+// form dY and G_main, add dY x T_aux to every entry's gradient, and pull
+// G_aux from the neighbours' dY (the transposed SpMM).
+template <typename T>
+int enqueue_backward(pg_ctx* c, const pg_plan* p, int B, const GradPlan& gp)
+{
+    char* base = c->d_arena.as<char>();
+    const MemPlan& mp = gp.fwd;
+    const uint8_t* colors = (const uint8_t*)(base + mp.off_colors);
+    const uint8_t* ecol = (const uint8_t*)(base + mp.off_ecol);
+    const T* H = (const T*)(base + mp.off_H);
+    T* part = (T*)(base + mp.off_part);
+    T* Ybuf = (T*)(base + gp.off_Y);
+    T* dYbuf = (T*)(base + gp.off_dY);
+    const hipStream_t st = c->stream;
+    const int k = p->k;
+    const bool fp32 = sizeof(T) == 4;
+    const T* w = nullptr;
+    const T* wT = nullptr;
+    if (c->weighted) {
+        w = fp32 ? (const T*)c->d_w32.p : (const T*)c->d_w64.p;
+        wT = fp32 ? (const T*)c->d_wT32.p : (const T*)c->d_wT64.p;
+    }
+    SubArgs<T> base_args{};
+    base_args.rowptr = c->d_rowptr.as<int32_t>();
+    base_args.col = c->d_col.as<int32_t>();
+    base_args.colors = colors;
+    base_args.ecol = ecol;
+    base_args.B = B;
+    base_args.k = k;
+    base_args.K = p->K;
+    base_args.ng = 1;
+    base_args.rank_tab = p->rank_tab.as<int16_t>();
+    base_args.slot_begin = c->d_slot_begin.as<int32_t>();
+    base_args.out_mode = OUT_Y;
+    for (auto it = p->subs.rbegin(); it != p->subs.rend(); ++it) {
+        const DevSub& d = **it;
+        const T* Y = H;
+        int y_from_h = 1;
+        if (d.aa > 1) {
+            SubArgs<T> a = base_args;
+            a.w = w;
+            a.aux = (const T*)(base + mp.off_tab[d.ai]);
+            a.Wa = d.Wa;
+            a.lda = (int64_t)B * d.Wa;
+            a.Wy = d.Wy;
+            a.Wo = d.Wy;
+            a.ymask = d.ymask.as<uint16_t>();
+            a.out = Ybuf;
+            a.ldo = (int64_t)B * d.Wy;
+            int rc = launch_gathering<T>(c, AUX_GATHER, MAIN_LEAF, a, part, B * d.Wy, 0.0);
+            if (rc) return rc;
+            Y = Ybuf;
+            y_from_h = 0;
+        }
+        NodeArgs<T> na{};
+        na.n_act = c->n_act;
+        na.B = B;
+        na.k = k;
+        na.colors = colors;
+        na.G = d.root ? nullptr : (const T*)(base + gp.off_G[d.s]);
+        na.root_cot = (const double*)(base + gp.off_cot);
+        na.Tm = d.mm > 1 ? (const T*)(base + mp.off_tab[d.mi]) : nullptr;
+        na.Y = Y;
+        na.y_from_h = y_from_h;
+        na.dY = dYbuf;
+        na.Gm = d.mm > 1 ? (T*)(base + gp.off_G[d.mi]) : nullptr;
+        na.Wo = d.Wo;
+        na.Wy = d.Wy;
+        na.Wm = d.Wm;
+        na.Pa = d.Pa;
+        na.Pm = d.Pm;
+        na.by_aux = d.by_aux.as<uint32_t>();
+        na.by_main = d.by_main.as<uint32_t>();
+        na.scale = (d.scaled && fp32) ? (double)1.0e-12f : 1.0;
+        HIP_TRY(c, launch_bwd_node<T>(na, c->num_cus, st));
+        EdgeGradArgs<T> eg{};
+        eg.nnz = c->nnz;
+        eg.B = B;
+        eg.k = k;
+        eg.K = p->K;
+        eg.erow = c->d_erow.as<int32_t>();
+        eg.col = c->d_col.as<int32_t>();
+        eg.colors = colors;
+        eg.ecol = ecol;
+        eg.dY = dYbuf;
+        eg.Wy = d.Wy;
+        eg.Ta = d.aa > 1 ? (const T*)(base + mp.off_tab[d.ai]) : nullptr;
+        eg.Wa = d.Wa;
+        eg.ymask = d.ymask.as<uint16_t>();
+        eg.rank_tab = p->rank_tab.as<int16_t>();
+        eg.grad = c->d_grad.as<double>();
+        HIP_TRY(c, launch_edge_grad<T>(eg, st));
+        if (d.aa > 1) {
+            SubArgs<T> a = base_args;
+            a.w = wT;
+            a.aux = dYbuf;
+            a.lda = (int64_t)B * d.Wy;
+            a.Wa = d.Wy;  // per-coloring stride of the neighbours' dY rows
+            a.Wy = d.Wa;  // values per coloring: the aux child's row
+            a.Wo = d.Wa;
+            a.ymask = d.amask.as<uint16_t>();
+            a.out = (T*)(base + gp.off_G[d.ai]);
+            a.ldo = (int64_t)B * d.Wa;
+            int rc = launch_gathering<T>(c, AUX_GATHER_T, MAIN_LEAF, a, part, B * d.Wa, 0.0);
+            if (rc) return rc;
+        }
+    }
+    return PG_OK;
+}
+
+template <typename T>
+int count_grad_impl(pg_ctx* c, const pg_plan* p, int ncol, const uint8_t* colors, uint64_t seed, uint32_t flags,
+                    const double* cotangent, double* raw_out, void* grad, int grad_on_device)
+{
+    int rc = build_buckets(c);
+    if (rc) return rc;
+    rc = build_transpose(c);
+    if (rc) return rc;
+    if (!c->symmetric)
+        return set_err(c, PG_ERR_INVALID, "gradients need a symmetric pattern (every entry's transpose present)");
+    const size_t tsz = sizeof(T);
+    size_t budget;
+    if (c->mem_budget_mb > 0) {
+        budget = (size_t)c->mem_budget_mb << 20;
+    } else {
+        size_t fr = 0, tot = 0;
+        HIP_TRY(c, hipMemGetInfo(&fr, &tot));
+        budget = (size_t)(0.8 * (double)(fr + c->d_arena.bytes));
+    }
+    int tile = 0;
+    GradPlan gp;
+    const int cap = c->batch_tile > 0 ? (int)std::min<int64_t>(c->batch_tile, ncol) : std::min(ncol, 64);
+    for (int B = cap; B >= 1; --B) {
+        size_t lds = 0;
+        for (auto& d : p->subs) lds = std::max(lds, sub_lds_max(p, *d, B, tsz));
+        if (lds > kLdsLimit) continue;
+        GradPlan g = plan_memory_grad(c, p, B, tsz);
+        if (g.total > budget) continue;
+        tile = B;
+        gp = g;
+        break;
+    }
+    if (tile == 0) return set_err(c, PG_ERR_NOMEM, "adjoint tables of one coloring do not fit the device");
+    if (gp.total > c->d_arena.bytes) {
+        drop_graphs(c);
+        HIP_TRY(c, c->d_arena.alloc(gp.total));
+    }
+    if (c->weighted) {
+        HIP_TRY(c, c->d_wT32.alloc(c->nnz * 4));
+        HIP_TRY(c, c->d_wT64.alloc(c->nnz * 8));
+        HIP_TRY(c, launch_gather_weights_f32(c->d_wT32.as<float>(), c->d_w32.as<float>(), c->d_tr.as<int32_t>(),
+                                             c->nnz, c->stream));
+        HIP_TRY(c, launch_gather_weights_f64(c->d_wT64.as<double>(), c->d_w32.as<float>(), c->d_tr.as<int32_t>(),
+                                             c->nnz, c->stream));
+    }
+    HIP_TRY(c, c->d_grad.alloc(std::max<int64_t>(c->nnz, 1) * 8));
+    HIP_TRY(c, launch_zero_f64(c->d_grad.as<double>(), c->nnz, c->stream));
+    const uint8_t* d_in = nullptr;
+    if (colors) {
+        if ((size_t)ncol * c->n > c->d_colors_in.bytes) {
+            drop_graphs(c);
+            HIP_TRY(c, c->d_colors_in.alloc((size_t)ncol * c->n));
+        }
+        HIP_TRY(c, hipMemcpyAsync(c->d_colors_in.p, colors, (size_t)ncol * c->n, hipMemcpyHostToDevice, c->stream));
+        d_in = c->d_colors_in.as<uint8_t>();
+    }
+    c->agg.clear();
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    HIP_TRY(c, hipEventCreate(&t0));
+    HIP_TRY(c, hipEventCreate(&t1));
+    HIP_TRY(c, hipEventRecord(t0, c->stream));
+    char* base = c->d_arena.as<char>();
+    std::vector<double> raw;
+    for (int first = 0; first < ncol && rc == PG_OK; first += tile) {
+        const int B = std::min(tile, ncol - first);
+        const GradPlan g = B == tile ? gp : plan_memory_grad(c, p, B, tsz);
+        const PassParams hp{seed, first};
+        HIP_TRY(c, hipMemcpyAsync(base + g.fwd.off_params, &hp, sizeof hp, hipMemcpyHostToDevice, c->stream));
+        std::vector<double> cot(B, 1.0);
+        if (cotangent) std::copy(cotangent + first, cotangent + first + B, cot.begin());
+        HIP_TRY(c, hipMemcpyAsync(base + g.off_cot, cot.data(), sizeof(double) * B, hipMemcpyHostToDevice, c->stream));
+        rc = enqueue_pass<T>(c, p, B, d_in, g.fwd);
+        if (rc) break;
+        rc = enqueue_backward<T>(c, p, B, g);
+        if (rc) break;
+        raw.assign(B, 0.0);
+        HIP_TRY(c, hipMemcpyAsync(raw.data(), base + g.fwd.off_rawdev, sizeof(double) * B, hipMemcpyDeviceToHost,
+                                  c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        for (int b = 0; b < B; ++b) raw_out[first + b] = finish_raw(p, sizeof(T) == 4, flags, raw[b]);
+    }
+    if (rc == PG_OK && grad) {
+        const double factor = finish_raw(p, sizeof(T) == 4, flags, 1.0);
+        if (grad_on_device) {
+            if (sizeof(T) == 4)
+                HIP_TRY(c, launch_scatter_grad_f32((float*)grad, c->d_grad.as<double>(), c->d_ent.as<int32_t>(),
+                                                   c->nnz, factor, c->stream));
+            else
+                HIP_TRY(c, launch_scatter_grad_f64((double*)grad, c->d_grad.as<double>(), c->d_ent.as<int32_t>(),
+                                                   c->nnz, factor, c->stream));
+        } else {
+            HIP_TRY(c, c->d_grad_out.alloc(std::max<int64_t>(c->nnz, 1) * 8));
+            HIP_TRY(c, launch_scatter_grad_f64(c->d_grad_out.as<double>(), c->d_grad.as<double>(),
+                                               c->d_ent.as<int32_t>(), c->nnz, factor, c->stream));
+            HIP_TRY(c, hipMemcpyAsync(grad, c->d_grad_out.p, c->nnz * 8, hipMemcpyDeviceToHost, c->stream));
+        }
     }
     (void)hipEventRecord(t1, c->stream);
     (void)hipEventSynchronize(t1);
@@ -606,7 +1042,7 @@ extern "C" void pg_destroy(pg_ctx* c)
     if (!c) return;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
-    for (auto& r : c->recs) { (void)hipEventDestroy(r.beg); (void)hipEventDestroy(r.end); }
+    drop_graphs(c);
     (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -637,6 +1073,8 @@ extern "C" int pg_set_option(pg_ctx* c, const char* key, int64_t v)
         c->mem_budget_mb = v;
     } else if (k == "timing") {
         c->timing = v ? 1 : 0;
+    } else if (k == "graphs") {
+        c->use_graphs = v != 0;
     } else if (k == "autotune") {
         c->autotune = v ? 1 : 0;
     } else {
@@ -657,6 +1095,7 @@ extern "C" int pg_get_option(const pg_ctx* c, const char* key, int64_t* v)
     else if (k == "mem_budget_mb") *v = c->mem_budget_mb;
     else if (k == "timing") *v = c->timing;
     else if (k == "autotune") *v = c->autotune;
+    else if (k == "graphs") *v = c->use_graphs ? 1 : 0;
     else return PG_ERR_INVALID;
     return PG_OK;
 }
@@ -728,6 +1167,9 @@ extern "C" int pg_graph_upload(pg_ctx* c, int32_t n, int64_t nnz, const int32_t*
     c->n_act = na;
     c->max_deg = na > 0 ? rdeg[perm[0]] : 0;
     c->h_rowptr = hrow;
+    c->h_col = hcol;
+    c->h_perm = perm;
+    c->has_transpose = false;
     HIP_TRY(c, c->d_rowptr.alloc(hrow.size() * 4));
     HIP_TRY(c, c->d_col.alloc(std::max<size_t>(hcol.size(), 1) * 4));
     HIP_TRY(c, c->d_ent.alloc(std::max<size_t>(hent.size(), 1) * 4));
@@ -739,6 +1181,8 @@ extern "C" int pg_graph_upload(pg_ctx* c, int32_t n, int64_t nnz, const int32_t*
     }
     if (na > 0) HIP_TRY(c, hipMemcpy(c->d_perm.p, perm.data(), perm.size() * 4, hipMemcpyHostToDevice));
     c->built_deg = -1;
+    drop_graphs(c);
+    c->generation++;
     c->weighted = false;
     c->has_graph = true;
     int rc = build_buckets(c);
@@ -754,6 +1198,7 @@ extern "C" int pg_graph_set_weights(pg_ctx* c, const float* vals, int on_device)
     HIP_TRY(c, hipSetDevice(c->device));
     if (!vals) {
         c->weighted = false;
+        c->generation++;
         return PG_OK;
     }
     const float* src = vals;
@@ -768,6 +1213,7 @@ extern "C" int pg_graph_set_weights(pg_ctx* c, const float* vals, int on_device)
     HIP_TRY(c, launch_gather_weights_f64(c->d_w64.as<double>(), src, c->d_ent.as<int32_t>(), c->nnz, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     c->weighted = true;
+    c->generation++;
     return PG_OK;
 }
 
@@ -935,11 +1381,38 @@ extern "C" int pg_plan_create(pg_ctx* c, int k, int nsub, const int32_t* sub_siz
             }
             HIP_TRY(c, d->split.alloc(tab.size() * 4));
             HIP_TRY(c, hipMemcpy(d->split.p, tab.data(), tab.size() * 4, hipMemcpyHostToDevice));
+            // adjoint lists: every (output, split) pair filed under its aux
+            // set and under its main set, in output order (equal lengths)
+            std::vector<std::vector<uint32_t>> by_a(d->Wy), by_m(d->Wm);
+            for (int o = 0; o < d->Wo; ++o)
+                for (int j = 0; j < d->P; ++j) {
+                    const uint32_t e = tab[(size_t)o * d->P + j];
+                    by_a[e & 0xffffu].push_back(((uint32_t)o << 16) | (e >> 16));
+                    by_m[e >> 16].push_back(((uint32_t)o << 16) | (e & 0xffffu));
+                }
+            d->Pa = (int)by_a[0].size();
+            d->Pm = (int)by_m[0].size();
+            std::vector<uint32_t> fa, fm;
+            for (auto& l : by_a) {
+                if ((int)l.size() != d->Pa) return set_err(c, PG_ERR_INVALID, "uneven adjoint split lists");
+                fa.insert(fa.end(), l.begin(), l.end());
+            }
+            for (auto& l : by_m) {
+                if ((int)l.size() != d->Pm) return set_err(c, PG_ERR_INVALID, "uneven adjoint split lists");
+                fm.insert(fm.end(), l.begin(), l.end());
+            }
+            HIP_TRY(c, d->by_aux.alloc(fa.size() * 4));
+            HIP_TRY(c, hipMemcpy(d->by_aux.p, fa.data(), fa.size() * 4, hipMemcpyHostToDevice));
+            HIP_TRY(c, d->by_main.alloc(fm.size() * 4));
+            HIP_TRY(c, hipMemcpy(d->by_main.p, fm.data(), fm.size() * 4, hipMemcpyHostToDevice));
         }
         if (d->aa > 1) {
             const std::vector<uint16_t> ym = masks_of_size(K, d->aa);
             HIP_TRY(c, d->ymask.alloc(ym.size() * 2));
             HIP_TRY(c, hipMemcpy(d->ymask.p, ym.data(), ym.size() * 2, hipMemcpyHostToDevice));
+            const std::vector<uint16_t> am = masks_of_size(K, d->aa - 1);
+            HIP_TRY(c, d->amask.alloc(am.size() * 2));
+            HIP_TRY(c, hipMemcpy(d->amask.p, am.data(), am.size() * 2, hipMemcpyHostToDevice));
         }
         plan->subs.push_back(std::move(d));
     }
@@ -970,6 +1443,7 @@ extern "C" void pg_plan_destroy(pg_plan* p)
 {
     if (!p) return;
     (void)hipSetDevice(p->device);
+    if (p->ctx) drop_graphs(p->ctx, p);
     delete p;
 }
 
@@ -1024,10 +1498,31 @@ extern "C" int pg_count_multi(pg_ctx* c, int nplans, const pg_plan* const* plans
     return PG_OK;
 }
 
-extern "C" int pg_count_grad(pg_ctx* c, const pg_plan*, int, const uint8_t*, uint64_t, int, uint32_t, double*, void*,
-                             int)
+extern "C" int pg_count_grad(pg_ctx* c, const pg_plan* p, int ncol, const uint8_t* colors, uint64_t seed,
+                             int precision, uint32_t flags, const double* cotangent, double* raw_out, void* grad,
+                             int grad_on_device)
 {
-    return set_err(c, PG_ERR_STATE, "pg_count_grad: not built yet");
+    if (!c) return PG_ERR_INVALID;
+    if (!p || p->ctx != c || ncol <= 0 || !raw_out)
+        return set_err(c, PG_ERR_INVALID, "invalid gradient arguments");
+    if (precision != PG_FP32 && precision != PG_FP64)
+        return set_err(c, PG_ERR_INVALID, "precision must be PG_FP32 or PG_FP64");
+    if (!c->has_graph) return set_err(c, PG_ERR_STATE, "no graph uploaded");
+    if (colors) {
+        const size_t total = (size_t)ncol * c->n;
+        for (size_t i = 0; i < total; ++i)
+            if (colors[i] >= p->k) return set_err(c, PG_ERR_INVALID, "color %d >= k at %zu", colors[i], i);
+    }
+    HIP_TRY(c, hipSetDevice(c->device));
+    if (c->n_act == 0) {
+        for (int i = 0; i < ncol; ++i) raw_out[i] = 0.0;
+        if (grad && !grad_on_device)
+            for (int64_t e = 0; e < c->nnz; ++e) ((double*)grad)[e] = 0.0;
+        return PG_OK;
+    }
+    return precision == PG_FP32
+               ? count_grad_impl<float>(c, p, ncol, colors, seed, flags, cotangent, raw_out, grad, grad_on_device)
+               : count_grad_impl<double>(c, p, ncol, colors, seed, flags, cotangent, raw_out, grad, grad_on_device);
 }
 
 extern "C" int pg_last_timing(const pg_ctx* c, double* total_ms, double* spmm_ms, int64_t* spmm_launches,

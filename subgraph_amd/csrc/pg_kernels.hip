@@ -49,8 +49,10 @@ __device__ __forceinline__ uint32_t squeeze_bit(uint32_t m, uint32_t c)
     return (m & ((1u << c) - 1u)) | ((m >> (c + 1u)) << c);
 }
 
-// one neighbour's contribution to a Y value: GATHER reads the neighbour's
+// one neighbour's contribution to a value: GATHER reads the neighbour's
 // color-relative row at the rank of (Y set minus the neighbour's color);
+// GATHER_T reads the neighbour's dY at the rank of the vertex's row set in
+// the neighbour's frame, when the neighbour's color is outside the set;
 // COLORS is the one-hot leaf (amask holds the value's color)
 template <typename T, int AUX>
 __device__ __forceinline__ T aux_term(const SubArgs<T>& a, const int16_t* s_rank, uint32_t amask, int aoff, int u,
@@ -63,10 +65,19 @@ __device__ __forceinline__ T aux_term(const SubArgs<T>& a, const int16_t* s_rank
         }
         return T(0);
     }
+    if (AUX == AUX_GATHER_T) {
+        if (!((amask >> cu) & 1u)) {
+            const int idx = s_rank[squeeze_bit(amask, cu)];
+            return a.aux[(int64_t)u * a.lda + aoff + idx];
+        }
+        return T(0);
+    }
     return cu == amask ? T(1) : T(0);
 }
 
 __host__ __device__ constexpr size_t rank_lds_bytes(int K) { return (((size_t)2 << K) + 15) & ~(size_t)15; }
+
+__host__ __device__ constexpr bool gathers_rows(int aux) { return aux == AUX_GATHER || aux == AUX_GATHER_T; }
 
 }  // namespace
 
@@ -74,7 +85,7 @@ template <typename T>
 size_t sub_lds_bytes(int team, int aux, int main_mode, const SubArgs<T>& a)
 {
     const int teams = kBlock / team;
-    size_t bytes = aux == AUX_GATHER ? rank_lds_bytes(a.K) : 0;
+    size_t bytes = gathers_rows(aux) ? rank_lds_bytes(a.K) : 0;
     if (a.out_mode == OUT_Y || a.out_mode == OUT_PARTIAL) return bytes;
     const size_t per = (size_t)a.B * a.Wy + (main_mode == MAIN_TABLE ? (size_t)a.B * a.Wm : 0);
     return bytes + (size_t)teams * per * sizeof(T);
@@ -94,12 +105,12 @@ __global__ __launch_bounds__(kBlock) void k_sub(const SubArgs<T> a)
     const int mode = a.out_mode;
     const int nY = B * a.Wy;
     const int nM = (MAIN == MAIN_TABLE) ? B * a.Wm : 0;
-    const size_t rank_bytes = (AUX == AUX_GATHER) ? rank_lds_bytes(a.K) : 0;
+    const size_t rank_bytes = gathers_rows(AUX) ? rank_lds_bytes(a.K) : 0;
     int16_t* s_rank = reinterpret_cast<int16_t*>(smem_raw);
     T* sY = reinterpret_cast<T*>(smem_raw + rank_bytes) + (size_t)team * (nY + nM);
     T* sM = sY + nY;
 
-    if (AUX == AUX_GATHER) {
+    if (gathers_rows(AUX)) {
         for (int i = threadIdx.x; i < (1 << a.K); i += kBlock) s_rank[i] = a.rank_tab[i];
         __syncthreads();
     }
@@ -119,15 +130,15 @@ __global__ __launch_bounds__(kBlock) void k_sub(const SubArgs<T> a)
                 v = it.x; e0 = it.y; e1 = it.z; slot = it.w;
             } else {
                 v = a.item_base + (int)item;
-                if (AUX == AUX_GATHER || AUX == AUX_COLORS) { e0 = a.rowptr[v]; e1 = a.rowptr[v + 1]; }
+                if (gathers_rows(AUX) || AUX == AUX_COLORS) { e0 = a.rowptr[v]; e1 = a.rowptr[v + 1]; }
             }
         }
 
         // ------------------------------------------------------ phase 1: Y
-        if (AUX == AUX_GATHER || AUX == AUX_COLORS) {
+        if (gathers_rows(AUX) || AUX == AUX_COLORS) {
             // a slot is kRegCols consecutive Y values of one coloring; column
             // lane cln of every group owns slot s0 + cln of each chunk
-            const int W = (AUX == AUX_GATHER) ? a.Wy : a.k;
+            const int W = gathers_rows(AUX) ? a.Wy : a.k;
             const int spb = (W + kRegCols - 1) / kRegCols;
             const int nslots = B * spb;
             for (int s0 = 0; s0 < nslots; s0 += cl) {
@@ -136,20 +147,24 @@ __global__ __launch_bounds__(kBlock) void k_sub(const SubArgs<T> a)
                 const int b = sv ? sid / spb : 0;
                 const int i0 = (sid - b * spb) * kRegCols;
                 T acc[kRegCols];
-                uint32_t amask[kRegCols];  // GATHER: k-bit color set of the value (0 = unused); COLORS: its color
-                const uint32_t cv = sv && AUX == AUX_GATHER ? a.colors[(int64_t)v * B + b] : 0u;
+                // k-bit color set of each value (GATHER: avoids col(v), 0 = unused;
+                // GATHER_T: holds col(v), all ones = unused); COLORS: the value's color
+                uint32_t amask[kRegCols];
+                const uint32_t cv = sv && gathers_rows(AUX) ? a.colors[(int64_t)v * B + b] : 0u;
 #pragma unroll
                 for (int r = 0; r < kRegCols; ++r) {
                     acc[r] = T(0);
                     const int i = i0 + r;
                     const bool ok = sv && i < W;
                     if (AUX == AUX_GATHER) amask[r] = ok ? expand_bit(a.ymask[i], cv) : 0u;
+                    else if (AUX == AUX_GATHER_T) amask[r] = ok ? expand_bit(a.ymask[i], cv) | (1u << cv) : ~0u;
                     else amask[r] = ok ? (uint32_t)i : 0xffu;
                 }
                 const int aoff = b * a.Wa;
                 // a neighbour past the end reads as the vertex itself with a
-                // color no value uses (GATHER: col(v) is never in a Y set)
-                const uint32_t dead = AUX == AUX_GATHER ? cv : 0xffu;
+                // color that adds nothing (col(v) is never in a GATHER set and
+                // always in a GATHER_T set)
+                const uint32_t dead = gathers_rows(AUX) ? cv : 0xffu;
                 if (active) {
                     const int step = ng;
                     int e = e0 + grp;
@@ -329,6 +344,7 @@ static hipError_t launch_team(int aux, int main_mode, const SubArgs<T>& a, int m
     if (main_mode == MAIN_LEAF) {
         switch (aux) {
         case AUX_GATHER: return launch_one<T, TEAM, AUX_GATHER, MAIN_LEAF>(a, max_blocks, st);
+        case AUX_GATHER_T: return launch_one<T, TEAM, AUX_GATHER_T, MAIN_LEAF>(a, max_blocks, st);
         case AUX_COLORS: return launch_one<T, TEAM, AUX_COLORS, MAIN_LEAF>(a, max_blocks, st);
         case AUX_READ: return launch_one<T, TEAM, AUX_READ, MAIN_LEAF>(a, max_blocks, st);
         case AUX_PARTIALS: return launch_one<T, TEAM, AUX_PARTIALS, MAIN_LEAF>(a, max_blocks, st);
@@ -348,7 +364,7 @@ hipError_t launch_sub(int team, int aux, int main_mode, const SubArgs<T>& a, int
 {
     if (a.n_items <= 0) return hipSuccess;
     if (a.ng < 1 || team % a.ng != 0 || (a.ng & (a.ng - 1)) != 0) return hipErrorInvalidValue;
-    if (aux == AUX_GATHER && (a.K < 1 || a.K > kMaxK - 1)) return hipErrorInvalidValue;
+    if (gathers_rows(aux) && (a.K < 1 || a.K > kMaxK - 1)) return hipErrorInvalidValue;
     switch (team) {
     case 8: return launch_team<T, 8>(aux, main_mode, a, max_blocks, st);
     case 16: return launch_team<T, 16>(aux, main_mode, a, max_blocks, st);
@@ -363,45 +379,44 @@ template hipError_t launch_sub<double>(int, int, int, const SubArgs<double>&, in
 
 // ------------------------------------------------------------ utilities ---
 
-__global__ void k_colors_rng(uint8_t* out, const int32_t* perm, int32_t n_act, int B, uint64_t seed,
-                             int64_t first, int k)
+__global__ void k_colors_rng(uint8_t* out, const int32_t* perm, int32_t n_act, int B, const PassParams* pp, int k)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (int64_t)n_act * B) return;
     const int64_t v = i / B;
     const int b = (int)(i - v * B);
-    const uint64_t h = d_splitmix64(seed ^ d_splitmix64((uint64_t)(first + b) * 0x9E3779B97F4A7C15ull +
-                                                         (uint64_t)perm[v]));
+    const uint64_t h = d_splitmix64(pp->seed ^ d_splitmix64((uint64_t)(pp->first + b) * 0x9E3779B97F4A7C15ull +
+                                                             (uint64_t)perm[v]));
     out[i] = (uint8_t)(((h >> 32) * (uint64_t)k) >> 32);
 }
 
-hipError_t launch_colors_rng(uint8_t* out, const int32_t* perm, int32_t n_act, int B, uint64_t seed,
-                             int64_t first, int k, hipStream_t st)
+hipError_t launch_colors_rng(uint8_t* out, const int32_t* perm, int32_t n_act, int B, const PassParams* pp, int k,
+                             hipStream_t st)
 {
     const int64_t total = (int64_t)n_act * B;
     if (total == 0) return hipSuccess;
     hipLaunchKernelGGL(k_colors_rng, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, out, perm, n_act, B,
-                       seed, first, k);
+                       pp, k);
     return hipGetLastError();
 }
 
-__global__ void k_colors_gather(uint8_t* out, const uint8_t* in, const int32_t* perm, int32_t n_act, int B,
-                                int64_t n)
+__global__ void k_colors_gather(uint8_t* out, const uint8_t* in, const int32_t* perm, int32_t n_act, int B, int64_t n,
+                                const PassParams* pp)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (int64_t)n_act * B) return;
     const int64_t v = i / B;
     const int b = (int)(i - v * B);
-    out[i] = in[(int64_t)b * n + perm[v]];
+    out[i] = in[(pp->first + b) * n + perm[v]];
 }
 
 hipError_t launch_colors_gather(uint8_t* out, const uint8_t* in, const int32_t* perm, int32_t n_act, int B,
-                                int64_t n, hipStream_t st)
+                                int64_t n, const PassParams* pp, hipStream_t st)
 {
     const int64_t total = (int64_t)n_act * B;
     if (total == 0) return hipSuccess;
     hipLaunchKernelGGL(k_colors_gather, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, out, in, perm,
-                       n_act, B, n);
+                       n_act, B, n, pp);
     return hipGetLastError();
 }
 
@@ -491,6 +506,146 @@ hipError_t launch_gather_weights_f64(double* dst, const float* src, const int32_
     if (nnz == 0) return hipSuccess;
     hipLaunchKernelGGL(k_gather_weights<double>, dim3((unsigned)((nnz + 255) / 256)), dim3(256), 0, st, dst, src,
                        ent, nnz);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------- adjoint ---
+
+// One thread per (vertex, coloring, value): the first n_act*B*Wy threads
+// form dY, the next n_act*B*Wm form G_main.  Sums run over the transposed
+// split lists in their fixed order, in FP64, then round once to T.
+template <typename T>
+__global__ void k_bwd_node(const NodeArgs<T> a)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t ny = (int64_t)a.n_act * a.B * a.Wy;
+    const int64_t nm = a.Tm ? (int64_t)a.n_act * a.B * a.Wm : 0;
+    if (i >= ny + nm) return;
+    const bool is_dy = i < ny;
+    const int64_t j = is_dy ? i : i - ny;
+    const int W = is_dy ? a.Wy : a.Wm;
+    const int64_t vb = j / W;            // v * B + b
+    const int x = (int)(j - vb * W);     // value index
+    const int64_t v = vb / a.B;
+    const int b = (int)(vb - v * a.B);
+    const T* G = a.G ? a.G + vb * a.Wo : nullptr;
+    const double g0 = a.root_cot[b];  // the root's single "table" value
+    double acc = 0.0;
+    if (is_dy) {
+        if (!a.Tm) {
+            acc = G ? (double)G[x] : g0;  // main leaf: T_s[o] = Y[o]
+        } else {
+            const T* m = a.Tm + vb * a.Wm;
+            const uint32_t* lst = a.by_aux + (size_t)x * a.Pa;
+            for (int p = 0; p < a.Pa; ++p) {
+                const uint32_t e = lst[p];
+                acc += (G ? (double)G[e >> 16] : g0) * (double)m[e & 0xffffu];
+            }
+        }
+        a.dY[vb * a.Wy + x] = (T)(acc * a.scale);
+    } else {
+        const int cv = a.colors[vb];
+        const uint32_t* lst = a.by_main + (size_t)x * a.Pm;
+        for (int p = 0; p < a.Pm; ++p) {
+            const uint32_t e = lst[p];
+            const int ra = (int)(e & 0xffffu);
+            double y;
+            if (a.y_from_h) y = (double)a.Y[vb * a.k + ra + (ra >= cv ? 1 : 0)];
+            else y = (double)a.Y[vb * a.Wy + ra];
+            acc += (G ? (double)G[e >> 16] : g0) * y;
+        }
+        a.Gm[vb * a.Wm + x] = (T)(acc * a.scale);
+    }
+}
+
+template <typename T>
+hipError_t launch_bwd_node(const NodeArgs<T>& a, int, hipStream_t st)
+{
+    const int64_t total = (int64_t)a.n_act * a.B * (a.Wy + (a.Tm ? a.Wm : 0));
+    if (total == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_bwd_node<T>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+template hipError_t launch_bwd_node<float>(const NodeArgs<float>&, int, hipStream_t);
+template hipError_t launch_bwd_node<double>(const NodeArgs<double>&, int, hipStream_t);
+
+// One thread per entry e = (row v, column u), all colorings: the derivative of
+// Y[v] = sum_e w_e * T_a-term(u) with respect to w_e, contracted with dY[v].
+template <typename T>
+__global__ void k_edge_grad(const EdgeGradArgs<T> a)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= a.nnz) return;
+    const int64_t v = a.erow[e];
+    const int64_t u = a.col[e];
+    double acc = 0.0;
+    for (int b = 0; b < a.B; ++b) {
+        const uint32_t cv = a.colors[v * a.B + b];
+        const uint32_t cu = a.ecol[e * a.B + b];
+        if (cu == cv) continue;
+        const T* dy = a.dY + (v * a.B + b) * a.Wy;
+        if (!a.Ta) {
+            acc += (double)dy[cu - (cu > cv ? 1u : 0u)];  // aux leaf: Y value of color cu
+            continue;
+        }
+        const T* ta = a.Ta + (u * a.B + b) * a.Wa;
+        for (int i = 0; i < a.Wy; ++i) {
+            const uint32_t A = expand_bit(a.ymask[i], cv);
+            if (!((A >> cu) & 1u)) continue;
+            const int idx = a.rank_tab[squeeze_bit(A & ~(1u << cu), cu)];
+            acc += (double)dy[i] * (double)ta[idx];
+        }
+    }
+    a.grad[e] += acc;
+}
+
+template <typename T>
+hipError_t launch_edge_grad(const EdgeGradArgs<T>& a, hipStream_t st)
+{
+    if (a.nnz == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_edge_grad<T>, dim3((unsigned)((a.nnz + 255) / 256)), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+template hipError_t launch_edge_grad<float>(const EdgeGradArgs<float>&, hipStream_t);
+template hipError_t launch_edge_grad<double>(const EdgeGradArgs<double>&, hipStream_t);
+
+template <typename TO>
+__global__ void k_scatter_grad(TO* out, const double* grad, const int32_t* ent, int64_t nnz, double factor)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nnz) out[ent[i]] = (TO)(factor * grad[i]);
+}
+
+hipError_t launch_scatter_grad_f32(float* out, const double* grad, const int32_t* ent, int64_t nnz, double factor,
+                                   hipStream_t st)
+{
+    if (nnz == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_scatter_grad<float>, dim3((unsigned)((nnz + 255) / 256)), dim3(256), 0, st, out, grad, ent,
+                       nnz, factor);
+    return hipGetLastError();
+}
+
+hipError_t launch_scatter_grad_f64(double* out, const double* grad, const int32_t* ent, int64_t nnz, double factor,
+                                   hipStream_t st)
+{
+    if (nnz == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_scatter_grad<double>, dim3((unsigned)((nnz + 255) / 256)), dim3(256), 0, st, out, grad, ent,
+                       nnz, factor);
+    return hipGetLastError();
+}
+
+__global__ void k_zero_f64(double* p, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0;
+}
+
+hipError_t launch_zero_f64(double* p, int64_t n, hipStream_t st)
+{
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_zero_f64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n);
     return hipGetLastError();
 }
 

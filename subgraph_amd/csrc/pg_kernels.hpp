@@ -23,6 +23,8 @@ enum AuxMode : int {
     AUX_COLORS = 1,    // the same with the one-hot leaf table: neighbour-color histogram H
     AUX_READ = 2,      // Y read from H[v][.] (aux child is a leaf)
     AUX_PARTIALS = 3,  // Y = ordered sum of a hub's partial rows
+    AUX_GATHER_T = 4,  // adjoint of AUX_GATHER: the vertex's row entry S (a set holding its color)
+                       // sums the neighbours' dY at S, over neighbours whose color is not in S
 };
 // Where the main-child values come from.
 enum MainMode : int {
@@ -93,12 +95,19 @@ size_t sub_lds_bytes(int team, int aux, int main_mode, const SubArgs<T>& a);
 template <typename T>
 hipError_t launch_sub(int team, int aux, int main_mode, const SubArgs<T>& a, int num_cus, hipStream_t st);
 
+// Per-pass values read from device memory, so that one captured graph serves
+// every pass of the same shape (the host writes them before each replay).
+struct PassParams {
+    uint64_t seed;
+    int64_t first;  // index of the pass's first coloring
+};
+
 // colors[v][b] for the pass: from the counter RNG keyed by the caller id
-hipError_t launch_colors_rng(uint8_t* out, const int32_t* perm, int32_t n_act, int B, uint64_t seed,
-                             int64_t first, int k, hipStream_t st);
-// colors[v][b] gathered from caller colors in[c][caller id]
+hipError_t launch_colors_rng(uint8_t* out, const int32_t* perm, int32_t n_act, int B, const PassParams* pp, int k,
+                             hipStream_t st);
+// colors[v][b] gathered from caller colors in[first + b][caller id]
 hipError_t launch_colors_gather(uint8_t* out, const uint8_t* in, const int32_t* perm, int32_t n_act, int B,
-                                int64_t n, hipStream_t st);
+                                int64_t n, const PassParams* pp, hipStream_t st);
 // ecol[e][b] = colors[col[e]][b]: the neighbour colors streamed beside the ids
 hipError_t launch_edge_colors(uint8_t* ecol, const int32_t* col, const uint8_t* colors, int64_t nnz, int B,
                               hipStream_t st);
@@ -108,5 +117,57 @@ hipError_t launch_root_reduce(const double* vals, int64_t n, int B, double* tmp,
 // dst[i] = (T) src[ent[i]] (weights into internal entry order)
 hipError_t launch_gather_weights_f32(float* dst, const float* src, const int32_t* ent, int64_t nnz, hipStream_t st);
 hipError_t launch_gather_weights_f64(double* dst, const float* src, const int32_t* ent, int64_t nnz, hipStream_t st);
+
+// ---- adjoint DP -------------------------------------------------------
+// Per vertex and coloring, from G = dR/dT_s (ones at the root), the main row
+// and Y: dY[i] = sum over splits using aux set i of G[o] * T_m[rm], and
+// G_m[r] = sum over splits using main set r of G[o] * Y[ra] (reference split
+// list transposed by aux set / by main set).  main leaf: dY = G.
+template <typename T>
+struct NodeArgs {
+    int32_t n_act, B, k;
+    const uint8_t* colors;   // [n_act][B]
+    const T* G;              // [n_act][B*Wo], nullptr at the root (G = root_cot[b])
+    const double* root_cot;  // [B] cotangent of each coloring's raw count
+    const T* Tm;             // [n_act][B*Wm] (main not a leaf)
+    const T* Y;              // aux sums: Ybuf [n_act][B*Wy], or H [n_act][B*k] (aux leaf)
+    int32_t y_from_h;        // 1: Y value i = H at the i-th color other than col(v)
+    T* dY;                   // [n_act][B*Wy]
+    T* Gm;                   // [n_act][B*Wm] (main not a leaf)
+    int32_t Wo, Wy, Wm, Pa, Pm;
+    const uint32_t* by_aux;  // [Wy][Pa] (o << 16 | main rank)
+    const uint32_t* by_main; // [Wm][Pm] (o << 16 | y rank)
+    double scale;            // 1, or float(1e-12) for the scaled subtemplate
+};
+template <typename T>
+hipError_t launch_bwd_node(const NodeArgs<T>& a, int num_cus, hipStream_t st);
+
+// Per entry e = (row v, column u): grad[e] += sum_b sum_i dY[v][b][i] * T_a-term(u)
+// (aux table) or dY[v][b][i(col u)] (aux leaf).  One thread per entry.
+template <typename T>
+struct EdgeGradArgs {
+    int64_t nnz;
+    int32_t B, k, K;
+    const int32_t* erow;     // [nnz] row (internal vertex) of each entry
+    const int32_t* col;
+    const uint8_t* colors;   // [n_act][B]
+    const uint8_t* ecol;     // [nnz][B]
+    const T* dY;             // [n_act][B*Wy]
+    int32_t Wy;
+    const T* Ta;             // [n_act][B*Wa] (aux not a leaf), nullptr for a leaf aux
+    int32_t Wa;
+    const uint16_t* ymask;   // [Wy]
+    const int16_t* rank_tab;
+    double* grad;            // [nnz] internal order, accumulated
+};
+template <typename T>
+hipError_t launch_edge_grad(const EdgeGradArgs<T>& a, hipStream_t st);
+
+// out[ent[i]] (=/+=) (TO) (factor * grad[i])
+hipError_t launch_scatter_grad_f32(float* out, const double* grad, const int32_t* ent, int64_t nnz, double factor,
+                                   hipStream_t st);
+hipError_t launch_scatter_grad_f64(double* out, const double* grad, const int32_t* ent, int64_t nnz, double factor,
+                                   hipStream_t st);
+hipError_t launch_zero_f64(double* p, int64_t n, hipStream_t st);
 
 }  // namespace pgbsc

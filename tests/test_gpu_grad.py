@@ -1,0 +1,164 @@
+"""The adjoint DP (pg_count_grad) against brute force and FP64 central
+finite differences, and the torch.autograd wrapper around it.
+
+* per-entry gradients at unit weights equal brute-force per-entry counts of
+  colorful maps (exact, FP64);
+* with weights U(0.5, 1.5) the count is affine in every single entry weight,
+  so FP64 central differences (h = 1e-3) equal the gradient up to rounding;
+* FP32 gradients agree with FP64 to 1e-5 relative (norm-wise).
+"""
+import numpy as np
+import pytest
+
+import oracle
+import subgraph_amd as sg
+
+pytestmark = pytest.mark.gpu
+
+STAR5 = [(0, 1), (0, 2), (0, 3), (0, 4)]
+U5 = [(0, 1), (1, 2), (2, 3), (3, 4)]
+TREE7 = [(0, 1), (1, 2), (2, 3), (1, 4), (4, 5), (4, 6)]
+
+
+def _simple_graph(rng, n, m):
+    s = rng.integers(0, n, m)
+    d = rng.integers(0, n, m)
+    keep = s != d
+    pairs = {(int(min(a, b)), int(max(a, b))) for a, b in zip(s[keep], d[keep])}
+    e = np.array(sorted(pairs), np.int32).reshape(-1, 2)
+    return e[:, 0].copy(), e[:, 1].copy()
+
+
+@pytest.mark.parametrize("tv,te", [(3, [(0, 1), (1, 2)]), (5, STAR5), (5, U5), (6, [(0, 1), (1, 2), (1, 3), (3, 4), (3, 5)]),
+                                   (7, TREE7)])
+def test_grad_equals_brute_force_per_entry(gpu_ctx, tv, te):
+    rng = np.random.default_rng(tv * 7 + len(te))
+    for trial in range(3):
+        n = int(rng.integers(20, 60))
+        s, d = _simple_graph(rng, n, 4 * n)
+        colptr, rowidx = sg.csc_from_edges(n, s, d)
+        gpu_ctx.upload_graph(colptr, rowidx)
+        plan = gpu_ctx.plan(tv, te)
+        cols = rng.integers(0, tv, size=(2, n)).astype(np.uint8)
+        print(f"[bf-grad] k={tv} trial={trial} n={n} nnz={len(rowidx)}", flush=True)
+        raw, grad = gpu_ctx.count_grad(plan, 2, colors=cols, precision="fp64")
+        want = np.zeros(len(rowidx))
+        for i in range(2):
+            tot, pe = oracle.brute_force(tv, te, colptr, rowidx, cols[i], per_edge=True)
+            assert raw[i] == tot
+            want += pe
+        assert np.array_equal(grad, want), np.flatnonzero(grad != want)[:10]
+
+
+def test_weighted_grad_brute_force_and_fd(gpu_ctx):
+    rng = np.random.default_rng(3)
+    n = 40
+    s, d = _simple_graph(rng, n, 170)
+    colptr, rowidx = sg.csc_from_edges(n, s, d)
+    # symmetric weights: one value per undirected edge, both entries
+    w = np.zeros(len(rowidx))
+    wu = {}
+    for u in range(n):
+        for e in range(colptr[u], colptr[u + 1]):
+            key = (min(u, rowidx[e]), max(u, rowidx[e]))
+            wu.setdefault(key, rng.uniform(0.5, 1.5))
+            w[e] = np.float32(wu[key])
+    gpu_ctx.upload_graph(colptr, rowidx, vals=w.astype(np.float32))
+    plan = gpu_ctx.plan(5, U5)
+    cols = rng.integers(0, 5, size=(3, n)).astype(np.uint8)
+    raw, grad = gpu_ctx.count_grad(plan, 3, colors=cols, precision="fp64")
+    want = np.zeros(len(rowidx))
+    for i in range(3):
+        tot, pe = oracle.brute_force(5, U5, colptr, rowidx, cols[i], vals=w, per_edge=True)
+        assert abs(raw[i] - tot) <= 1e-12 * abs(tot)
+        want += pe
+    assert np.allclose(grad, want, rtol=1e-12, atol=1e-12)
+    # central differences of the FP64 forward on single entries (affine in each)
+    _, g1 = gpu_ctx.count_grad(plan, 1, colors=cols[:1], precision="fp64")
+    op = oracle.Plan(5, U5)
+    h = 1e-3
+    for e in rng.choice(len(rowidx), 8, replace=False):
+        wp, wm = w.copy(), w.copy()
+        wp[e] += h
+        wm[e] -= h
+        fp = oracle.count_f64(op, colptr, rowidx, cols[0], vals=wp)
+        fm = oracle.count_f64(op, colptr, rowidx, cols[0], vals=wm)
+        assert abs((fp - fm) / (2 * h) - g1[e]) <= 1e-7 * max(1.0, abs(g1[e]))
+
+
+def test_fp32_grad_close_to_fp64(gpu_ctx):
+    n = 1 << 11
+    s, d = sg.rmat_edges(11, 8, seed=2)
+    colptr, rowidx = sg.csc_from_edges(n, s, d)
+    gpu_ctx.upload_graph(colptr, rowidx)
+    for te, tv in ((TREE7, 7), (STAR5, 5)):
+        plan = gpu_ctx.plan(tv, te)
+        r64, g64 = gpu_ctx.count_grad(plan, 4, seed=9, precision="fp64")
+        r32, g32 = gpu_ctx.count_grad(plan, 4, seed=9, precision="fp32")
+        assert np.all(np.abs(r32 - r64) <= 1e-5 * np.abs(r64))
+        assert np.linalg.norm(g32 - g64) <= 1e-5 * np.linalg.norm(g64)
+        # the gradient's forward equals the plain forward
+        assert np.array_equal(r64, gpu_ctx.count(plan, 4, seed=9, precision="fp64"))
+
+
+def test_config4_fd_s18(gpu_ctx):
+    """R-MAT s18/ef16, weights U(0.5,1.5) (symmetric), 5-star and 5-path, 16
+    colorings: FP64 central differences (h=1e-3) of the device forward on 32
+    sampled edges (both entries of each) against the adjoint."""
+    rng = np.random.default_rng(18)
+    n = 1 << 18
+    s, d = sg.rmat_edges(18, 16, seed=1)
+    w_und = rng.uniform(0.5, 1.5, len(s)).astype(np.float32)
+    colptr, rowidx = sg.csc_from_edges(n, s, d)
+    # entry weights: look up the undirected edge of every entry
+    key = np.minimum(np.repeat(np.arange(n), np.diff(colptr)), rowidx).astype(np.int64) * n + \
+        np.maximum(np.repeat(np.arange(n), np.diff(colptr)), rowidx)
+    ukey = s.astype(np.int64) * n + d
+    order = np.argsort(ukey)
+    w = w_und[order[np.searchsorted(ukey[order], key)]]
+    gpu_ctx.upload_graph(colptr, rowidx, vals=w)
+    for name, te in (("star5", STAR5), ("u5", U5)):
+        plan = gpu_ctx.plan(5, te)
+        print(f"[config4] {name}: adjoint", flush=True)
+        raw, grad = gpu_ctx.count_grad(plan, 16, seed=4, precision="fp64")
+        for t in range(32):
+            i = int(rng.integers(0, len(s)))
+            a, b = int(s[i]), int(d[i])
+            ents = [colptr[a] + int(np.searchsorted(rowidx[colptr[a]:colptr[a + 1]], b)),
+                    colptr[b] + int(np.searchsorted(rowidx[colptr[b]:colptr[b + 1]], a))]
+            # weights are FP32: perturb to the FP32 values nearest w +- h and
+            # divide by the step actually taken (the count is affine in w_e)
+            h = 1e-3
+            wp = w.copy()
+            wm = w.copy()
+            wp[ents] = (w[ents].astype(np.float64) + h).astype(np.float32)
+            wm[ents] = (w[ents].astype(np.float64) - h).astype(np.float32)
+            gpu_ctx.set_weights(wp)
+            fp = gpu_ctx.count(plan, 16, seed=4, precision="fp64").sum()
+            gpu_ctx.set_weights(wm)
+            fm = gpu_ctx.count(plan, 16, seed=4, precision="fp64").sum()
+            step = float(wp[ents[0]]) - float(wm[ents[0]])
+            fd = (fp - fm) / step
+            g = grad[ents].sum()
+            tol = 1e-6 * abs(g) + 1e-13 * abs(raw.sum()) / step
+            assert abs(fd - g) <= tol, (name, t, fd, g)
+        gpu_ctx.set_weights(w)
+
+
+def test_autograd_function(gpu_ctx):
+    torch = pytest.importorskip("torch")
+    from subgraph_amd.autograd import colorful_count
+    n = 1 << 10
+    s, d = sg.rmat_edges(10, 8, seed=3)
+    colptr, rowidx = sg.csc_from_edges(n, s, d)
+    gpu_ctx.upload_graph(colptr, rowidx)
+    plan = gpu_ctx.plan(5, U5)
+    w = torch.rand(len(rowidx), dtype=torch.float32, device="cuda") + 0.5
+    w.requires_grad_(True)
+    raw = colorful_count(w, gpu_ctx, plan, 4, seed=2, precision="fp64")
+    coef = torch.tensor([1.0, -2.0, 0.5, 3.0], dtype=torch.float64, device="cuda")
+    (raw * coef).sum().backward()
+    gpu_ctx.set_weights(w.detach().cpu().numpy())
+    r2, g2 = gpu_ctx.count_grad(plan, 4, seed=2, precision="fp64", cotangent=coef.cpu().numpy())
+    assert np.allclose(raw.detach().cpu().numpy(), r2, rtol=1e-12)
+    assert np.allclose(w.grad.cpu().numpy(), g2.astype(np.float32), rtol=1e-6, atol=1e-6)

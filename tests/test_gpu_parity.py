@@ -147,6 +147,31 @@ def test_batch_tiles_agree(gpu_ctx):
     assert np.array_equal(again, ref32)  # run to run: bit-identical
 
 
+def test_graph_replays_match_eager(gpu_ctx):
+    """Each pass runs as a captured hipGraph, replayed for every later pass of
+    the same shape: several replays, with eager passes in between, all equal
+    the eager launches bit for bit."""
+    n = 1 << 12
+    s, d = sg.rmat_edges(12, 16, seed=6)
+    colptr, rowidx = sg.csc_from_edges(n, s, d)
+    gpu_ctx.upload_graph(colptr, rowidx)
+    plan = gpu_ctx.plan(7, TREE7)
+    gpu_ctx.set_option("graphs", 0)
+    try:
+        eager = [gpu_ctx.count(plan, 5, seed=40 + r, precision="fp32") for r in range(3)]
+        eager64 = gpu_ctx.count(plan, 5, seed=40, precision="fp64")
+    finally:
+        gpu_ctx.set_option("graphs", 1)
+    for rep in range(3):
+        print(f"[graphs] replay round {rep}", flush=True)
+        for r in range(3):
+            assert np.array_equal(gpu_ctx.count(plan, 5, seed=40 + r, precision="fp32"), eager[r])
+        gpu_ctx.set_option("graphs", 0)
+        assert np.array_equal(gpu_ctx.count(plan, 5, seed=40, precision="fp64"), eager64)
+        gpu_ctx.set_option("graphs", 1)
+        assert np.array_equal(gpu_ctx.count(plan, 5, seed=40, precision="fp64"), eager64)
+
+
 def test_device_colors_equal_host_stream(gpu_ctx):
     n = 1 << 11
     s, d = sg.rmat_edges(11, 8, seed=9)
