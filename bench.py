@@ -83,6 +83,50 @@ def cpu_baseline(n_decl, src, dst, seed, ncol, gpu_raw_fn):
             "seconds_per_coloring": sum(secs) / ncol, "parity_max_rel_err": err}
 
 
+def _timed(fn, reps, sync):
+    fn()
+    sync()
+    t0 = time.perf_counter()
+    for r in range(reps):
+        fn(r)
+    sync()
+    return (time.perf_counter() - t0) / reps
+
+
+def fwd_bwd_variant(sync):
+    """Differentiation workload: R-MAT s18/ef16 with symmetric weights
+    U(0.5,1.5), 5-star and 5-path, 16 colorings per call, forward + adjoint
+    (per-entry gradient), beside the plain one-coloring forward on the same
+    graph.  FP32 tables."""
+    import subgraph_amd as sg
+    rng = np.random.default_rng(18)
+    n = 1 << 18
+    src, dst = sg.rmat_edges(18, 16, seed=1)
+    w_und = rng.uniform(0.5, 1.5, len(src)).astype(np.float32)
+    colptr, rowidx = sg.csc_from_edges(n, src, dst)
+    rows = np.repeat(np.arange(n), np.diff(colptr))
+    key = np.minimum(rows, rowidx).astype(np.int64) * n + np.maximum(rows, rowidx)
+    ukey = src.astype(np.int64) * n + dst
+    order = np.argsort(ukey)
+    w = w_und[order[np.searchsorted(ukey[order], key)]]
+    ctx = sg.Context(0)
+    ctx.upload_graph(colptr, rowidx, vals=w)
+    out = {"graph": "R-MAT scale 18 edge factor 16 (seed 1), weights U(0.5,1.5) symmetric",
+           "nnz": int(len(rowidx)), "colorings_per_call": 16, "dtype": "fp32", "templates": {}}
+    for name, te in (("star5", [(0, 1), (0, 2), (0, 3), (0, 4)]), ("u5", [(0, 1), (1, 2), (2, 3), (3, 4)])):
+        plan = ctx.plan(5, te)
+        t_plain = _timed(lambda r=0: ctx.count(plan, 1, seed=50 + r), 10, sync)
+        t_fwd = _timed(lambda r=0: ctx.count(plan, 16, seed=60 + r), 5, sync)
+        t_fb = _timed(lambda r=0: ctx.count_grad(plan, 16, seed=70 + r, precision="fp32"), 3, sync)
+        rec = {"plain_fwd_colorings_per_s": 1.0 / t_plain, "batched_fwd_colorings_per_s": 16 / t_fwd,
+               "fwd_bwd_colorings_per_s": 16 / t_fb}
+        rec["fwd_bwd_vs_plain_fwd"] = rec["fwd_bwd_colorings_per_s"] / rec["plain_fwd_colorings_per_s"]
+        out["templates"][name] = rec
+        log(f"fwd+bwd s18 {name}: plain fwd {rec['plain_fwd_colorings_per_s']:.0f}/s, batched fwd "
+            f"{rec['batched_fwd_colorings_per_s']:.0f}/s, fwd+bwd {rec['fwd_bwd_colorings_per_s']:.0f}/s")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -91,6 +135,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--cpu-colorings", type=int, default=12)
+    ap.add_argument("--variants", default="grad", help="comma list of extra measurements: grad")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print(json.dumps({"error": "this build measures one GPU: bench.py supports --gpus 1 only; "
@@ -168,8 +213,9 @@ def main():
         bel = time.perf_counter() - t0
         bval = reps * B / bel
         result["batched"] = {"colorings_per_call": B, "value": bval, "unit": "colorings/s",
-                             "vs_plain": bval / value, "batch_tile": ctx.get_option("batch_tile") or "auto"}
+                             "vs_plain": bval / value, "workload": "same graph and template as the plain line"}
         log(f"batched: {bval:.1f} colorings/s ({bval / value:.2f}x plain)")
+
 
     if not args.no_cpu:
         seed = 4242
@@ -186,6 +232,9 @@ def main():
         except (subprocess.SubprocessError, OSError) as e:
             result["cpu_baseline"] = {"value": None, "unit": "colorings/s", "cores": physical_cores(),
                                       "kind": "reference", "sample": f"failed: {e}"}
+    if "grad" in args.variants.split(","):
+        del ctx, plan
+        result["fwd_bwd"] = fwd_bwd_variant(sync)
     print(json.dumps(result), flush=True)
 
 

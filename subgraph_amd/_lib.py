@@ -36,6 +36,24 @@ class PGError(RuntimeError):
         self.code = code
 
 
+def _share_hip_runtime() -> None:
+    """Load PyTorch before libpgbsc.so when PyTorch is installed.
+
+    The PyTorch-ROCm wheel bundles its own libamdhip64 / libhsa-runtime64,
+    which its libraries request under unversioned names.  If libpgbsc.so came
+    first, the system runtime (soname libamdhip64.so.7) would be loaded and
+    PyTorch would then load its bundled copy beside it: two HSA runtimes in
+    one process, and the second one finds no GPU.  Loading PyTorch first makes
+    libpgbsc.so's libamdhip64.so.7 resolve to the copy already in the process
+    (same soname), so the library and PyTorch share one runtime and device
+    pointers pass freely between them.  Set PGBSC_SYSTEM_HIP=1 to skip this
+    (processes that never touch PyTorch's GPU side)."""
+    import importlib.util
+    if os.environ.get("PGBSC_SYSTEM_HIP") == "1" or importlib.util.find_spec("torch") is None:
+        return
+    import torch  # noqa: F401  (loads libtorch_hip and PyTorch's HIP runtime; no GPU init)
+
+
 def lib() -> C.CDLL:
     global _lib
     if _lib is not None:
@@ -43,6 +61,7 @@ def lib() -> C.CDLL:
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"libpgbsc.so not built at {LIB_PATH}: run __graft_entry__.build() "
                           f"or make -C subgraph_amd/csrc")
+    _share_hip_runtime()
     L = C.CDLL(LIB_PATH)
     sig = {
         "pg_create": (C.c_int, [C.POINTER(_vp), C.c_int]),
