@@ -150,6 +150,13 @@ int pg_last_timing(const pg_ctx* ctx, double* total_ms, double* spmm_ms, int64_t
 /* name of that dominant kernel ("k_sub<float,64,0,1>": dtype, team lanes,
  * aux mode, main mode); timing is collected when option "timing" is 1 */
 int pg_last_timing_kernel(const pg_ctx* ctx, char* buf, int len);
+/* per-kernel breakdown of the same call (the VERBOSE-style per-phase report
+ * of CountMat::colorCounting, CountMat.cpp:268-322, as device timings):
+ * entry idx (0-based, kernels in name order) -> name, summed ms, launches and
+ * algorithmic bytes.  Returns the number of kernels, or PG_ERR_RANGE for an
+ * idx past the end (name/ms/launches/bytes may be NULL). */
+int pg_last_timing_entry(const pg_ctx* ctx, int idx, char* name, int len, double* ms,
+                         int64_t* launches, double* bytes);
 
 /* ---- host surface helpers used by the CLI and the Python layer ---- */
 /* R-MAT generator: n = 2^scale, edge_factor*n draws with quadrant

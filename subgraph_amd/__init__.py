@@ -246,6 +246,25 @@ class Context:
         return {"total_ms": t.value, "kernel": buf.value.decode(), "kernel_ms": km.value,
                 "kernel_launches": kl.value, "kernel_bytes": kb.value}
 
+    def last_timing_kernels(self) -> dict:
+        """{kernel name: (summed ms, launches, algorithmic bytes)} of the last
+        count call made with option "timing" = 1."""
+        out = {}
+        buf = C.create_string_buffer(128)
+        ms, nl, nb = C.c_double(), C.c_int64(), C.c_double()
+        idx = 0
+        while True:
+            rc = lib().pg_last_timing_entry(self.ptr, idx, buf, 128, C.byref(ms), C.byref(nl), C.byref(nb))
+            if rc == -6:          # PG_ERR_RANGE: past the last kernel
+                break
+            if rc < 0:
+                self._check(rc, "pg_last_timing_entry")
+            out[buf.value.decode()] = (ms.value, nb.value, nl.value)
+            idx += 1
+            if idx >= rc:
+                break
+        return out
+
 
 class CountMat:
     """CountMat.hpp:19-191: initialization(graph, ...) then compute(template)."""

@@ -85,6 +85,8 @@ def lib() -> C.CDLL:
         "pg_last_timing": (C.c_int, [_vp, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                      C.POINTER(C.c_int64), C.POINTER(C.c_double)]),
         "pg_last_timing_kernel": (C.c_int, [_vp, C.c_char_p, C.c_int]),
+        "pg_last_timing_entry": (C.c_int, [_vp, C.c_int, C.c_char_p, C.c_int, C.POINTER(C.c_double),
+                                           C.POINTER(C.c_int64), C.POINTER(C.c_double)]),
         "pg_rmat_edges": (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_uint64,
                                     C.c_int64, _vp, _vp, C.POINTER(C.c_int64)]),
         "pg_csc_from_edges": (C.c_int, [C.c_int32, C.c_int64, _i32p, _i32p, _i32p, _vp]),
@@ -107,5 +109,5 @@ def exported_symbols() -> list[str]:
             "pg_graph_upload", "pg_graph_set_weights", "pg_graph_stats", "pg_plan_create",
             "pg_plan_from_edges", "pg_plan_destroy", "pg_plan_info", "pg_colors_fill", "pg_count",
             "pg_count_multi", "pg_count_grad", "pg_last_timing", "pg_last_timing_kernel",
-            "pg_rmat_edges", "pg_csc_from_edges", "pg_edges_remove_holes", "pg_template_sizes",
-            "pg_template_tables"]
+            "pg_last_timing_entry", "pg_rmat_edges", "pg_csc_from_edges", "pg_edges_remove_holes",
+            "pg_template_sizes", "pg_template_tables"]

@@ -72,17 +72,15 @@ struct GraphKey {
     const void* colors_in;  // caller colors buffer, nullptr = device RNG
     const void* arena;
     uint64_t generation;    // graph / weights / bucket version
-    bool timed;
     bool operator<(const GraphKey& o) const
     {
-        return std::tie(plan, B, tsize, colors_in, arena, generation, timed) <
-               std::tie(o.plan, o.B, o.tsize, o.colors_in, o.arena, o.generation, o.timed);
+        return std::tie(plan, B, tsize, colors_in, arena, generation) <
+               std::tie(o.plan, o.B, o.tsize, o.colors_in, o.arena, o.generation);
     }
 };
 
 struct PassGraph {
     hipGraphExec_t exec = nullptr;
-    std::vector<LaunchRec> recs;  // event pairs recorded by the graph's nodes
 };
 
 // One non-leaf subtemplate as the device schedule sees it.  Widths are per
@@ -304,9 +302,9 @@ size_t sub_lds_max(const pg_plan* p, const DevSub& d, int B, size_t tsz)
 
 constexpr size_t kLdsLimit = 64 * 1024;
 
-// Timing: an event pair around each launch, appended to c->cur_recs (the
-// context's eager list, or the list owned by the graph being captured, whose
-// event-record nodes re-record on every replay).
+// Timing: an event pair around each launch, appended to c->cur_recs.  Passes
+// run eagerly while timing is on: event-record nodes inside an instantiated
+// graph give no usable elapsed times on this runtime, so graphs carry none.
 void record(pg_ctx* c, const std::string& name, double bytes, bool begin)
 {
     if (!c->timing || !c->cur_recs) return;
@@ -581,13 +579,13 @@ void drop_graphs(pg_ctx* c, const pg_plan* only = nullptr)
             continue;
         }
         (void)hipGraphExecDestroy(it->second.exec);
-        destroy_recs(it->second.recs);
         it = c->graphs.erase(it);
     }
 }
 
-// One pass: through its cached graph when graphs are on (capturing it on
-// first use), else eagerly.  Writes B raw counts (before finish_raw).
+// One pass: through its cached graph when graphs are on and timing is off
+// (capturing it on first use), else eagerly.  Writes B raw counts (before
+// finish_raw).
 template <typename T>
 int run_pass(pg_ctx* c, const pg_plan* p, int B, int64_t first, const uint8_t* d_in, uint64_t seed,
              const MemPlan& mp, double* raw)
@@ -596,7 +594,7 @@ int run_pass(pg_ctx* c, const pg_plan* p, int B, int64_t first, const uint8_t* d
     const PassParams hp{seed, first};
     // pageable source: the copy completes before the call returns
     HIP_TRY(c, hipMemcpyAsync(base + mp.off_params, &hp, sizeof hp, hipMemcpyHostToDevice, c->stream));
-    if (!c->use_graphs) {
+    if (!c->use_graphs || c->timing) {
         std::vector<LaunchRec> recs;
         c->cur_recs = &recs;
         int rc = enqueue_pass<T>(c, p, B, d_in, mp);
@@ -611,34 +609,27 @@ int run_pass(pg_ctx* c, const pg_plan* p, int B, int64_t first, const uint8_t* d
         destroy_recs(recs);
         return PG_OK;
     }
-    const GraphKey key{p, B, (int)sizeof(T), d_in, c->d_arena.p, c->generation, c->timing != 0};
+    const GraphKey key{p, B, (int)sizeof(T), d_in, c->d_arena.p, c->generation};
     auto it = c->graphs.find(key);
     if (it == c->graphs.end()) {
         PassGraph g;
-        c->cur_recs = &g.recs;
         HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         int rc = enqueue_pass<T>(c, p, B, d_in, mp);
         hipGraph_t graph = nullptr;
         hipError_t e = hipStreamEndCapture(c->stream, &graph);
-        c->cur_recs = nullptr;
         if (rc || e != hipSuccess) {
             if (graph) (void)hipGraphDestroy(graph);
-            destroy_recs(g.recs);
             if (rc) return rc;
             return set_err(c, PG_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
         }
         e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
         (void)hipGraphDestroy(graph);
-        if (e != hipSuccess) {
-            destroy_recs(g.recs);
-            return set_err(c, PG_ERR_HIP, "graph instantiation failed: %s", hipGetErrorString(e));
-        }
+        if (e != hipSuccess) return set_err(c, PG_ERR_HIP, "graph instantiation failed: %s", hipGetErrorString(e));
         it = c->graphs.emplace(key, std::move(g)).first;
     }
     HIP_TRY(c, hipGraphLaunch(it->second.exec, c->stream));
     HIP_TRY(c, hipMemcpyAsync(raw, base + mp.off_rawdev, sizeof(double) * B, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
-    accumulate_timing(c, it->second.recs);
     return PG_OK;
 }
 
@@ -1541,4 +1532,18 @@ extern "C" int pg_last_timing_kernel(const pg_ctx* c, char* buf, int len)
     if (!c || !buf || len <= 0) return PG_ERR_INVALID;
     snprintf(buf, (size_t)len, "%s", c->dom_name.c_str());
     return PG_OK;
+}
+
+extern "C" int pg_last_timing_entry(const pg_ctx* c, int idx, char* name, int len, double* ms,
+                                    int64_t* launches, double* bytes)
+{
+    if (!c || idx < 0 || (name && len <= 0)) return PG_ERR_INVALID;
+    if ((size_t)idx >= c->agg.size()) return PG_ERR_RANGE;
+    auto it = c->agg.begin();
+    std::advance(it, idx);
+    if (name) snprintf(name, (size_t)len, "%s", it->first.c_str());
+    if (ms) *ms = it->second.ms;
+    if (launches) *launches = it->second.launches;
+    if (bytes) *bytes = it->second.bytes;
+    return (int)c->agg.size();
 }

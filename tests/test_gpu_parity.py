@@ -172,6 +172,30 @@ def test_graph_replays_match_eager(gpu_ctx):
         assert np.array_equal(gpu_ctx.count(plan, 5, seed=40, precision="fp64"), eager64)
 
 
+def test_timing_breakdown(gpu_ctx):
+    """Option "timing" runs passes eagerly with an event pair per launch: the
+    per-kernel breakdown is non-empty, its durations positive, its total no
+    more than the call's device time, and the counts unchanged."""
+    n = 1 << 12
+    s, d = sg.rmat_edges(12, 16, seed=6)
+    colptr, rowidx = sg.csc_from_edges(n, s, d)
+    gpu_ctx.upload_graph(colptr, rowidx)
+    plan = gpu_ctx.plan(7, TREE7)
+    want = gpu_ctx.count(plan, 3, seed=11)
+    gpu_ctx.set_option("timing", 1)
+    try:
+        got = gpu_ctx.count(plan, 3, seed=11)
+        kern = gpu_ctx.last_timing_kernels()
+        tot = gpu_ctx.last_timing()
+    finally:
+        gpu_ctx.set_option("timing", 0)
+    assert np.array_equal(got, want)
+    assert kern and all(ms > 0 and nl > 0 for ms, _, nl in kern.values()), kern
+    assert any(name.startswith("k_sub<float") and nb > 0 for name, (_, nb, _) in kern.items())
+    assert sum(ms for ms, _, _ in kern.values()) <= tot["total_ms"] * 1.01
+    assert tot["kernel"] in kern and tot["kernel_ms"] == max(ms for ms, _, _ in kern.values())
+
+
 def test_device_colors_equal_host_stream(gpu_ctx):
     n = 1 << 11
     s, d = sg.rmat_edges(11, 8, seed=9)
