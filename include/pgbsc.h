@@ -57,7 +57,13 @@ const char* pg_last_error(const pg_ctx* ctx);
  *   "bucket_mid"   ... at least this degree 16-lane teams, the rest 8 (default 8)
  *   "batch_tile"   colorings per pass, 0 = automatic (default 0)
  *   "mem_budget_mb" device bytes the tables may use, 0 = 90% of free memory
- *   "autotune"     1 = pick hub_degree/hub_chunk/batch_tile by timing (default 0) */
+ *   "autotune"     1 = pick hub_degree/hub_chunk/batch_tile by timing (default 0)
+ *   "timing"       1 = event pair around every launch, passes run eagerly (default 0)
+ *   "graphs"       1 = each pass replays a captured hipGraph (default 1)
+ *   "blocks_per_cu" workgroups per CU of persistent launches, 0 = occupancy query
+ *   "stage_rows"   1 = gathers stage neighbour rows in LDS, 0 = direct loads (default 0)
+ *   "aux_rows"     neighbour sums row-wise with LDS atomics: 1 always, 0 never (default),
+ *                  -1 when most of a neighbour row is used */
 int pg_set_option(pg_ctx* ctx, const char* key, int64_t value);
 int pg_get_option(const pg_ctx* ctx, const char* key, int64_t* value);
 

@@ -102,7 +102,15 @@ struct DevSub {
     DevBuf by_aux;   // [Wy][Pa] uint32 (o << 16 | main rank)
     DevBuf by_main;  // [Wm][Pm] uint32 (o << 16 | y rank)
     DevBuf amask;    // [Wa] uint16
+    // gather index table of AUX_GATHER (SubArgs::gidx), when it fits kGidxMaxBytes
+    DevBuf gidx;     // [(k+1)*k][Wy4] int16
+    int Wy4 = 0;
+    // row-wise index table of AUX_ROWS (SubArgs::xidx), when it fits kGidxMaxBytes
+    DevBuf xidx;     // [k*k][Wa4] int16
+    int Wa4 = 0;
 };
+
+constexpr size_t kGidxMaxBytes = 24 * 1024;
 
 }  // namespace
 
@@ -122,6 +130,9 @@ struct pg_ctx {
     // options
     int64_t hub_degree = 256, hub_chunk = 256, batch_tile = 0, mem_budget_mb = 0, timing = 0, autotune = 0;
     int64_t bucket_high = 32, bucket_mid = 8;
+    int64_t blocks_per_cu = 0;  // 0: resident workgroups from the occupancy query
+    int64_t aux_rows = 0;       // neighbour sums row-wise: 1 always, 0 never, -1 per subtemplate
+    int64_t stage_rows = 0;     // gathers read neighbour rows staged in LDS (1) or directly (0)
     // graph
     bool has_graph = false, weighted = false;
     int32_t n = 0, n_act = 0, max_deg = 0;
@@ -189,6 +200,8 @@ int slots_for(int B, int W) { return B * ((W + kRegCols - 1) / kRegCols); }
 int team_for(int cols) { return std::max(8, cl_for(cols)); }
 
 // (Re)build degree buckets and hub chunk items for the current options.
+void drop_graphs(pg_ctx* c, const pg_plan* only = nullptr);
+
 int build_buckets(pg_ctx* c)
 {
     if (c->built_deg == c->hub_degree && c->built_chunk == c->hub_chunk && c->built_high == c->bucket_high &&
@@ -222,7 +235,8 @@ int build_buckets(pg_ctx* c)
         HIP_TRY(c, hipMemcpy(c->d_hub_items.p, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice));
     HIP_TRY(c, hipMemcpy(c->d_slot_begin.p, slot_begin.data(), slot_begin.size() * sizeof(int32_t),
                          hipMemcpyHostToDevice));
-    c->generation++;  // captured passes hold the old bucket ranges
+    drop_graphs(c);  // captured passes hold the old bucket ranges
+    c->generation++;
     c->built_deg = c->hub_degree;
     c->built_chunk = c->hub_chunk;
     c->built_high = c->bucket_high;
@@ -289,15 +303,94 @@ MemPlan plan_memory(const pg_ctx* c, const pg_plan* p, int B, size_t tsz)
     return m;
 }
 
-// LDS the largest launch of a sub needs for B colorings: the one with the
-// smallest team (most teams per workgroup)
-size_t sub_lds_max(const pg_plan* p, const DevSub& d, int B, size_t tsz)
+// How a subtemplate with a non-leaf aux child sums over neighbours.  Row-wise
+// (AUX_ROWS) reads each neighbour row whole, which pays when most of it is
+// used: a neighbour of another color adds C(K-1, aa-1) of its C(K, aa-1)
+// values, a fraction (K - aa + 1) / K.  Option "aux_rows": 1 always, 0 never,
+// -1 by that fraction.
+int aux_mode_for(const pg_ctx* c, const pg_plan* p, const DevSub& d)
 {
-    const size_t rank = d.aa > 1 ? ((((size_t)2 << p->K) + 15) & ~(size_t)15) : 0;
-    const int team = d.aa > 1 ? std::max(8, cl_for(slots_for(B, d.Wy))) : team_for(std::max(B * d.Wy, B * d.Wo));
-    const int team_fin = team_for(std::max(B * d.Wy, B * d.Wo));  // hub rows from partials
-    const int teams = kBlock / std::min(team, team_fin);
-    return rank + (size_t)teams * B * (d.Wy + (d.mm > 1 ? d.Wm : 0)) * tsz;
+    if (c->aux_rows == 0) return AUX_GATHER;
+    if (c->aux_rows == 1) return AUX_ROWS;
+    return 2 * (p->K - d.aa + 1) >= p->K ? AUX_ROWS : AUX_GATHER;
+}
+
+// column lanes of a neighbour-summing launch
+int cl_of(int aux, int B, int k, int Wy, int Wa)
+{
+    if (aux == AUX_ROWS) return cl_for(B * Wa);
+    return cl_for(slots_for(B, aux == AUX_COLORS ? k : Wy));
+}
+
+// LDS row staging of a gathering launch (AUX_GATHER with a gather index
+// table): the longest row segment one chunk of cl slots needs, padded to a
+// multiple of cl; 0 (direct gathers) when a segment needs more than
+// kStageRegs values per lane or option "stage_rows" is 0
+constexpr size_t kStageLdsMax = 32 * 1024;  // staged rows per workgroup
+
+int stage_pad(const pg_ctx* c, int aux, bool has_gidx, int B, int Wy, int Wa, int cl, size_t tsz)
+{
+    if (aux != AUX_GATHER || !has_gidx || c->stage_rows == 0) return 0;
+    const int spb = (Wy + kRegCols - 1) / kRegCols;
+    const int nslots = B * spb;
+    int mx = 0;
+    for (int s0 = 0; s0 < nslots; s0 += cl) {
+        const int bf = s0 / spb, bl = std::min(B - 1, (s0 + cl - 1) / spb);
+        mx = std::max(mx, (bl - bf + 1) * Wa);
+    }
+    if (mx > kStageRegs * cl) return 0;
+    const int pad = (mx + cl - 1) / cl * cl;
+    // every workgroup stages kBlock / cl groups' rows, whatever the team shape
+    if ((size_t)(kBlock / cl) * stage_group_stride(pad, cl) * tsz > kStageLdsMax) return 0;
+    return pad;
+}
+
+template <typename T>
+size_t lds_of(int team, int aux, int main_mode, int ng, int out_mode, int B, const pg_plan* p, const DevSub& d,
+              int segpad = 0)
+{
+    SubArgs<T> a{};
+    a.B = B;
+    a.k = p->k;
+    a.K = p->K;
+    a.Wy = d.Wy;
+    a.Wa = d.Wa;
+    a.Wm = d.Wm;
+    a.ng = ng;
+    a.out_mode = out_mode;
+    a.gidx = (const int16_t*)d.gidx.p;
+    a.Wy4 = d.Wy4;
+    a.xidx = (const int16_t*)d.xidx.p;
+    a.Wa4 = d.Wa4;
+    a.segpad = segpad;
+    return sub_lds_bytes<T>(team, aux, main_mode, a);
+}
+
+// LDS of the largest launch of subtemplate d for B colorings, over every
+// launch shape launch_gathering (or the aux-leaf read) would use
+template <typename T>
+size_t sub_lds_max_t(const pg_ctx* c, const pg_plan* p, const DevSub& d, int B)
+{
+    const int mm = d.mm > 1 ? MAIN_TABLE : MAIN_LEAF;
+    const int out = d.root ? OUT_ROOT : OUT_EMA;
+    const int fin_team = team_for(std::max(B * d.Wy, B * d.Wo));
+    size_t m = lds_of<T>(fin_team, d.aa == 1 ? AUX_READ : AUX_PARTIALS, mm, 1, out, B, p, d);
+    if (d.aa == 1) return m;
+    const int aux = aux_mode_for(c, p, d);
+    const int cl = cl_of(aux, B, p->k, d.Wy, d.Wa);
+    const int sp = stage_pad(c, aux, d.gidx.p != nullptr, B, d.Wy, d.Wa, cl, sizeof(T));
+    const int aux_l = sp > 0 ? AUX_STAGED : aux;
+    m = std::max(m, lds_of<T>(64, aux_l, mm, 64 / cl, OUT_PARTIAL, B, p, d, sp));
+    for (int lanes : {64, 16, 8}) {
+        const int team = std::max(lanes, cl);
+        m = std::max(m, lds_of<T>(team, aux_l, mm, team / cl, out, B, p, d, sp));
+    }
+    return m;
+}
+
+size_t sub_lds_max(const pg_ctx* c, const pg_plan* p, const DevSub& d, int B, size_t tsz)
+{
+    return tsz == 4 ? sub_lds_max_t<float>(c, p, d, B) : sub_lds_max_t<double>(c, p, d, B);
 }
 
 constexpr size_t kLdsLimit = 64 * 1024;
@@ -358,7 +451,9 @@ std::string kname(int team, int aux, int main_mode)
 template <typename T>
 int launch_gathering(pg_ctx* c, int aux, int main_mode, SubArgs<T> a, T* part, int y_cols, double bytes)
 {
-    const int cl = cl_for(slots_for(a.B, aux == AUX_COLORS ? a.k : a.Wy));
+    const int cl = cl_of(aux, a.B, a.k, a.Wy, a.Wa);
+    a.segpad = stage_pad(c, aux, a.gidx != nullptr, a.B, a.Wy, a.Wa, cl, sizeof(T));
+    if (a.segpad > 0) aux = AUX_STAGED;
     const hipStream_t st = c->stream;
     const int cus = c->num_cus;
     const double units = (double)c->nnz + (double)c->n_act;
@@ -373,7 +468,7 @@ int launch_gathering(pg_ctx* c, int aux, int main_mode, SubArgs<T> a, T* part, i
         h.item_base = 0;
         h.ng = 64 / cl;
         record(c, kname<T>(64, aux, main_mode), share(c->h_rowptr[c->n_hub], 0), true);
-        HIP_TRY(c, launch_sub<T>(64, aux, main_mode, h, cus, st));
+        HIP_TRY(c, launch_sub<T>(64, aux, main_mode, h, cus, c->blocks_per_cu, st));
         record(c, "", 0, false);
     }
     struct Bucket { int32_t lo, hi; int lanes; };
@@ -388,7 +483,7 @@ int launch_gathering(pg_ctx* c, int aux, int main_mode, SubArgs<T> a, T* part, i
         r.ng = team / cl;
         record(c, kname<T>(team, aux, main_mode),
                share((double)c->h_rowptr[bk.hi] - c->h_rowptr[bk.lo], bk.hi - bk.lo), true);
-        HIP_TRY(c, launch_sub<T>(team, aux, main_mode, r, cus, st));
+        HIP_TRY(c, launch_sub<T>(team, aux, main_mode, r, cus, c->blocks_per_cu, st));
         record(c, "", 0, false);
     }
     if (c->n_hub > 0) {
@@ -401,7 +496,7 @@ int launch_gathering(pg_ctx* c, int aux, int main_mode, SubArgs<T> a, T* part, i
         f.ng = 1;
         const int team = team_for(std::max(y_cols, a.B * a.Wo));
         record(c, kname<T>(team, AUX_PARTIALS, main_mode), share(0, c->n_hub), true);
-        HIP_TRY(c, launch_sub<T>(team, AUX_PARTIALS, main_mode, f, cus, st));
+        HIP_TRY(c, launch_sub<T>(team, AUX_PARTIALS, main_mode, f, cus, c->blocks_per_cu, st));
         record(c, "", 0, false);
     }
     return PG_OK;
@@ -499,13 +594,18 @@ int enqueue_pass(pg_ctx* c, const pg_plan* p, int B, const uint8_t* d_colors_in,
             a.n_items = c->n_act;
             a.item_base = 0;
             record(c, kname<T>(team, AUX_READ, main_mode), tab_bytes, true);
-            HIP_TRY(c, launch_sub<T>(team, AUX_READ, main_mode, a, c->num_cus, st));
+            HIP_TRY(c, launch_sub<T>(team, AUX_READ, main_mode, a, c->num_cus, c->blocks_per_cu, st));
             record(c, "", 0, false);
         } else {
             a.aux = (const T*)(base + mp.off_tab[d.ai]);
             a.Wa = d.Wa;
             a.lda = (int64_t)B * d.Wa;
-            int rc = launch_gathering<T>(c, AUX_GATHER, main_mode, a, part, B * d.Wy, nb_csc + tab_bytes);
+            a.xmask = d.amask.as<uint16_t>();
+            a.gidx = d.gidx.as<int16_t>();
+            a.Wy4 = d.Wy4;
+            a.xidx = d.xidx.as<int16_t>();
+            a.Wa4 = d.Wa4;
+            int rc = launch_gathering<T>(c, aux_mode_for(c, p, d), main_mode, a, part, B * d.Wy, nb_csc + tab_bytes);
             if (rc) return rc;
         }
         if (d.root) HIP_TRY(c, launch_root_reduce(rootv, c->n_act, B, tmp, rawd, st));
@@ -538,7 +638,7 @@ int choose_tile(pg_ctx* c, const pg_plan* p, int ncol, size_t tsz, int* tile, Me
     const int cap = c->batch_tile > 0 ? (int)std::min<int64_t>(c->batch_tile, ncol) : std::min(ncol, 64);
     for (int B = cap; B >= 1; --B) {
         size_t lds = 0;
-        for (auto& d : p->subs) lds = std::max(lds, sub_lds_max(p, *d, B, tsz));
+        for (auto& d : p->subs) lds = std::max(lds, sub_lds_max(c, p, *d, B, tsz));
         if (lds > kLdsLimit) {
             if (c->batch_tile > 0) break;
             continue;
@@ -571,7 +671,7 @@ void finish_timing(pg_ctx* c)
     }
 }
 
-void drop_graphs(pg_ctx* c, const pg_plan* only = nullptr)
+void drop_graphs(pg_ctx* c, const pg_plan* only)
 {
     for (auto it = c->graphs.begin(); it != c->graphs.end();) {
         if (only && it->first.plan != only) {
@@ -811,6 +911,8 @@ int enqueue_backward(pg_ctx* c, const pg_plan* p, int B, const GradPlan& gp)
             a.Wy = d.Wy;
             a.Wo = d.Wy;
             a.ymask = d.ymask.as<uint16_t>();
+            a.gidx = d.gidx.as<int16_t>();
+            a.Wy4 = d.Wy4;
             a.out = Ybuf;
             a.ldo = (int64_t)B * d.Wy;
             int rc = launch_gathering<T>(c, AUX_GATHER, MAIN_LEAF, a, part, B * d.Wy, 0.0);
@@ -898,7 +1000,7 @@ int count_grad_impl(pg_ctx* c, const pg_plan* p, int ncol, const uint8_t* colors
     const int cap = c->batch_tile > 0 ? (int)std::min<int64_t>(c->batch_tile, ncol) : std::min(ncol, 64);
     for (int B = cap; B >= 1; --B) {
         size_t lds = 0;
-        for (auto& d : p->subs) lds = std::max(lds, sub_lds_max(p, *d, B, tsz));
+        for (auto& d : p->subs) lds = std::max(lds, sub_lds_max(c, p, *d, B, tsz));
         if (lds > kLdsLimit) continue;
         GradPlan g = plan_memory_grad(c, p, B, tsz);
         if (g.total > budget) continue;
@@ -992,6 +1094,48 @@ std::vector<int16_t> colex_rank_table(int K)
     return rank;
 }
 
+// SubArgs::gidx: for a vertex of color cv, Y value i (the K-bit set ymask[i]
+// relative to cv) takes from a neighbour of color cu its row value at the
+// rank of (set minus cu) relative to cu, when cu is in the set
+std::vector<int16_t> gather_index_table(int k, const std::vector<uint16_t>& ymask, const std::vector<int16_t>& rank,
+                                        int Wy4)
+{
+    const int Wy = (int)ymask.size();
+    std::vector<int16_t> g((size_t)(k + 1) * k * Wy4, (int16_t)-1);
+    auto expand = [](uint32_t m, uint32_t c) { return (m & ((1u << c) - 1u)) | ((m >> c) << (c + 1u)); };
+    auto squeeze = [](uint32_t m, uint32_t c) { return (m & ((1u << c) - 1u)) | ((m >> (c + 1u)) << c); };
+    for (int cv = 0; cv < k; ++cv)
+        for (int cu = 0; cu < k; ++cu) {
+            if (cu == cv) continue;
+            for (int i = 0; i < Wy; ++i) {
+                const uint32_t A = expand(ymask[i], (uint32_t)cv);
+                if (!((A >> cu) & 1u)) continue;
+                g[((size_t)cv * k + cu) * Wy4 + i] = rank[squeeze(A & ~(1u << cu), (uint32_t)cu)];
+            }
+        }
+    return g;
+}
+
+// SubArgs::xidx: value j of the row of a neighbour of color cu (the set
+// amask[j] relative to cu, plus cu) adds, at a vertex of color cv, to the Y
+// value of that set relative to cv, when cv is outside it
+std::vector<int16_t> rows_index_table(int k, const std::vector<uint16_t>& amask, const std::vector<int16_t>& rank,
+                                      int Wa4)
+{
+    const int Wa = (int)amask.size();
+    std::vector<int16_t> x((size_t)k * k * Wa4, (int16_t)-1);
+    auto expand = [](uint32_t m, uint32_t c) { return (m & ((1u << c) - 1u)) | ((m >> c) << (c + 1u)); };
+    auto squeeze = [](uint32_t m, uint32_t c) { return (m & ((1u << c) - 1u)) | ((m >> (c + 1u)) << c); };
+    for (int cv = 0; cv < k; ++cv)
+        for (int cu = 0; cu < k; ++cu)
+            for (int j = 0; j < Wa; ++j) {
+                const uint32_t S = expand(amask[j], (uint32_t)cu) | (1u << cu);
+                if ((S >> cv) & 1u) continue;
+                x[((size_t)cv * k + cu) * Wa4 + j] = rank[squeeze(S, (uint32_t)cv)];
+            }
+    return x;
+}
+
 std::vector<uint16_t> masks_of_size(int K, int size)
 {
     std::vector<uint16_t> out;
@@ -1068,6 +1212,24 @@ extern "C" int pg_set_option(pg_ctx* c, const char* key, int64_t v)
         c->use_graphs = v != 0;
     } else if (k == "autotune") {
         c->autotune = v ? 1 : 0;
+    } else if (k == "aux_rows") {
+        if (v < -1 || v > 1) return set_err(c, PG_ERR_INVALID, "aux_rows must be -1, 0 or 1");
+        if (v != c->aux_rows) {
+            drop_graphs(c);
+            c->aux_rows = v;
+        }
+    } else if (k == "stage_rows") {
+        if (v != 0 && v != 1) return set_err(c, PG_ERR_INVALID, "stage_rows must be 0 or 1");
+        if (v != c->stage_rows) {
+            drop_graphs(c);
+            c->stage_rows = v;
+        }
+    } else if (k == "blocks_per_cu") {
+        if (v < 0 || v > 64) return set_err(c, PG_ERR_INVALID, "blocks_per_cu must be in [0, 64]");
+        if (v != c->blocks_per_cu) {
+            drop_graphs(c);  // captured launches hold the old grid sizes
+            c->blocks_per_cu = v;
+        }
     } else {
         return set_err(c, PG_ERR_INVALID, "unknown option %s", key);
     }
@@ -1086,6 +1248,9 @@ extern "C" int pg_get_option(const pg_ctx* c, const char* key, int64_t* v)
     else if (k == "mem_budget_mb") *v = c->mem_budget_mb;
     else if (k == "timing") *v = c->timing;
     else if (k == "autotune") *v = c->autotune;
+    else if (k == "blocks_per_cu") *v = c->blocks_per_cu;
+    else if (k == "aux_rows") *v = c->aux_rows;
+    else if (k == "stage_rows") *v = c->stage_rows;
     else if (k == "graphs") *v = c->use_graphs ? 1 : 0;
     else return PG_ERR_INVALID;
     return PG_OK;
@@ -1404,6 +1569,20 @@ extern "C" int pg_plan_create(pg_ctx* c, int k, int nsub, const int32_t* sub_siz
             const std::vector<uint16_t> am = masks_of_size(K, d->aa - 1);
             HIP_TRY(c, d->amask.alloc(am.size() * 2));
             HIP_TRY(c, hipMemcpy(d->amask.p, am.data(), am.size() * 2, hipMemcpyHostToDevice));
+            d->Wy4 = (d->Wy + kRegCols - 1) / kRegCols * kRegCols;
+            const size_t gbytes = (size_t)2 * (k + 1) * k * d->Wy4;
+            if (gbytes <= kGidxMaxBytes) {
+                const std::vector<int16_t> g = gather_index_table(k, ym, rank, d->Wy4);
+                HIP_TRY(c, d->gidx.alloc(gbytes));
+                HIP_TRY(c, hipMemcpy(d->gidx.p, g.data(), gbytes, hipMemcpyHostToDevice));
+            }
+            d->Wa4 = (d->Wa + 3) / 4 * 4;
+            const size_t xbytes = (size_t)2 * k * k * d->Wa4;
+            if (xbytes <= kGidxMaxBytes) {
+                const std::vector<int16_t> x = rows_index_table(k, am, rank, d->Wa4);
+                HIP_TRY(c, d->xidx.alloc(xbytes));
+                HIP_TRY(c, hipMemcpy(d->xidx.p, x.data(), xbytes, hipMemcpyHostToDevice));
+            }
         }
         plan->subs.push_back(std::move(d));
     }

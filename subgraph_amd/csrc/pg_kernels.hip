@@ -77,7 +77,27 @@ __device__ __forceinline__ T aux_term(const SubArgs<T>& a, const int16_t* s_rank
 
 __host__ __device__ constexpr size_t rank_lds_bytes(int K) { return (((size_t)2 << K) + 15) & ~(size_t)15; }
 
-__host__ __device__ constexpr bool gathers_rows(int aux) { return aux == AUX_GATHER || aux == AUX_GATHER_T; }
+// per-lane gathers of neighbour rows (through the rank table or the gather index table)
+__host__ __device__ constexpr bool gathers_rows(int aux)
+{
+    return aux == AUX_GATHER || aux == AUX_GATHER_T || aux == AUX_STAGED;
+}
+// launches that stage the rank table (and, row-wise, the aux row masks) in LDS
+__host__ __device__ constexpr bool uses_rank(int aux) { return gathers_rows(aux) || aux == AUX_ROWS; }
+// launches that walk a vertex's entries
+__host__ __device__ constexpr bool walks_entries(int aux) { return uses_rank(aux) || aux == AUX_COLORS; }
+
+// LDS before the per-team rows: the gather index table (AUX_GATHER with
+// gidx), else the rank table, then (AUX_ROWS) the Wa row masks
+template <typename T>
+__host__ __device__ size_t head_lds_bytes(int aux, const SubArgs<T>& a)
+{
+    if ((aux == AUX_GATHER || aux == AUX_STAGED) && a.gidx)
+        return ((size_t)2 * (a.k + 1) * a.k * a.Wy4 + 15) & ~(size_t)15;
+    if (aux == AUX_ROWS && a.xidx) return ((size_t)2 * a.k * a.k * a.Wa4 + 15) & ~(size_t)15;
+    return !uses_rank(aux) ? 0
+                           : rank_lds_bytes(a.K) + (aux == AUX_ROWS ? (((size_t)2 * a.Wa + 15) & ~(size_t)15) : 0);
+}
 
 }  // namespace
 
@@ -85,9 +105,12 @@ template <typename T>
 size_t sub_lds_bytes(int team, int aux, int main_mode, const SubArgs<T>& a)
 {
     const int teams = kBlock / team;
-    size_t bytes = gathers_rows(aux) ? rank_lds_bytes(a.K) : 0;
-    if (a.out_mode == OUT_Y || a.out_mode == OUT_PARTIAL) return bytes;
-    const size_t per = (size_t)a.B * a.Wy + (main_mode == MAIN_TABLE ? (size_t)a.B * a.Wm : 0);
+    const size_t bytes = head_lds_bytes(aux, a);
+    const size_t ycopies = aux == AUX_ROWS ? (size_t)a.ng : 1;  // one Y accumulator per neighbour group
+    const size_t stage = aux == AUX_STAGED ? (size_t)a.ng * stage_group_stride(a.segpad, team / a.ng) : 0;
+    const bool y_in_lds = !(a.out_mode == OUT_Y || a.out_mode == OUT_PARTIAL) || aux == AUX_ROWS;
+    const bool keeps_main = main_mode == MAIN_TABLE && a.out_mode != OUT_Y && a.out_mode != OUT_PARTIAL;
+    const size_t per = (y_in_lds ? ycopies * a.B * a.Wy : 0) + (keeps_main ? (size_t)a.B * a.Wm : 0) + stage;
     return bytes + (size_t)teams * per * sizeof(T);
 }
 
@@ -104,21 +127,47 @@ __global__ __launch_bounds__(kBlock) void k_sub(const SubArgs<T> a)
     const int B = a.B;
     const int mode = a.out_mode;
     const int nY = B * a.Wy;
-    const int nM = (MAIN == MAIN_TABLE) ? B * a.Wm : 0;
-    const size_t rank_bytes = gathers_rows(AUX) ? rank_lds_bytes(a.K) : 0;
-    int16_t* s_rank = reinterpret_cast<int16_t*>(smem_raw);
-    T* sY = reinterpret_cast<T*>(smem_raw + rank_bytes) + (size_t)team * (nY + nM);
-    T* sM = sY + nY;
-
-    if (gathers_rows(AUX)) {
-        for (int i = threadIdx.x; i < (1 << a.K); i += kBlock) s_rank[i] = a.rank_tab[i];
-        __syncthreads();
-    }
-
+    const int nM = (MAIN == MAIN_TABLE && mode != OUT_Y && mode != OUT_PARTIAL) ? B * a.Wm : 0;
     const int ng = a.ng;
     const int cl = TEAM / ng;   // column lanes per group
     const int grp = lane / cl;
     const int cln = lane - grp * cl;
+    const int ycopies = AUX == AUX_ROWS ? ng : 1;
+    const size_t head = head_lds_bytes(AUX, a);
+    const bool use_gidx = (AUX == AUX_GATHER || AUX == AUX_STAGED) && a.gidx != nullptr;
+    constexpr bool staged = AUX == AUX_STAGED;
+    const bool y_in_lds = !(mode == OUT_Y || mode == OUT_PARTIAL) || AUX == AUX_ROWS;
+    const int gstride = staged ? stage_group_stride(a.segpad, TEAM / ng) : 0;
+    const int nstage = ng * gstride;
+    const int per_team = (y_in_lds ? ycopies * nY : 0) + nM + nstage;
+    int16_t* s_rank = reinterpret_cast<int16_t*>(smem_raw);
+    int16_t* s_gidx = reinterpret_cast<int16_t*>(smem_raw);
+    uint16_t* s_xmask = reinterpret_cast<uint16_t*>(smem_raw + rank_lds_bytes(a.K));
+    T* sY = reinterpret_cast<T*>(smem_raw + head) + (size_t)team * per_team;
+    T* sM = sY + (y_in_lds ? ycopies * nY : 0);
+    T* s_stage = sM + nM;
+
+    const bool use_xidx = AUX == AUX_ROWS && a.xidx != nullptr;
+    if (use_xidx) {
+        // k*k rows of Wa4 int16: a whole number of 8-byte words
+        const int words = a.k * a.k * a.Wa4 / 4;
+        const int2* src = reinterpret_cast<const int2*>(a.xidx);
+        int2* dst = reinterpret_cast<int2*>(s_gidx);
+        for (int i = threadIdx.x; i < words; i += kBlock) dst[i] = src[i];
+        __syncthreads();
+    } else if (use_gidx) {
+        // (k+1)*k rows of Wy4 int16: a whole number of 8-byte words
+        const int words = (a.k + 1) * a.k * a.Wy4 / 4;
+        const int2* src = reinterpret_cast<const int2*>(a.gidx);
+        int2* dst = reinterpret_cast<int2*>(s_gidx);
+        for (int i = threadIdx.x; i < words; i += kBlock) dst[i] = src[i];
+        __syncthreads();
+    } else if (uses_rank(AUX)) {
+        for (int i = threadIdx.x; i < (1 << a.K); i += kBlock) s_rank[i] = a.rank_tab[i];
+        if (AUX == AUX_ROWS)
+            for (int i = threadIdx.x; i < a.Wa; i += kBlock) s_xmask[i] = a.xmask[i];
+        __syncthreads();
+    }
 
     for (int64_t base = (int64_t)blockIdx.x * TEAMS; base < a.n_items; base += (int64_t)gridDim.x * TEAMS) {
         const int64_t item = base + team;
@@ -130,12 +179,97 @@ __global__ __launch_bounds__(kBlock) void k_sub(const SubArgs<T> a)
                 v = it.x; e0 = it.y; e1 = it.z; slot = it.w;
             } else {
                 v = a.item_base + (int)item;
-                if (gathers_rows(AUX) || AUX == AUX_COLORS) { e0 = a.rowptr[v]; e1 = a.rowptr[v + 1]; }
+                if (walks_entries(AUX)) { e0 = a.rowptr[v]; e1 = a.rowptr[v + 1]; }
             }
         }
 
         // ------------------------------------------------------ phase 1: Y
-        if (gathers_rows(AUX) || AUX == AUX_COLORS) {
+        if (AUX == AUX_ROWS) {
+            // Value q = b*Wa + j of neighbour row u is the aux child at set R_j
+            // (relative to col(u)); it belongs to the Y set S = R_j + {col(u)}
+            // (relative to col(v)) when col(v) is outside S, and to no other.
+            // Lane cln of group grp owns values c0 + cln of each chunk and adds
+            // them, neighbour after neighbour, into the group's own LDS copy of
+            // Y (LDS atomics of one wavefront apply in program order, and a
+            // neighbour adds to distinct sets, so the sums are deterministic);
+            // the copies then fold in group order.
+            T* acc = sY + grp * nY;
+            for (int t = cln; t < nY; t += cl) acc[t] = T(0);
+            __syncthreads();
+            if (active) {
+                const int Wa = a.Wa;
+                const int nq = B * Wa;
+                for (int c0 = 0; c0 < nq; c0 += cl) {
+                    const int q = c0 + cln;
+                    const bool qv = q < nq;
+                    const int b = qv ? q / Wa : 0;
+                    const uint32_t xm = qv && !use_xidx ? (uint32_t)s_xmask[q - b * Wa] : 0u;
+                    const uint32_t cv = a.colors[(int64_t)v * B + b];
+                    // xidx row of this value for neighbour color 0 (past the end: -1 rows)
+                    const int xbase = qv ? (int)cv * a.k * a.Wa4 + (q - b * Wa) : 0;
+                    T* accb = acc + b * a.Wy;
+                    const int step = ng;
+                    int e = e0 + grp;
+                    // a neighbour past the end has col(v) (its set holds col(v): adds nothing)
+                    int un[4];
+                    uint32_t cn[4];
+                    T wn[4];
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int ee = e + r * step;
+                        const bool ok = ee < e1;
+                        un[r] = ok ? a.col[ee] : v;
+                        cn[r] = ok ? (uint32_t)a.ecol[(int64_t)ee * B + b] : cv;
+                        wn[r] = ok && a.w ? a.w[ee] : T(1);
+                    }
+                    for (; e < e1; e += 4 * step) {
+                        int u[4];
+                        uint32_t cu[4];
+                        T wv[4];
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) {
+                            u[r] = un[r];
+                            cu[r] = cn[r];
+                            wv[r] = wn[r];
+                            const int ee = e + (4 + r) * step;
+                            const bool ok = ee < e1;
+                            un[r] = ok ? a.col[ee] : v;
+                            cn[r] = ok ? (uint32_t)a.ecol[(int64_t)ee * B + b] : cv;
+                            wn[r] = ok && a.w ? a.w[ee] : T(1);
+                        }
+                        T x[4];
+                        int t[4];
+                        bool ok[4];
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) {
+                            if (use_xidx) {
+                                // cu == cv (also a past-the-end neighbour) reads a -1 row
+                                const int tt = s_gidx[xbase + (int)__umul24(cu[r], a.Wa4)];
+                                ok[r] = qv && tt >= 0;
+                                t[r] = ok[r] ? tt : 0;
+                            } else {
+                                const uint32_t S = expand_bit(xm, cu[r]) | (1u << cu[r]);
+                                ok[r] = qv && !((S >> cv) & 1u);
+                                t[r] = ok[r] ? s_rank[squeeze_bit(S, cv)] : 0;
+                            }
+                            x[r] = ok[r] ? a.aux[(int64_t)u[r] * a.lda + q] : T(0);
+                        }
+#pragma unroll
+                        for (int r = 0; r < 4; ++r)
+                            if (ok[r]) atomicAdd(accb + t[r], a.w ? wv[r] * x[r] : x[r]);
+                    }
+                }
+            }
+            __syncthreads();
+            for (int t = lane; t < nY; t += TEAM) {
+                T sum = sY[t];
+                for (int g = 1; g < ng; ++g) sum = sum + sY[g * nY + t];
+                if (!active) continue;
+                if (mode == OUT_Y) a.out[(int64_t)v * a.ldo + t] = sum;
+                else if (mode == OUT_PARTIAL) a.out[(int64_t)slot * a.ldo + t] = sum;
+                else sY[t] = sum;
+            }
+        } else if (gathers_rows(AUX) || AUX == AUX_COLORS) {
             // a slot is kRegCols consecutive Y values of one coloring; column
             // lane cln of every group owns slot s0 + cln of each chunk
             const int W = gathers_rows(AUX) ? a.Wy : a.k;
@@ -151,12 +285,16 @@ __global__ __launch_bounds__(kBlock) void k_sub(const SubArgs<T> a)
                 // GATHER_T: holds col(v), all ones = unused); COLORS: the value's color
                 uint32_t amask[kRegCols];
                 const uint32_t cv = sv && gathers_rows(AUX) ? a.colors[(int64_t)v * B + b] : 0u;
+                // gidx row of this slot's kRegCols values for neighbour color 0
+                // (a slot past the end reads the all -1 rows)
+                const int gbase = sv ? ((int)cv * a.k) * a.Wy4 + i0 : a.k * a.k * a.Wy4;
 #pragma unroll
                 for (int r = 0; r < kRegCols; ++r) {
                     acc[r] = T(0);
                     const int i = i0 + r;
                     const bool ok = sv && i < W;
                     if (AUX == AUX_GATHER) amask[r] = ok ? expand_bit(a.ymask[i], cv) : 0u;
+                    else if (AUX == AUX_STAGED) amask[r] = 0u;  // the gather index table replaces it
                     else if (AUX == AUX_GATHER_T) amask[r] = ok ? expand_bit(a.ymask[i], cv) | (1u << cv) : ~0u;
                     else amask[r] = ok ? (uint32_t)i : 0xffu;
                 }
@@ -165,43 +303,145 @@ __global__ __launch_bounds__(kBlock) void k_sub(const SubArgs<T> a)
                 // color that adds nothing (col(v) is never in a GATHER set and
                 // always in a GATHER_T set)
                 const uint32_t dead = gathers_rows(AUX) ? cv : 0xffu;
-                if (active) {
+                if (staged && active) {
+                    // Rows staged in LDS: the group copies, per neighbour, the
+                    // segment of its row that holds this chunk's colorings
+                    // (coalesced), then each lane reads its values there.
+                    // Pipeline: ids two batches ahead, rows one batch ahead.
+                    const int step = ng;
+                    const int b_first = s0 / spb;
+                    const int b_last = min(B - 1, (s0 + cl - 1) / spb);
+                    const int seglen = (b_last - b_first + 1) * a.Wa;
+                    const int nt = (seglen + cl - 1) / cl;  // <= kStageRegs (host)
+                    const int spad = a.segpad;
+                    T* stg = s_stage + grp * gstride;
+                    const int lb = sv ? (b - b_first) * a.Wa : 0;
+                    const T* tab = a.aux + b_first * a.Wa;
+                    const int sbase = a.k * a.k * a.Wy4;  // gidx rows of all -1
+                    int uA[4], uB[4], uC[4];
+                    uint32_t cA[4], cB[4], cC[4];
+                    T wA[4], wB[4], wC[4];
+                    bool oA[4], oB[4], oC[4];
+                    auto fetch4 = [&](int eb, int* uo, uint32_t* co, T* wo, bool* oo) {
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) {
+                            const int ee = eb + q * step;
+                            oo[q] = ee < e1;
+                            const int ec = oo[q] ? ee : e1 - 1;  // a real entry of this item
+                            uo[q] = a.col[ec];
+                            co[q] = a.ecol[(int64_t)ec * B + b];
+                            wo[q] = a.w ? a.w[ec] : T(1);
+                        }
+                    };
+                    T rv[4][kStageRegs];
+                    auto load_rows = [&](const int* uo) {
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) {
+                            const T* src = tab + (int64_t)uo[q] * a.lda;
+#pragma unroll
+                            for (int t = 0; t < kStageRegs; ++t)
+                                if (t < nt) rv[q][t] = src[min(cln + t * cl, seglen - 1)];
+                        }
+                    };
+                    int e = e0 + grp;
+                    fetch4(e, uA, cA, wA, oA);
+                    fetch4(e + 4 * step, uB, cB, wB, oB);
+                    load_rows(uA);
+                    for (; e < e1; e += 4 * step) {
+                        fetch4(e + 8 * step, uC, cC, wC, oC);
+#pragma unroll
+                        for (int q = 0; q < 4; ++q)
+#pragma unroll
+                            for (int t = 0; t < kStageRegs; ++t)
+                                if (t < nt) stg[q * spad + cln + t * cl] = rv[q][t];
+                        load_rows(uB);
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) {
+                            const int gb = oA[q] ? gbase : sbase;
+                            const int2 ids =
+                                *reinterpret_cast<const int2*>(s_gidx + gb + (int)__umul24(cA[q], a.Wy4));
+                            const int id[4] = {(int)(int16_t)(ids.x & 0xffff), ids.x >> 16,
+                                               (int)(int16_t)(ids.y & 0xffff), ids.y >> 16};
+                            const T* row = stg + q * spad + lb;
+#pragma unroll
+                            for (int r = 0; r < kRegCols; ++r) {
+                                const T val = row[id[r] < 0 ? 0 : id[r]];
+                                const T xv = id[r] < 0 ? T(0) : val;
+                                acc[r] = a.w ? acc[r] + wA[q] * xv : acc[r] + xv;
+                            }
+                        }
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) {
+                            uA[q] = uB[q]; cA[q] = cB[q]; wA[q] = wB[q]; oA[q] = oB[q];
+                            uB[q] = uC[q]; cB[q] = cC[q]; wB[q] = wC[q]; oB[q] = oC[q];
+                        }
+                    }
+                } else if (!staged && active) {
                     const int step = ng;
                     int e = e0 + grp;
                     // software pipeline: the ids, colors and weights of the
-                    // next 4 neighbours load while the current rows are read
+                    // next 4 neighbours load while the current rows are read.
+                    // Loads are unconditional: an entry past the end reads the
+                    // item's last entry (which exists) and is flagged.  With
+                    // gidx the flag only moves the table base to the all -1
+                    // rows, so no loaded value feeds a select (which would let
+                    // the compiler sink the loads under branches).
                     int un[4];
                     uint32_t cn[4];
                     T wn[4];
+                    bool okn[4];
+                    auto fetch = [&](int ee, int q) {
+                        okn[q] = ee < e1;
+                        const int ec = okn[q] ? ee : e1 - 1;
+                        un[q] = a.col[ec];
+                        cn[q] = a.ecol[(int64_t)ec * B + b];
+                        wn[q] = a.w ? a.w[ec] : T(1);
+                    };
 #pragma unroll
-                    for (int q = 0; q < 4; ++q) {
-                        const int ee = e + q * step;
-                        const bool ok = ee < e1;
-                        un[q] = ok ? a.col[ee] : v;
-                        cn[q] = ok ? (uint32_t)a.ecol[(int64_t)ee * B + b] : dead;
-                        wn[q] = ok && a.w ? a.w[ee] : T(1);
-                    }
+                    for (int q = 0; q < 4; ++q) fetch(e + q * step, q);
+                    const T* rowbase = a.aux + aoff;
+                    const int sbase = a.k * a.k * a.Wy4;  // gidx rows of all -1
                     for (; e < e1; e += 4 * step) {
                         int u[4];
                         uint32_t cu[4];
                         T wv[4];
+                        bool ok[4];
 #pragma unroll
                         for (int q = 0; q < 4; ++q) {
                             u[q] = un[q];
                             cu[q] = cn[q];
                             wv[q] = wn[q];
-                            const int ee = e + (4 + q) * step;
-                            const bool ok = ee < e1;
-                            un[q] = ok ? a.col[ee] : v;
-                            cn[q] = ok ? (uint32_t)a.ecol[(int64_t)ee * B + b] : dead;
-                            wn[q] = ok && a.w ? a.w[ee] : T(1);
+                            ok[q] = okn[q];
+                            fetch(e + (4 + q) * step, q);
                         }
                         T x[4][kRegCols];
+                        if (use_gidx) {
+                            static_assert(kRegCols == 4, "one 8-byte LDS read holds a slot's indices");
 #pragma unroll
-                        for (int q = 0; q < 4; ++q)
+                            for (int q = 0; q < 4; ++q) {
+                                const int gb = ok[q] ? gbase : sbase;
+                                const int2 ids =
+                                    *reinterpret_cast<const int2*>(s_gidx + gb + (int)__umul24(cu[q], a.Wy4));
+                                const int id[4] = {(int)(int16_t)(ids.x & 0xffff), ids.x >> 16,
+                                                   (int)(int16_t)(ids.y & 0xffff), ids.y >> 16};
+                                const T* row = rowbase + (int64_t)u[q] * a.lda;
+                                // every lane reads (index 0 for "no value") and selects
 #pragma unroll
-                            for (int r = 0; r < kRegCols; ++r)
-                                x[q][r] = aux_term<T, AUX>(a, s_rank, amask[r], aoff, u[q], cu[q]);
+                                for (int r = 0; r < kRegCols; ++r) {
+                                    const T val = row[(unsigned)(id[r] < 0 ? 0 : id[r])];
+                                    x[q][r] = id[r] < 0 ? T(0) : val;
+                                }
+                            }
+                        } else {
+#pragma unroll
+                            for (int q = 0; q < 4; ++q) {
+                                const int uq = ok[q] ? u[q] : v;
+                                const uint32_t cq = ok[q] ? cu[q] : dead;
+#pragma unroll
+                                for (int r = 0; r < kRegCols; ++r)
+                                    x[q][r] = aux_term<T, AUX>(a, s_rank, amask[r], aoff, uq, cq);
+                            }
+                        }
 #pragma unroll
                         for (int q = 0; q < 4; ++q)
 #pragma unroll
@@ -313,24 +553,27 @@ __global__ __launch_bounds__(kBlock) void k_sub(const SubArgs<T> a)
 }
 
 // Persistent grid: as many workgroups as can be resident at once (occupancy
-// for this instantiation and LDS size, times the CU count), never more than
-// there are items.
+// for this instantiation and LDS size, or the caller's blocks_per_cu, times
+// the CU count), never more than there are items.
 template <typename T, int TEAM, int AUX, int MAIN>
-static hipError_t launch_one(const SubArgs<T>& a, int num_cus, hipStream_t st)
+static hipError_t launch_one(const SubArgs<T>& a, int num_cus, int blocks_per_cu, hipStream_t st)
 {
     const size_t lds = sub_lds_bytes<T>(TEAM, AUX, MAIN, a);
     const int64_t need = (a.n_items + (kBlock / TEAM) - 1) / (kBlock / TEAM);
     if (need <= 0) return hipSuccess;
     static thread_local std::map<size_t, int> per_cu_cache;
-    int per_cu = 0;
-    auto it = per_cu_cache.find(lds);
-    if (it != per_cu_cache.end()) {
-        per_cu = it->second;
-    } else {
-        hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sub<T, TEAM, AUX, MAIN>, kBlock, lds);
-        if (e != hipSuccess) return e;
-        if (per_cu < 1) return hipErrorInvalidConfiguration;
-        per_cu_cache[lds] = per_cu;
+    int per_cu = blocks_per_cu;
+    if (per_cu <= 0) {
+        auto it = per_cu_cache.find(lds);
+        if (it != per_cu_cache.end()) {
+            per_cu = it->second;
+        } else {
+            hipError_t e =
+                hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sub<T, TEAM, AUX, MAIN>, kBlock, lds);
+            if (e != hipSuccess) return e;
+            if (per_cu < 1) return hipErrorInvalidConfiguration;
+            per_cu_cache[lds] = per_cu;
+        }
     }
     const int64_t cap = (int64_t)per_cu * num_cus;
     const int grid = (int)(need < cap ? need : cap);
@@ -339,43 +582,48 @@ static hipError_t launch_one(const SubArgs<T>& a, int num_cus, hipStream_t st)
 }
 
 template <typename T, int TEAM>
-static hipError_t launch_team(int aux, int main_mode, const SubArgs<T>& a, int max_blocks, hipStream_t st)
+static hipError_t launch_team(int aux, int main_mode, const SubArgs<T>& a, int num_cus, int per_cu, hipStream_t st)
 {
     if (main_mode == MAIN_LEAF) {
         switch (aux) {
-        case AUX_GATHER: return launch_one<T, TEAM, AUX_GATHER, MAIN_LEAF>(a, max_blocks, st);
-        case AUX_GATHER_T: return launch_one<T, TEAM, AUX_GATHER_T, MAIN_LEAF>(a, max_blocks, st);
-        case AUX_COLORS: return launch_one<T, TEAM, AUX_COLORS, MAIN_LEAF>(a, max_blocks, st);
-        case AUX_READ: return launch_one<T, TEAM, AUX_READ, MAIN_LEAF>(a, max_blocks, st);
-        case AUX_PARTIALS: return launch_one<T, TEAM, AUX_PARTIALS, MAIN_LEAF>(a, max_blocks, st);
+        case AUX_GATHER: return launch_one<T, TEAM, AUX_GATHER, MAIN_LEAF>(a, num_cus, per_cu, st);
+        case AUX_ROWS: return launch_one<T, TEAM, AUX_ROWS, MAIN_LEAF>(a, num_cus, per_cu, st);
+        case AUX_STAGED: return launch_one<T, TEAM, AUX_STAGED, MAIN_LEAF>(a, num_cus, per_cu, st);
+        case AUX_GATHER_T: return launch_one<T, TEAM, AUX_GATHER_T, MAIN_LEAF>(a, num_cus, per_cu, st);
+        case AUX_COLORS: return launch_one<T, TEAM, AUX_COLORS, MAIN_LEAF>(a, num_cus, per_cu, st);
+        case AUX_READ: return launch_one<T, TEAM, AUX_READ, MAIN_LEAF>(a, num_cus, per_cu, st);
+        case AUX_PARTIALS: return launch_one<T, TEAM, AUX_PARTIALS, MAIN_LEAF>(a, num_cus, per_cu, st);
         default: return hipErrorInvalidValue;
         }
     }
     switch (aux) {
-    case AUX_GATHER: return launch_one<T, TEAM, AUX_GATHER, MAIN_TABLE>(a, max_blocks, st);
-    case AUX_READ: return launch_one<T, TEAM, AUX_READ, MAIN_TABLE>(a, max_blocks, st);
-    case AUX_PARTIALS: return launch_one<T, TEAM, AUX_PARTIALS, MAIN_TABLE>(a, max_blocks, st);
+    case AUX_GATHER: return launch_one<T, TEAM, AUX_GATHER, MAIN_TABLE>(a, num_cus, per_cu, st);
+    case AUX_ROWS: return launch_one<T, TEAM, AUX_ROWS, MAIN_TABLE>(a, num_cus, per_cu, st);
+    case AUX_STAGED: return launch_one<T, TEAM, AUX_STAGED, MAIN_TABLE>(a, num_cus, per_cu, st);
+    case AUX_READ: return launch_one<T, TEAM, AUX_READ, MAIN_TABLE>(a, num_cus, per_cu, st);
+    case AUX_PARTIALS: return launch_one<T, TEAM, AUX_PARTIALS, MAIN_TABLE>(a, num_cus, per_cu, st);
     default: return hipErrorInvalidValue;
     }
 }
 
 template <typename T>
-hipError_t launch_sub(int team, int aux, int main_mode, const SubArgs<T>& a, int max_blocks, hipStream_t st)
+hipError_t launch_sub(int team, int aux, int main_mode, const SubArgs<T>& a, int num_cus, int blocks_per_cu,
+                      hipStream_t st)
 {
     if (a.n_items <= 0) return hipSuccess;
     if (a.ng < 1 || team % a.ng != 0 || (a.ng & (a.ng - 1)) != 0) return hipErrorInvalidValue;
-    if (gathers_rows(aux) && (a.K < 1 || a.K > kMaxK - 1)) return hipErrorInvalidValue;
+    if (uses_rank(aux) && (a.K < 1 || a.K > kMaxK - 1)) return hipErrorInvalidValue;
     switch (team) {
-    case 8: return launch_team<T, 8>(aux, main_mode, a, max_blocks, st);
-    case 16: return launch_team<T, 16>(aux, main_mode, a, max_blocks, st);
-    case 32: return launch_team<T, 32>(aux, main_mode, a, max_blocks, st);
-    case 64: return launch_team<T, 64>(aux, main_mode, a, max_blocks, st);
+    case 8: return launch_team<T, 8>(aux, main_mode, a, num_cus, blocks_per_cu, st);
+    case 16: return launch_team<T, 16>(aux, main_mode, a, num_cus, blocks_per_cu, st);
+    case 32: return launch_team<T, 32>(aux, main_mode, a, num_cus, blocks_per_cu, st);
+    case 64: return launch_team<T, 64>(aux, main_mode, a, num_cus, blocks_per_cu, st);
     default: return hipErrorInvalidValue;
     }
 }
 
-template hipError_t launch_sub<float>(int, int, int, const SubArgs<float>&, int, hipStream_t);
-template hipError_t launch_sub<double>(int, int, int, const SubArgs<double>&, int, hipStream_t);
+template hipError_t launch_sub<float>(int, int, int, const SubArgs<float>&, int, int, hipStream_t);
+template hipError_t launch_sub<double>(int, int, int, const SubArgs<double>&, int, int, hipStream_t);
 
 // ------------------------------------------------------------ utilities ---
 

@@ -25,6 +25,10 @@ enum AuxMode : int {
     AUX_PARTIALS = 3,  // Y = ordered sum of a hub's partial rows
     AUX_GATHER_T = 4,  // adjoint of AUX_GATHER: the vertex's row entry S (a set holding its color)
                        // sums the neighbours' dY at S, over neighbours whose color is not in S
+    AUX_ROWS = 5,      // AUX_GATHER computed row-wise: each neighbour row is read whole (coalesced)
+                       // and every value is added to the one Y set it belongs to
+    AUX_STAGED = 6,    // AUX_GATHER through the gather index table with neighbour rows staged in
+                       // LDS (needs SubArgs::gidx and segpad)
 };
 // Where the main-child values come from.
 enum MainMode : int {
@@ -42,7 +46,8 @@ enum OutMode : int {
 
 constexpr int kBlock = 256;  // threads per workgroup (4 wavefronts of 64)
 constexpr int kRegCols = 4;  // Y values a lane accumulates per neighbour sweep
-constexpr int kMaxK = 14;    // colors supported on the device (rank table 2^(k-1) int16 in LDS)
+constexpr int kMaxK = 14;      // colors supported on the device (rank table 2^(k-1) int16 in LDS)
+constexpr int kStageRegs = 4;  // staged row values per lane per neighbour
 
 // A work item is (vertex, first entry, end entry, partial slot).  Items may
 // be explicit (hub chunks) or implicit: item i is vertex item_base + i with
@@ -64,19 +69,33 @@ struct SubArgs {
     int32_t ng;              // neighbour groups per team (power of two)
     // aux side
     const T* aux;            // aux table / H / partial rows
-    int64_t lda;             // aux row stride (B * Wa, or B * k for H)
+    int32_t lda;             // aux row stride (B * Wa, or B * k for H)
     int32_t Wa;              // aux values per coloring in a row of `aux`
     int32_t Wy;              // Y values per coloring
     const uint16_t* ymask;   // [Wy] K-bit mask of each Y set (AUX_GATHER)
+    const uint16_t* xmask;   // [Wa] K-bit mask of each aux-child row set (AUX_ROWS)
     const int16_t* rank_tab; // [2^K] colex rank of a K-bit mask among masks of its popcount
+    // AUX_GATHER, optional: gidx[(cv*k + cu)*Wy4 + i] = index in a neighbour's
+    // row (color cu) of the value Y value i (vertex color cv) takes from it, or
+    // -1; rows k*k .. k*k+k-1 are all -1 (a slot past the end reads there)
+    const int16_t* gidx;
+    int32_t Wy4;             // Wy rounded up to kRegCols
+    // AUX_GATHER with gidx, optional (0 = off): neighbour rows are staged in
+    // LDS, 4 per neighbour group, each a segment of segpad values (the
+    // colorings of one chunk of slots); segpad <= kStageRegs * column lanes
+    int32_t segpad;
+    // AUX_ROWS, optional: xidx[(cv*k + cu)*Wa4 + j] = the Y value (vertex color
+    // cv) that value j of a neighbour row (color cu) adds to, or -1
+    const int16_t* xidx;
+    int32_t Wa4;             // Wa rounded up to 4
     const int32_t* slot_begin;  // hub h's partial slots [slot_begin[h], slot_begin[h+1])
     // main side
     const T* main;
-    int64_t ldm;
+    int32_t ldm;
     int32_t Wm;
     // output
     T* out;
-    int64_t ldo;
+    int32_t ldo;
     int32_t Wo;              // output values per coloring
     int32_t P;               // non-zero splits per output set
     const uint32_t* split;   // [Wo][P] (main_rank << 16 | y_rank), reference order
@@ -86,14 +105,24 @@ struct SubArgs {
     float scale;
 };
 
+// LDS stride between neighbour groups' staged rows (4 rows of segpad values,
+// segpad a multiple of cl): for cl < 32 one more cl, an odd multiple of cl,
+// so the groups' lanes of one wavefront spread over the 32 banks
+__host__ __device__ constexpr int stage_group_stride(int segpad, int cl)
+{
+    return cl >= 32 ? 4 * segpad : 4 * segpad + cl;
+}
+
 template <typename T>
 size_t sub_lds_bytes(int team, int aux, int main_mode, const SubArgs<T>& a);
 
-// Launches k_sub<T, team, aux, main> as a persistent grid (resident
-// workgroups per CU x num_cus); returns hipErrorInvalidValue for an
-// unsupported combination.
+// Launches k_sub<T, team, aux, main> as a persistent grid of
+// blocks_per_cu x num_cus workgroups (blocks_per_cu = 0: the occupancy
+// query's resident count), never more than there are items; returns
+// hipErrorInvalidValue for an unsupported combination.
 template <typename T>
-hipError_t launch_sub(int team, int aux, int main_mode, const SubArgs<T>& a, int num_cus, hipStream_t st);
+hipError_t launch_sub(int team, int aux, int main_mode, const SubArgs<T>& a, int num_cus, int blocks_per_cu,
+                      hipStream_t st);
 
 // Per-pass values read from device memory, so that one captured graph serves
 // every pass of the same shape (the host writes them before each replay).

@@ -102,6 +102,52 @@ def test_random_graphs_and_trees(gpu_ctx, hub):
         gpu_ctx.set_option("hub_chunk", saved[1])
 
 
+VARIANTS = [  # (aux_rows, stage_rows): every neighbour-sum kernel the library can pick
+    (0, 0),   # direct gathers through the gather index table
+    (0, 1),   # gathers from rows staged in LDS
+    (1, 0),   # row-wise sums with LDS atomics
+    (-1, 1),  # per-subtemplate choice
+]
+
+
+@pytest.mark.parametrize("aux_rows,stage_rows", VARIANTS)
+@pytest.mark.parametrize("B", [1, 5, 16])
+def test_kernel_variants_match_oracle(gpu_ctx, aux_rows, stage_rows, B):
+    """Every neighbour-sum variant, at several batch widths (segments of
+    one or several colorings per chunk), against the oracle: FP64 exact,
+    FP32 within the tolerance; the hub split path included."""
+    rng = np.random.default_rng(500 + 10 * B + 3 * aux_rows + stage_rows)
+    saved = [gpu_ctx.get_option(k) for k in ("aux_rows", "stage_rows", "hub_degree", "hub_chunk")]
+    gpu_ctx.set_option("aux_rows", aux_rows)
+    gpu_ctx.set_option("stage_rows", stage_rows)
+    gpu_ctx.set_option("hub_degree", 24)
+    gpu_ctx.set_option("hub_chunk", 24)
+    try:
+        for trial, (k, te) in enumerate([(7, TREE7), (5, [(0, 1), (1, 2), (2, 3), (3, 4)]),
+                                         (10, [(0, 1), (1, 2), (2, 3), (3, 4), (1, 5), (5, 6), (5, 7), (7, 8),
+                                               (7, 9)])]):
+            n = int(rng.integers(60, 300))
+            s, d = _simple_graph(rng, n, int(rng.integers(4 * n, 10 * n)))
+            colptr, rowidx = sg.csc_from_edges(n, s, d)
+            gpu_ctx.upload_graph(colptr, rowidx)
+            print(f"[variants] aux_rows={aux_rows} stage_rows={stage_rows} B={B} k={k} n={n} nnz={len(rowidx)}",
+                  flush=True)
+            plan = gpu_ctx.plan(k, te)
+            op = oracle.Plan(k, te)
+            cols = rng.integers(0, k, size=(B, n)).astype(np.uint8)
+            g32 = gpu_ctx.count(plan, B, colors=cols, precision="fp32")
+            g64 = gpu_ctx.count(plan, B, colors=cols, precision="fp64")
+            for i in range(B):
+                o64 = oracle.count_f64(op, colptr, rowidx, cols[i])
+                assert g64[i] == o64, (trial, i, g64[i], o64)
+                if i < 3:
+                    o32 = oracle.count_f32(op, colptr, rowidx, cols[i])
+                    assert _rel(g32[i], o32) <= TOL32, (trial, i, g32[i], o32)
+    finally:
+        for key, v in zip(("aux_rows", "stage_rows", "hub_degree", "hub_chunk"), saved):
+            gpu_ctx.set_option(key, v)
+
+
 def test_fp64_equals_brute_force(gpu_ctx):
     rng = np.random.default_rng(8)
     trees = [(3, [(0, 1), (1, 2)]), (5, [(0, 1), (0, 2), (0, 3), (0, 4)]), (5, [(0, 1), (1, 2), (2, 3), (3, 4)]),
