@@ -127,6 +127,107 @@ def fwd_bwd_variant(sync):
     return out
 
 
+TREE10 = [(0, 1), (1, 2), (2, 3), (3, 4), (1, 5), (5, 6), (5, 7), (7, 8), (7, 9)]
+TREE12 = [(0, 1), (1, 2), (2, 3), (3, 4), (4, 5), (2, 6), (6, 7), (7, 8), (6, 9), (9, 10), (10, 11)]
+U5 = [(0, 1), (1, 2), (2, 3), (3, 4)]
+
+
+def hashed_weights(n, colptr, rowidx):
+    """Symmetric entry weights U(0.5, 1.5) in FP32: a splitmix64 hash of the
+    undirected edge {u, v}, so both entries of an edge get the same value."""
+    rows = np.repeat(np.arange(n, dtype=np.uint64), np.diff(colptr).astype(np.int64))
+    cols = rowidx.astype(np.uint64)
+    x = np.minimum(rows, cols) * np.uint64(n) + np.maximum(rows, cols)
+    with np.errstate(over="ignore"):
+        x = x + np.uint64(0x9E3779B97F4A7C15)
+        x = (x ^ (x >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        x = (x ^ (x >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        x = x ^ (x >> np.uint64(31))
+    return (0.5 + (x >> np.uint64(40)).astype(np.float64) / float(1 << 24)).astype(np.float32)
+
+
+def dominant_kernel(ctx, run):
+    """Run once with per-launch HIP events; the dominant kernel's name,
+    launches, ms per launch and algorithmic GB/s."""
+    ctx.set_option("timing", 1)
+    try:
+        run()
+        kern = ctx.last_timing_kernels()
+    finally:
+        ctx.set_option("timing", 0)
+    name, (ms, nbytes, nl) = max(kern.items(), key=lambda kv: kv[1][0])
+    return {"kernel": name, "launches": nl, "avg_launch_ms": ms / nl,
+            "achieved_gbs": nbytes / (ms * 1e-3) / 1e9, "frac": nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+
+
+TUNED_KEYS = ("stage_rows", "aux_rows", "bucket_high", "bucket_mid", "hub_degree", "batch_tile")
+
+
+def s22_variant(sync, with_grad=True):
+    """Batching on R-MAT scale 22, edge factor 16 (seed 1, dedup,
+    symmetrized): 64 colorings per pass of the 5-path, the 7-vertex tree and
+    the 10-vertex tree (autotuned), each beside its plain one-coloring pass,
+    then an uneven mixed batch of the three templates in one call; then the
+    12-vertex tree forward and forward+backward at unit and U(0.5,1.5)
+    weights, 16 colorings.  FP32 tables."""
+    import subgraph_amd as sg
+    t0 = time.perf_counter()
+    n = 1 << 22
+    src, dst = sg.rmat_edges(22, 16, seed=1)
+    colptr, rowidx = sg.csc_from_edges(n, src, dst)
+    del src, dst
+    ctx = sg.Context(0)
+    ctx.upload_graph(colptr, rowidx)
+    st = ctx.stats()
+    log(f"s22: n={n} nnz={len(rowidx)} n_active={st['n_active']} ({time.perf_counter() - t0:.1f}s)")
+    out = {"graph": "R-MAT scale 22 edge factor 16 (a=.57 b=.19 c=.19 seed 1), dedup, symmetrized",
+           "n": n, "nnz": int(len(rowidx)), "n_active": st["n_active"], "dtype": "fp32", "templates": {}}
+    ctx.set_option("autotune", 1)
+    plans = {}
+    for name, nv, te, reps in (("u5", 5, U5, 3), ("tree7", 7, TREE7, 2), ("tree10", 10, TREE10, 1)):
+        plan = plans[name] = ctx.plan(nv, te)
+        B = 64
+        t1 = time.perf_counter()
+        ctx.count(plan, B, seed=11)  # first call: autotuning of this (plan, tile)
+        sync()
+        tune_s = time.perf_counter() - t1
+        tuned = {k: ctx.get_option(k) for k in TUNED_KEYS}
+        t_b = _timed(lambda r=0: ctx.count(plan, B, seed=200 + r), reps, sync)
+        ctx.set_option("autotune", 0)
+        t_p = _timed(lambda r=0: ctx.count(plan, 1, seed=300 + r), 3, sync)
+        ctx.set_option("autotune", 1)
+        dom = dominant_kernel(ctx, lambda: ctx.count(plan, B, seed=12))
+        rec = {"batched_colorings_per_call": B, "batched_colorings_per_s": B / t_b,
+               "plain_colorings_per_s": 1.0 / t_p, "batched_vs_plain": (B / t_b) * t_p,
+               "tuned": tuned, "tuning_seconds": tune_s, "roofline_dominant": dom}
+        out["templates"][name] = rec
+        log(f"s22 {name}: batched {B / t_b:.1f}/s, plain {1 / t_p:.1f}/s ({rec['batched_vs_plain']:.2f}x), "
+            f"dominant {dom['kernel']} {dom['achieved_gbs']:.0f} GB/s, tuned {tuned} in {tune_s:.1f}s")
+    counts = [64, 40, 16]
+    pl = [plans["u5"], plans["tree7"], plans["tree10"]]
+    ctx.count_multi(pl, counts, [21, 22, 23])
+    t_m = _timed(lambda r=0: ctx.count_multi(pl, counts, [31 + r, 41 + r, 51 + r]), 1, sync)
+    out["mixed"] = {"templates": ["u5", "tree7", "tree10"], "colorings": counts,
+                    "colorings_per_s": sum(counts) / t_m, "seconds": t_m}
+    log(f"s22 mixed {counts}: {sum(counts) / t_m:.1f} colorings/s")
+    if with_grad:
+        ctx.set_option("autotune", 0)
+        plan = ctx.plan(12, TREE12)
+        rec = {}
+        for wname in ("unit", "uniform_0.5_1.5"):
+            if wname == "unit":
+                ctx.set_weights(None)
+            else:
+                ctx.set_weights(hashed_weights(n, colptr, rowidx))
+            t_f = _timed(lambda r=0: ctx.count(plan, 16, seed=400 + r), 1, sync)
+            t_fb = _timed(lambda r=0: ctx.count_grad(plan, 16, seed=500 + r, precision="fp32"), 1, sync)
+            rec[wname] = {"fwd_colorings_per_s": 16 / t_f, "fwd_bwd_colorings_per_s": 16 / t_fb,
+                          "fwd_bwd_vs_fwd": t_f / t_fb}
+            log(f"s22 tree12 {wname}: fwd {16 / t_f:.2f}/s, fwd+bwd {16 / t_fb:.2f}/s")
+        out["tree12_fwd_bwd"] = {"colorings_per_call": 16, **rec}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -135,7 +236,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--cpu-colorings", type=int, default=12)
-    ap.add_argument("--variants", default="grad", help="comma list of extra measurements: grad")
+    ap.add_argument("--variants", default="grad,s22", help="comma list of extra measurements: grad, s22, none")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print(json.dumps({"error": "this build measures one GPU: bench.py supports --gpus 1 only; "
@@ -244,9 +345,18 @@ def main():
         except (subprocess.SubprocessError, OSError) as e:
             result["cpu_baseline"] = {"value": None, "unit": "colorings/s", "cores": physical_cores(),
                                       "kind": "reference", "sample": f"failed: {e}"}
-    if "grad" in args.variants.split(","):
+    variants = args.variants.split(",")
+    if "grad" in variants or "s22" in variants:
         del ctx, plan
-        result["fwd_bwd"] = fwd_bwd_variant(sync)
+    # the extra measurements never cost the primary line: a failure is recorded
+    for name, key, fn in (("grad", "fwd_bwd", fwd_bwd_variant), ("s22", "s22", s22_variant)):
+        if name not in variants:
+            continue
+        try:
+            result[key] = fn(sync)
+        except Exception as e:  # noqa: BLE001
+            log(f"variant {name} failed: {e!r}")
+            result[key] = {"error": repr(e)}
     print(json.dumps(result), flush=True)
 
 

@@ -57,7 +57,10 @@ const char* pg_last_error(const pg_ctx* ctx);
  *   "bucket_mid"   ... at least this degree 16-lane teams, the rest 8 (default 8)
  *   "batch_tile"   colorings per pass, 0 = automatic (default 0)
  *   "mem_budget_mb" device bytes the tables may use, 0 = 90% of free memory
- *   "autotune"     1 = pick hub_degree/hub_chunk/batch_tile by timing (default 0)
+ *   "autotune"     1 = before the first pass of each (plan, batch tile, precision), time
+ *                  one pass per candidate of stage_rows, aux_rows, bucket_high,
+ *                  bucket_mid and hub_degree (hub_chunk follows hub_degree) and keep
+ *                  the fastest; pg_get_option then reports the choice (default 0)
  *   "timing"       1 = event pair around every launch, passes run eagerly (default 0)
  *   "graphs"       1 = each pass replays a captured hipGraph (default 1)
  *   "blocks_per_cu" workgroups per CU of persistent launches, 0 = occupancy query

@@ -154,6 +154,7 @@ struct pg_ctx {
     bool use_graphs = true;
     uint64_t generation = 0;
     std::map<GraphKey, PassGraph> graphs;
+    std::map<std::tuple<const pg_plan*, int, int>, std::vector<int64_t>> tuned;  // (plan, B, tsize) -> settings
     // timing of the last call
     std::vector<LaunchRec>* cur_recs = nullptr;
     std::map<std::string, KernelTotals> agg;
@@ -733,6 +734,123 @@ int run_pass(pg_ctx* c, const pg_plan* p, int B, int64_t first, const uint8_t* d
     return PG_OK;
 }
 
+// ---------------------------------------------------------------- autotuner
+//
+// Option "autotune" = 1: before the first pass of a (plan, batch tile,
+// precision) the library times one pass per candidate setting of the kernel
+// variant and degree-bucket knobs (coordinate descent, one sweep over the
+// knobs below, each candidate kept only if at least 2% faster), applies the
+// fastest and remembers it for that key.  Tuning passes draw their colorings
+// on the device from a fixed seed and discard the counts.
+
+struct Knob {
+    const char* name;
+    int64_t pg_ctx::*field;
+    std::vector<int64_t> values;
+};
+
+const std::vector<Knob>& tune_knobs()
+{
+    static const std::vector<Knob> knobs = {
+        {"stage_rows", &pg_ctx::stage_rows, {0, 1}},
+        {"aux_rows", &pg_ctx::aux_rows, {0, -1}},
+        {"bucket_high", &pg_ctx::bucket_high, {16, 32, 64}},
+        {"bucket_mid", &pg_ctx::bucket_mid, {4, 8}},
+        {"hub_degree", &pg_ctx::hub_degree, {128, 256, 1024}},
+    };
+    return knobs;
+}
+
+std::vector<int64_t> current_settings(const pg_ctx* c)
+{
+    std::vector<int64_t> v;
+    for (const Knob& k : tune_knobs()) v.push_back(c->*k.field);
+    return v;
+}
+
+void apply_settings(pg_ctx* c, const std::vector<int64_t>& v)
+{
+    const auto& knobs = tune_knobs();
+    for (size_t i = 0; i < knobs.size(); ++i) {
+        if (c->*knobs[i].field != v[i]) drop_graphs(c);
+        c->*knobs[i].field = v[i];
+    }
+    c->hub_chunk = c->hub_degree;  // hub work items as long as the hub threshold
+}
+
+// Device time of one eager pass of B device-drawn colorings with the current
+// settings (best of two after a warmup; a first pass over 50 ms is taken as
+// is), or +inf when they do not fit the LDS limit or the arena as already
+// sized.
+template <typename T>
+int time_pass(pg_ctx* c, const pg_plan* p, int B, double* ms)
+{
+    *ms = INFINITY;
+    int rc = build_buckets(c);
+    if (rc) return rc;
+    size_t lds = 0;
+    for (auto& d : p->subs) lds = std::max(lds, sub_lds_max(c, p, *d, B, sizeof(T)));
+    const MemPlan mp = plan_memory(c, p, B, sizeof(T));
+    if (lds > kLdsLimit || mp.total > c->d_arena.bytes) return PG_OK;
+    std::vector<double> raw(B);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIP_TRY(c, hipEventCreate(&e0));
+    HIP_TRY(c, hipEventCreate(&e1));
+    for (int rep = 0; rep < 3 && rc == PG_OK; ++rep) {
+        (void)hipEventRecord(e0, c->stream);
+        rc = run_pass<T>(c, p, B, 0, nullptr, 0x7475ULL, mp, raw.data());
+        (void)hipEventRecord(e1, c->stream);
+        (void)hipEventSynchronize(e1);
+        float t = 0;
+        (void)hipEventElapsedTime(&t, e0, e1);
+        if (rep > 0 || t > 50.0f) *ms = std::min(*ms, (double)t);
+        if (rep == 0 && t > 50.0f) break;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return rc;
+}
+
+template <typename T>
+int autotune(pg_ctx* c, const pg_plan* p, int B)
+{
+    const auto key = std::make_tuple(p, B, (int)sizeof(T));
+    auto it = c->tuned.find(key);
+    if (it != c->tuned.end()) {
+        apply_settings(c, it->second);
+        return build_buckets(c);
+    }
+    const int64_t timing = c->timing;
+    const bool graphs = c->use_graphs;
+    c->timing = 0;
+    c->use_graphs = false;
+    std::vector<int64_t> best = current_settings(c);
+    double best_ms = INFINITY;
+    int rc = time_pass<T>(c, p, B, &best_ms);
+    const auto& knobs = tune_knobs();
+    for (size_t i = 0; i < knobs.size() && rc == PG_OK; ++i) {
+        for (int64_t v : knobs[i].values) {
+            if (v == best[i]) continue;
+            std::vector<int64_t> cand = best;
+            cand[i] = v;
+            apply_settings(c, cand);
+            double ms = INFINITY;
+            rc = time_pass<T>(c, p, B, &ms);
+            if (rc) break;
+            if (ms < 0.98 * best_ms) {
+                best = cand;
+                best_ms = ms;
+            }
+        }
+    }
+    c->timing = timing;
+    c->use_graphs = graphs;
+    apply_settings(c, best);
+    if (rc) return rc;
+    c->tuned[key] = best;
+    return build_buckets(c);
+}
+
 template <typename T>
 int count_impl(pg_ctx* c, const pg_plan* p, int ncol, const uint8_t* colors, uint64_t seed, uint32_t flags,
                double* raw_out)
@@ -746,6 +864,12 @@ int count_impl(pg_ctx* c, const pg_plan* p, int ncol, const uint8_t* colors, uin
     if (mp.total > c->d_arena.bytes) {
         drop_graphs(c);  // graphs hold arena addresses
         HIP_TRY(c, c->d_arena.alloc(mp.total));
+    }
+    if (c->autotune) {
+        rc = autotune<T>(c, p, tile);
+        if (rc) return rc;
+        // the chosen settings fit this tile (tuning skipped those that did not)
+        mp = plan_memory(c, p, tile, sizeof(T));
     }
     const uint8_t* d_in = nullptr;
     if (colors) {
@@ -1338,6 +1462,7 @@ extern "C" int pg_graph_upload(pg_ctx* c, int32_t n, int64_t nnz, const int32_t*
     if (na > 0) HIP_TRY(c, hipMemcpy(c->d_perm.p, perm.data(), perm.size() * 4, hipMemcpyHostToDevice));
     c->built_deg = -1;
     drop_graphs(c);
+    c->tuned.clear();
     c->generation++;
     c->weighted = false;
     c->has_graph = true;
@@ -1613,7 +1738,11 @@ extern "C" void pg_plan_destroy(pg_plan* p)
 {
     if (!p) return;
     (void)hipSetDevice(p->device);
-    if (p->ctx) drop_graphs(p->ctx, p);
+    if (p->ctx) {
+        drop_graphs(p->ctx, p);
+        for (auto it = p->ctx->tuned.begin(); it != p->ctx->tuned.end();)
+            it = std::get<0>(it->first) == p ? p->ctx->tuned.erase(it) : std::next(it);
+    }
     delete p;
 }
 

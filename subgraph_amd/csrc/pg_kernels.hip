@@ -627,15 +627,24 @@ template hipError_t launch_sub<double>(int, int, int, const SubArgs<double>&, in
 
 // ------------------------------------------------------------ utilities ---
 
+// Flat launches cover `total` work items with a grid-stride loop over at most
+// kFlatBlocks workgroups: a dispatch holds fewer than 2^32 work-items, and
+// n * B or nnz * B exceed that at the largest shapes.
+constexpr int64_t kFlatBlocks = 1 << 20;
+inline dim3 flat_grid(int64_t total) { return dim3((unsigned)std::min<int64_t>((total + 255) / 256, kFlatBlocks)); }
+#define FLAT_LOOP(i, total) \
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (total); i += (int64_t)gridDim.x * blockDim.x)
+
 __global__ void k_colors_rng(uint8_t* out, const int32_t* perm, int32_t n_act, int B, const PassParams* pp, int k)
 {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (int64_t)n_act * B) return;
-    const int64_t v = i / B;
-    const int b = (int)(i - v * B);
-    const uint64_t h = d_splitmix64(pp->seed ^ d_splitmix64((uint64_t)(pp->first + b) * 0x9E3779B97F4A7C15ull +
-                                                             (uint64_t)perm[v]));
-    out[i] = (uint8_t)(((h >> 32) * (uint64_t)k) >> 32);
+    FLAT_LOOP(i, (int64_t)n_act * B)
+    {
+        const int64_t v = i / B;
+        const int b = (int)(i - v * B);
+        const uint64_t h = d_splitmix64(pp->seed ^ d_splitmix64((uint64_t)(pp->first + b) * 0x9E3779B97F4A7C15ull +
+                                                                 (uint64_t)perm[v]));
+        out[i] = (uint8_t)(((h >> 32) * (uint64_t)k) >> 32);
+    }
 }
 
 hipError_t launch_colors_rng(uint8_t* out, const int32_t* perm, int32_t n_act, int B, const PassParams* pp, int k,
@@ -643,19 +652,19 @@ hipError_t launch_colors_rng(uint8_t* out, const int32_t* perm, int32_t n_act, i
 {
     const int64_t total = (int64_t)n_act * B;
     if (total == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_colors_rng, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, out, perm, n_act, B,
-                       pp, k);
+    hipLaunchKernelGGL(k_colors_rng, flat_grid(total), dim3(256), 0, st, out, perm, n_act, B, pp, k);
     return hipGetLastError();
 }
 
 __global__ void k_colors_gather(uint8_t* out, const uint8_t* in, const int32_t* perm, int32_t n_act, int B, int64_t n,
                                 const PassParams* pp)
 {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (int64_t)n_act * B) return;
-    const int64_t v = i / B;
-    const int b = (int)(i - v * B);
-    out[i] = in[(pp->first + b) * n + perm[v]];
+    FLAT_LOOP(i, (int64_t)n_act * B)
+    {
+        const int64_t v = i / B;
+        const int b = (int)(i - v * B);
+        out[i] = in[(pp->first + b) * n + perm[v]];
+    }
 }
 
 hipError_t launch_colors_gather(uint8_t* out, const uint8_t* in, const int32_t* perm, int32_t n_act, int B,
@@ -663,18 +672,18 @@ hipError_t launch_colors_gather(uint8_t* out, const uint8_t* in, const int32_t* 
 {
     const int64_t total = (int64_t)n_act * B;
     if (total == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_colors_gather, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, out, in, perm,
-                       n_act, B, n, pp);
+    hipLaunchKernelGGL(k_colors_gather, flat_grid(total), dim3(256), 0, st, out, in, perm, n_act, B, n, pp);
     return hipGetLastError();
 }
 
 __global__ void k_edge_colors(uint8_t* ecol, const int32_t* col, const uint8_t* colors, int64_t nnz, int B)
 {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nnz * B) return;
-    const int64_t e = i / B;
-    const int b = (int)(i - e * B);
-    ecol[i] = colors[(int64_t)col[e] * B + b];
+    FLAT_LOOP(i, nnz * B)
+    {
+        const int64_t e = i / B;
+        const int b = (int)(i - e * B);
+        ecol[i] = colors[(int64_t)col[e] * B + b];
+    }
 }
 
 hipError_t launch_edge_colors(uint8_t* ecol, const int32_t* col, const uint8_t* colors, int64_t nnz, int B,
@@ -682,8 +691,7 @@ hipError_t launch_edge_colors(uint8_t* ecol, const int32_t* col, const uint8_t* 
 {
     const int64_t total = nnz * B;
     if (total == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_edge_colors, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, ecol, col, colors,
-                       nnz, B);
+    hipLaunchKernelGGL(k_edge_colors, flat_grid(total), dim3(256), 0, st, ecol, col, colors, nnz, B);
     return hipGetLastError();
 }
 
@@ -737,14 +745,13 @@ hipError_t launch_root_reduce(const double* vals, int64_t n, int B, double* tmp,
 template <typename T>
 __global__ void k_gather_weights(T* dst, const float* src, const int32_t* ent, int64_t nnz)
 {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < nnz) dst[i] = (T)src[ent[i]];
+    FLAT_LOOP(i, nnz) dst[i] = (T)src[ent[i]];
 }
 
 hipError_t launch_gather_weights_f32(float* dst, const float* src, const int32_t* ent, int64_t nnz, hipStream_t st)
 {
     if (nnz == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_gather_weights<float>, dim3((unsigned)((nnz + 255) / 256)), dim3(256), 0, st, dst, src,
+    hipLaunchKernelGGL(k_gather_weights<float>, flat_grid(nnz), dim3(256), 0, st, dst, src,
                        ent, nnz);
     return hipGetLastError();
 }
@@ -752,7 +759,7 @@ hipError_t launch_gather_weights_f32(float* dst, const float* src, const int32_t
 hipError_t launch_gather_weights_f64(double* dst, const float* src, const int32_t* ent, int64_t nnz, hipStream_t st)
 {
     if (nnz == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_gather_weights<double>, dim3((unsigned)((nnz + 255) / 256)), dim3(256), 0, st, dst, src,
+    hipLaunchKernelGGL(k_gather_weights<double>, flat_grid(nnz), dim3(256), 0, st, dst, src,
                        ent, nnz);
     return hipGetLastError();
 }
@@ -763,12 +770,8 @@ hipError_t launch_gather_weights_f64(double* dst, const float* src, const int32_
 // form dY, the next n_act*B*Wm form G_main.  Sums run over the transposed
 // split lists in their fixed order, in FP64, then round once to T.
 template <typename T>
-__global__ void k_bwd_node(const NodeArgs<T> a)
+__device__ __forceinline__ void bwd_node_item(const NodeArgs<T>& a, int64_t i, int64_t ny)
 {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t ny = (int64_t)a.n_act * a.B * a.Wy;
-    const int64_t nm = a.Tm ? (int64_t)a.n_act * a.B * a.Wm : 0;
-    if (i >= ny + nm) return;
     const bool is_dy = i < ny;
     const int64_t j = is_dy ? i : i - ny;
     const int W = is_dy ? a.Wy : a.Wm;
@@ -807,11 +810,19 @@ __global__ void k_bwd_node(const NodeArgs<T> a)
 }
 
 template <typename T>
+__global__ void k_bwd_node(const NodeArgs<T> a)
+{
+    const int64_t ny = (int64_t)a.n_act * a.B * a.Wy;
+    const int64_t nm = a.Tm ? (int64_t)a.n_act * a.B * a.Wm : 0;
+    FLAT_LOOP(i, ny + nm) bwd_node_item(a, i, ny);
+}
+
+template <typename T>
 hipError_t launch_bwd_node(const NodeArgs<T>& a, int, hipStream_t st)
 {
     const int64_t total = (int64_t)a.n_act * a.B * (a.Wy + (a.Tm ? a.Wm : 0));
     if (total == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_bwd_node<T>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(k_bwd_node<T>, flat_grid(total), dim3(256), 0, st, a);
     return hipGetLastError();
 }
 
@@ -821,10 +832,8 @@ template hipError_t launch_bwd_node<double>(const NodeArgs<double>&, int, hipStr
 // One thread per entry e = (row v, column u), all colorings: the derivative of
 // Y[v] = sum_e w_e * T_a-term(u) with respect to w_e, contracted with dY[v].
 template <typename T>
-__global__ void k_edge_grad(const EdgeGradArgs<T> a)
+__device__ __forceinline__ void edge_grad_item(const EdgeGradArgs<T>& a, int64_t e)
 {
-    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= a.nnz) return;
     const int64_t v = a.erow[e];
     const int64_t u = a.col[e];
     double acc = 0.0;
@@ -849,10 +858,16 @@ __global__ void k_edge_grad(const EdgeGradArgs<T> a)
 }
 
 template <typename T>
+__global__ void k_edge_grad(const EdgeGradArgs<T> a)
+{
+    FLAT_LOOP(e, a.nnz) edge_grad_item(a, e);
+}
+
+template <typename T>
 hipError_t launch_edge_grad(const EdgeGradArgs<T>& a, hipStream_t st)
 {
     if (a.nnz == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_edge_grad<T>, dim3((unsigned)((a.nnz + 255) / 256)), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(k_edge_grad<T>, flat_grid(a.nnz), dim3(256), 0, st, a);
     return hipGetLastError();
 }
 
@@ -862,15 +877,14 @@ template hipError_t launch_edge_grad<double>(const EdgeGradArgs<double>&, hipStr
 template <typename TO>
 __global__ void k_scatter_grad(TO* out, const double* grad, const int32_t* ent, int64_t nnz, double factor)
 {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < nnz) out[ent[i]] = (TO)(factor * grad[i]);
+    FLAT_LOOP(i, nnz) out[ent[i]] = (TO)(factor * grad[i]);
 }
 
 hipError_t launch_scatter_grad_f32(float* out, const double* grad, const int32_t* ent, int64_t nnz, double factor,
                                    hipStream_t st)
 {
     if (nnz == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_scatter_grad<float>, dim3((unsigned)((nnz + 255) / 256)), dim3(256), 0, st, out, grad, ent,
+    hipLaunchKernelGGL(k_scatter_grad<float>, flat_grid(nnz), dim3(256), 0, st, out, grad, ent,
                        nnz, factor);
     return hipGetLastError();
 }
@@ -879,21 +893,20 @@ hipError_t launch_scatter_grad_f64(double* out, const double* grad, const int32_
                                    hipStream_t st)
 {
     if (nnz == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_scatter_grad<double>, dim3((unsigned)((nnz + 255) / 256)), dim3(256), 0, st, out, grad, ent,
+    hipLaunchKernelGGL(k_scatter_grad<double>, flat_grid(nnz), dim3(256), 0, st, out, grad, ent,
                        nnz, factor);
     return hipGetLastError();
 }
 
 __global__ void k_zero_f64(double* p, int64_t n)
 {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0.0;
+    FLAT_LOOP(i, n) p[i] = 0.0;
 }
 
 hipError_t launch_zero_f64(double* p, int64_t n, hipStream_t st)
 {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_zero_f64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n);
+    hipLaunchKernelGGL(k_zero_f64, flat_grid(n), dim3(256), 0, st, p, n);
     return hipGetLastError();
 }
 

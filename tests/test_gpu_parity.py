@@ -148,6 +148,41 @@ def test_kernel_variants_match_oracle(gpu_ctx, aux_rows, stage_rows, B):
             gpu_ctx.set_option(key, v)
 
 
+def test_autotune_keeps_results(gpu_ctx):
+    """Option "autotune": the tuned pass equals the oracle (FP64 exact, FP32
+    within the tolerance), the chosen settings are among the candidates,
+    and the second call of the same shape reuses them."""
+    rng = np.random.default_rng(77)
+    keys = ("autotune", "stage_rows", "aux_rows", "bucket_high", "bucket_mid", "hub_degree", "hub_chunk")
+    saved = [gpu_ctx.get_option(k) for k in keys]
+    s, d = sg.rmat_edges(12, 8, seed=4)
+    n = 1 << 12
+    colptr, rowidx = sg.csc_from_edges(n, s, d)
+    gpu_ctx.upload_graph(colptr, rowidx)
+    plan = gpu_ctx.plan(7, TREE7)
+    op = oracle.Plan(7, TREE7)
+    cols = rng.integers(0, 7, size=(4, n)).astype(np.uint8)
+    try:
+        gpu_ctx.set_option("autotune", 1)
+        print("[autotune] first call (tunes)", flush=True)
+        g64 = gpu_ctx.count(plan, 4, colors=cols, precision="fp64")
+        g32 = gpu_ctx.count(plan, 4, colors=cols, precision="fp32")
+        chosen = {k: gpu_ctx.get_option(k) for k in keys[1:]}
+        print(f"[autotune] chosen {chosen}", flush=True)
+        assert chosen["stage_rows"] in (0, 1) and chosen["aux_rows"] in (0, -1)
+        assert chosen["bucket_high"] in (16, 32, 64) and chosen["bucket_mid"] in (4, 8)
+        assert chosen["hub_degree"] in (128, 256, 1024) and chosen["hub_chunk"] == chosen["hub_degree"]
+        for i in range(4):
+            assert g64[i] == oracle.count_f64(op, colptr, rowidx, cols[i])
+            assert _rel(g32[i], oracle.count_f32(op, colptr, rowidx, cols[i])) <= TOL32
+        again = gpu_ctx.count(plan, 4, colors=cols, precision="fp64")
+        assert np.array_equal(again, g64)
+        assert {k: gpu_ctx.get_option(k) for k in keys[1:]} == chosen
+    finally:
+        for k, v in zip(keys, saved):
+            gpu_ctx.set_option(k, v)
+
+
 def test_fp64_equals_brute_force(gpu_ctx):
     rng = np.random.default_rng(8)
     trees = [(3, [(0, 1), (1, 2)]), (5, [(0, 1), (0, 2), (0, 3), (0, 4)]), (5, [(0, 1), (1, 2), (2, 3), (3, 4)]),
