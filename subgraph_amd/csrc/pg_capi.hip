@@ -108,6 +108,8 @@ struct DevSub {
     // row-wise index table of AUX_ROWS (SubArgs::xidx), when it fits kGidxMaxBytes
     DevBuf xidx;     // [k*k][Wa4] int16
     int Wa4 = 0;
+    // the adjoint's transposed gather (AUX_GATHER_T) through its own index table
+    DevBuf gidxT;    // [(k+1)*k][Wa4] int16
 };
 
 constexpr size_t kGidxMaxBytes = 24 * 1024;
@@ -1024,9 +1026,34 @@ int enqueue_backward(pg_ctx* c, const pg_plan* p, int B, const GradPlan& gp)
     base_args.out_mode = OUT_Y;
     for (auto it = p->subs.rbegin(); it != p->subs.rend(); ++it) {
         const DevSub& d = **it;
-        const T* Y = H;
-        int y_from_h = 1;
+        NodeArgs<T> na{};
+        na.n_act = c->n_act;
+        na.B = B;
+        na.k = k;
+        na.colors = colors;
+        na.G = d.root ? nullptr : (const T*)(base + gp.off_G[d.s]);
+        na.root_cot = (const double*)(base + gp.off_cot);
+        na.Tm = d.mm > 1 ? (const T*)(base + mp.off_tab[d.mi]) : nullptr;
+        na.Y = d.aa > 1 ? Ybuf : H;
+        na.y_from_h = d.aa > 1 ? 0 : 1;
+        na.dY = dYbuf;
+        na.Gm = d.mm > 1 ? (T*)(base + gp.off_G[d.mi]) : nullptr;
+        na.Wo = d.Wo;
+        na.Wy = d.Wy;
+        na.Wm = d.Wm;
+        na.Pa = d.Pa;
+        na.Pm = d.Pm;
+        na.by_aux = d.by_aux.as<uint32_t>();
+        na.by_main = d.by_main.as<uint32_t>();
+        na.scale = (d.scaled && fp32) ? (double)1.0e-12f : 1.0;
+        const std::string node_name = std::string("k_bwd_node<") + (fp32 ? "float>" : "double>");
         if (d.aa > 1) {
+            // dY (from G and the main row), then one gather pass that recomputes
+            // Y and adds dY . T_aux-term to every entry's gradient, then G_main
+            na.part = 1;
+            record(c, node_name, 0.0, true);
+            HIP_TRY(c, launch_bwd_node<T>(na, c->num_cus, st));
+            record(c, "", 0, false);
             SubArgs<T> a = base_args;
             a.w = w;
             a.aux = (const T*)(base + mp.off_tab[d.ai]);
@@ -1039,49 +1066,43 @@ int enqueue_backward(pg_ctx* c, const pg_plan* p, int B, const GradPlan& gp)
             a.Wy4 = d.Wy4;
             a.out = Ybuf;
             a.ldo = (int64_t)B * d.Wy;
-            int rc = launch_gathering<T>(c, AUX_GATHER, MAIN_LEAF, a, part, B * d.Wy, 0.0);
+            a.dy = dYbuf;
+            a.ldd = B * d.Wy;
+            a.grad = c->d_grad.as<double>();
+            int rc = launch_gathering<T>(c, AUX_GATHER_G, MAIN_LEAF, a, part, B * d.Wy, 0.0);
             if (rc) return rc;
-            Y = Ybuf;
-            y_from_h = 0;
+            if (na.Tm) {
+                na.part = 2;
+                record(c, node_name, 0.0, true);
+                HIP_TRY(c, launch_bwd_node<T>(na, c->num_cus, st));
+                record(c, "", 0, false);
+            }
+        } else {
+            // aux child a leaf: Y is the histogram H; the entry's derivative is
+            // dY at the neighbour's color
+            record(c, node_name, 0.0, true);
+            HIP_TRY(c, launch_bwd_node<T>(na, c->num_cus, st));
+            record(c, "", 0, false);
+            EdgeGradArgs<T> eg{};
+            eg.nnz = c->nnz;
+            eg.B = B;
+            eg.k = k;
+            eg.K = p->K;
+            eg.erow = c->d_erow.as<int32_t>();
+            eg.col = c->d_col.as<int32_t>();
+            eg.colors = colors;
+            eg.ecol = ecol;
+            eg.dY = dYbuf;
+            eg.Wy = d.Wy;
+            eg.Ta = nullptr;
+            eg.Wa = d.Wa;
+            eg.ymask = d.ymask.as<uint16_t>();
+            eg.rank_tab = p->rank_tab.as<int16_t>();
+            eg.grad = c->d_grad.as<double>();
+            record(c, std::string("k_edge_grad<") + (fp32 ? "float>" : "double>"), 0.0, true);
+            HIP_TRY(c, launch_edge_grad<T>(eg, st));
+            record(c, "", 0, false);
         }
-        NodeArgs<T> na{};
-        na.n_act = c->n_act;
-        na.B = B;
-        na.k = k;
-        na.colors = colors;
-        na.G = d.root ? nullptr : (const T*)(base + gp.off_G[d.s]);
-        na.root_cot = (const double*)(base + gp.off_cot);
-        na.Tm = d.mm > 1 ? (const T*)(base + mp.off_tab[d.mi]) : nullptr;
-        na.Y = Y;
-        na.y_from_h = y_from_h;
-        na.dY = dYbuf;
-        na.Gm = d.mm > 1 ? (T*)(base + gp.off_G[d.mi]) : nullptr;
-        na.Wo = d.Wo;
-        na.Wy = d.Wy;
-        na.Wm = d.Wm;
-        na.Pa = d.Pa;
-        na.Pm = d.Pm;
-        na.by_aux = d.by_aux.as<uint32_t>();
-        na.by_main = d.by_main.as<uint32_t>();
-        na.scale = (d.scaled && fp32) ? (double)1.0e-12f : 1.0;
-        HIP_TRY(c, launch_bwd_node<T>(na, c->num_cus, st));
-        EdgeGradArgs<T> eg{};
-        eg.nnz = c->nnz;
-        eg.B = B;
-        eg.k = k;
-        eg.K = p->K;
-        eg.erow = c->d_erow.as<int32_t>();
-        eg.col = c->d_col.as<int32_t>();
-        eg.colors = colors;
-        eg.ecol = ecol;
-        eg.dY = dYbuf;
-        eg.Wy = d.Wy;
-        eg.Ta = d.aa > 1 ? (const T*)(base + mp.off_tab[d.ai]) : nullptr;
-        eg.Wa = d.Wa;
-        eg.ymask = d.ymask.as<uint16_t>();
-        eg.rank_tab = p->rank_tab.as<int16_t>();
-        eg.grad = c->d_grad.as<double>();
-        HIP_TRY(c, launch_edge_grad<T>(eg, st));
         if (d.aa > 1) {
             SubArgs<T> a = base_args;
             a.w = wT;
@@ -1091,6 +1112,8 @@ int enqueue_backward(pg_ctx* c, const pg_plan* p, int B, const GradPlan& gp)
             a.Wy = d.Wa;  // values per coloring: the aux child's row
             a.Wo = d.Wa;
             a.ymask = d.amask.as<uint16_t>();
+            a.gidx = d.gidxT.as<int16_t>();
+            a.Wy4 = d.Wa4;
             a.out = (T*)(base + gp.off_G[d.ai]);
             a.ldo = (int64_t)B * d.Wa;
             int rc = launch_gathering<T>(c, AUX_GATHER_T, MAIN_LEAF, a, part, B * d.Wa, 0.0);
@@ -1171,14 +1194,21 @@ int count_grad_impl(pg_ctx* c, const pg_plan* p, int ncol, const uint8_t* colors
         std::vector<double> cot(B, 1.0);
         if (cotangent) std::copy(cotangent + first, cotangent + first + B, cot.begin());
         HIP_TRY(c, hipMemcpyAsync(base + g.off_cot, cot.data(), sizeof(double) * B, hipMemcpyHostToDevice, c->stream));
+        std::vector<LaunchRec> recs;  // filled when option "timing" is on
+        c->cur_recs = &recs;
         rc = enqueue_pass<T>(c, p, B, d_in, g.fwd);
-        if (rc) break;
-        rc = enqueue_backward<T>(c, p, B, g);
-        if (rc) break;
+        if (rc == PG_OK) rc = enqueue_backward<T>(c, p, B, g);
+        c->cur_recs = nullptr;
+        if (rc) {
+            destroy_recs(recs);
+            break;
+        }
         raw.assign(B, 0.0);
         HIP_TRY(c, hipMemcpyAsync(raw.data(), base + g.fwd.off_rawdev, sizeof(double) * B, hipMemcpyDeviceToHost,
                                   c->stream));
         HIP_TRY(c, hipStreamSynchronize(c->stream));
+        accumulate_timing(c, recs);
+        destroy_recs(recs);
         for (int b = 0; b < B; ++b) raw_out[first + b] = finish_raw(p, sizeof(T) == 4, flags, raw[b]);
     }
     if (rc == PG_OK && grad) {
@@ -1237,6 +1267,26 @@ std::vector<int16_t> gather_index_table(int k, const std::vector<uint16_t>& ymas
                 g[((size_t)cv * k + cu) * Wy4 + i] = rank[squeeze(A & ~(1u << cu), (uint32_t)cu)];
             }
         }
+    return g;
+}
+
+// SubArgs::gidx of the adjoint's transposed gather: the vertex's row value i
+// (the set amask[i] relative to cv, plus cv) takes from a neighbour of color
+// cu its dY value at the rank of that set relative to cu, when cu is outside it
+std::vector<int16_t> gather_index_table_t(int k, const std::vector<uint16_t>& amask, const std::vector<int16_t>& rank,
+                                          int Wa4)
+{
+    const int Wa = (int)amask.size();
+    std::vector<int16_t> g((size_t)(k + 1) * k * Wa4, (int16_t)-1);
+    auto expand = [](uint32_t m, uint32_t c) { return (m & ((1u << c) - 1u)) | ((m >> c) << (c + 1u)); };
+    auto squeeze = [](uint32_t m, uint32_t c) { return (m & ((1u << c) - 1u)) | ((m >> (c + 1u)) << c); };
+    for (int cv = 0; cv < k; ++cv)
+        for (int cu = 0; cu < k; ++cu)
+            for (int i = 0; i < Wa; ++i) {
+                const uint32_t A = expand(amask[i], (uint32_t)cv) | (1u << cv);
+                if ((A >> cu) & 1u) continue;  // also cu == cv
+                g[((size_t)cv * k + cu) * Wa4 + i] = rank[squeeze(A, (uint32_t)cu)];
+            }
     return g;
 }
 
@@ -1707,6 +1757,12 @@ extern "C" int pg_plan_create(pg_ctx* c, int k, int nsub, const int32_t* sub_siz
                 const std::vector<int16_t> x = rows_index_table(k, am, rank, d->Wa4);
                 HIP_TRY(c, d->xidx.alloc(xbytes));
                 HIP_TRY(c, hipMemcpy(d->xidx.p, x.data(), xbytes, hipMemcpyHostToDevice));
+            }
+            const size_t tbytes = (size_t)2 * (k + 1) * k * d->Wa4;
+            if (tbytes <= kGidxMaxBytes) {
+                const std::vector<int16_t> g = gather_index_table_t(k, am, rank, d->Wa4);
+                HIP_TRY(c, d->gidxT.alloc(tbytes));
+                HIP_TRY(c, hipMemcpy(d->gidxT.p, g.data(), tbytes, hipMemcpyHostToDevice));
             }
         }
         plan->subs.push_back(std::move(d));

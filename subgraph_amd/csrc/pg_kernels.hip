@@ -58,7 +58,7 @@ template <typename T, int AUX>
 __device__ __forceinline__ T aux_term(const SubArgs<T>& a, const int16_t* s_rank, uint32_t amask, int aoff, int u,
                                       uint32_t cu)
 {
-    if (AUX == AUX_GATHER) {
+    if (AUX == AUX_GATHER || AUX == AUX_GATHER_G) {
         if ((amask >> cu) & 1u) {
             const int idx = s_rank[squeeze_bit(amask & ~(1u << cu), cu)];
             return a.aux[(int64_t)u * a.lda + aoff + idx];
@@ -80,7 +80,7 @@ __host__ __device__ constexpr size_t rank_lds_bytes(int K) { return (((size_t)2 
 // per-lane gathers of neighbour rows (through the rank table or the gather index table)
 __host__ __device__ constexpr bool gathers_rows(int aux)
 {
-    return aux == AUX_GATHER || aux == AUX_GATHER_T || aux == AUX_STAGED;
+    return aux == AUX_GATHER || aux == AUX_GATHER_T || aux == AUX_STAGED || aux == AUX_GATHER_G;
 }
 // launches that stage the rank table (and, row-wise, the aux row masks) in LDS
 __host__ __device__ constexpr bool uses_rank(int aux) { return gathers_rows(aux) || aux == AUX_ROWS; }
@@ -92,7 +92,7 @@ __host__ __device__ constexpr bool walks_entries(int aux) { return uses_rank(aux
 template <typename T>
 __host__ __device__ size_t head_lds_bytes(int aux, const SubArgs<T>& a)
 {
-    if ((aux == AUX_GATHER || aux == AUX_STAGED) && a.gidx)
+    if ((aux == AUX_GATHER || aux == AUX_STAGED || aux == AUX_GATHER_T || aux == AUX_GATHER_G) && a.gidx)
         return ((size_t)2 * (a.k + 1) * a.k * a.Wy4 + 15) & ~(size_t)15;
     if (aux == AUX_ROWS && a.xidx) return ((size_t)2 * a.k * a.k * a.Wa4 + 15) & ~(size_t)15;
     return !uses_rank(aux) ? 0
@@ -134,7 +134,8 @@ __global__ __launch_bounds__(kBlock) void k_sub(const SubArgs<T> a)
     const int cln = lane - grp * cl;
     const int ycopies = AUX == AUX_ROWS ? ng : 1;
     const size_t head = head_lds_bytes(AUX, a);
-    const bool use_gidx = (AUX == AUX_GATHER || AUX == AUX_STAGED) && a.gidx != nullptr;
+    const bool use_gidx =
+        (AUX == AUX_GATHER || AUX == AUX_STAGED || AUX == AUX_GATHER_T || AUX == AUX_GATHER_G) && a.gidx != nullptr;
     constexpr bool staged = AUX == AUX_STAGED;
     const bool y_in_lds = !(mode == OUT_Y || mode == OUT_PARTIAL) || AUX == AUX_ROWS;
     const int gstride = staged ? stage_group_stride(a.segpad, TEAM / ng) : 0;
@@ -293,12 +294,19 @@ __global__ __launch_bounds__(kBlock) void k_sub(const SubArgs<T> a)
                     acc[r] = T(0);
                     const int i = i0 + r;
                     const bool ok = sv && i < W;
-                    if (AUX == AUX_GATHER) amask[r] = ok ? expand_bit(a.ymask[i], cv) : 0u;
+                    if (AUX == AUX_GATHER || AUX == AUX_GATHER_G) amask[r] = ok ? expand_bit(a.ymask[i], cv) : 0u;
                     else if (AUX == AUX_STAGED) amask[r] = 0u;  // the gather index table replaces it
                     else if (AUX == AUX_GATHER_T) amask[r] = ok ? expand_bit(a.ymask[i], cv) | (1u << cv) : ~0u;
                     else amask[r] = ok ? (uint32_t)i : 0xffu;
                 }
                 const int aoff = b * a.Wa;
+                // adjoint: this slot's dY values (zero past the row's end)
+                T dyv[kRegCols];
+                constexpr bool edge_grad = AUX == AUX_GATHER_G;
+                if (edge_grad)
+#pragma unroll
+                    for (int r = 0; r < kRegCols; ++r)
+                        dyv[r] = sv && i0 + r < W ? a.dy[(int64_t)v * a.ldd + b * W + i0 + r] : T(0);
                 // a neighbour past the end reads as the vertex itself with a
                 // color that adds nothing (col(v) is never in a GATHER set and
                 // always in a GATHER_T set)
@@ -447,6 +455,37 @@ __global__ __launch_bounds__(kBlock) void k_sub(const SubArgs<T> a)
 #pragma unroll
                             for (int r = 0; r < kRegCols; ++r)
                                 acc[r] = a.w ? acc[r] + wv[q] * x[q][r] : acc[r] + x[q][r];
+                        if (edge_grad) {
+                            // d Y[v] / d w_e = x_e: contract with dY over the group's
+                            // values in the table precision, one FP64 add per entry.
+                            // The 4 neighbours' dots reduce together (fixed order):
+                            // xor 1 and xor 2 halve the values a lane carries, so lane
+                            // bits (0,1) end up naming the neighbour 2*b0 + b1, then
+                            // the higher xor steps sum across the group.
+                            T dot[4];
+#pragma unroll
+                            for (int q = 0; q < 4; ++q) {
+                                dot[q] = T(0);
+#pragma unroll
+                                for (int r = 0; r < kRegCols; ++r) dot[q] = dot[q] + dyv[r] * x[q][r];
+                            }
+                            if (cl >= 4) {
+                                const bool h1 = cln & 1, h2 = (cln >> 1) & 1;
+                                const T pa = (h1 ? dot[2] : dot[0]) + __shfl_xor(h1 ? dot[0] : dot[2], 1, TEAM);
+                                const T pb = (h1 ? dot[3] : dot[1]) + __shfl_xor(h1 ? dot[1] : dot[3], 1, TEAM);
+                                T sum = (h2 ? pb : pa) + __shfl_xor(h2 ? pa : pb, 2, TEAM);
+                                for (int sh = 4; sh < cl; sh <<= 1) sum = sum + __shfl_xor(sum, sh, TEAM);
+                                const int q = 2 * (int)h1 + (int)h2;
+                                if (cln < 4 && ok[q]) a.grad[e + q * step] += (double)sum;
+                            } else {
+#pragma unroll
+                                for (int q = 0; q < 4; ++q) {
+                                    T sum = dot[q];
+                                    for (int sh = 1; sh < cl; sh <<= 1) sum = sum + __shfl_xor(sum, sh, TEAM);
+                                    if (cln == 0 && ok[q]) a.grad[e + q * step] += (double)sum;
+                                }
+                            }
+                        }
                     }
                 }
                 // fold the neighbour groups: fixed xor tree, identical in every lane
@@ -589,6 +628,7 @@ static hipError_t launch_team(int aux, int main_mode, const SubArgs<T>& a, int n
         case AUX_GATHER: return launch_one<T, TEAM, AUX_GATHER, MAIN_LEAF>(a, num_cus, per_cu, st);
         case AUX_ROWS: return launch_one<T, TEAM, AUX_ROWS, MAIN_LEAF>(a, num_cus, per_cu, st);
         case AUX_STAGED: return launch_one<T, TEAM, AUX_STAGED, MAIN_LEAF>(a, num_cus, per_cu, st);
+        case AUX_GATHER_G: return launch_one<T, TEAM, AUX_GATHER_G, MAIN_LEAF>(a, num_cus, per_cu, st);
         case AUX_GATHER_T: return launch_one<T, TEAM, AUX_GATHER_T, MAIN_LEAF>(a, num_cus, per_cu, st);
         case AUX_COLORS: return launch_one<T, TEAM, AUX_COLORS, MAIN_LEAF>(a, num_cus, per_cu, st);
         case AUX_READ: return launch_one<T, TEAM, AUX_READ, MAIN_LEAF>(a, num_cus, per_cu, st);
@@ -814,13 +854,17 @@ __global__ void k_bwd_node(const NodeArgs<T> a)
 {
     const int64_t ny = (int64_t)a.n_act * a.B * a.Wy;
     const int64_t nm = a.Tm ? (int64_t)a.n_act * a.B * a.Wm : 0;
-    FLAT_LOOP(i, ny + nm) bwd_node_item(a, i, ny);
+    const int64_t lo = a.part == 2 ? ny : 0;        // part 2: the G_main items only
+    const int64_t hi = a.part == 1 ? ny : ny + nm;  // part 1: the dY items only
+    FLAT_LOOP(i, hi - lo) bwd_node_item(a, lo + i, ny);
 }
 
 template <typename T>
 hipError_t launch_bwd_node(const NodeArgs<T>& a, int, hipStream_t st)
 {
-    const int64_t total = (int64_t)a.n_act * a.B * (a.Wy + (a.Tm ? a.Wm : 0));
+    const int64_t ny = (int64_t)a.n_act * a.B * a.Wy;
+    const int64_t nm = a.Tm ? (int64_t)a.n_act * a.B * a.Wm : 0;
+    const int64_t total = a.part == 1 ? ny : a.part == 2 ? nm : ny + nm;
     if (total == 0) return hipSuccess;
     hipLaunchKernelGGL(k_bwd_node<T>, flat_grid(total), dim3(256), 0, st, a);
     return hipGetLastError();

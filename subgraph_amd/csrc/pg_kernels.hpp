@@ -29,6 +29,7 @@ enum AuxMode : int {
                        // and every value is added to the one Y set it belongs to
     AUX_STAGED = 6,    // AUX_GATHER through the gather index table with neighbour rows staged in
                        // LDS (needs SubArgs::gidx and segpad)
+    AUX_GATHER_G = 7,  // AUX_GATHER of the adjoint: also grad[e] += dY . x_e per entry (needs dy, grad)
 };
 // Where the main-child values come from.
 enum MainMode : int {
@@ -75,9 +76,10 @@ struct SubArgs {
     const uint16_t* ymask;   // [Wy] K-bit mask of each Y set (AUX_GATHER)
     const uint16_t* xmask;   // [Wa] K-bit mask of each aux-child row set (AUX_ROWS)
     const int16_t* rank_tab; // [2^K] colex rank of a K-bit mask among masks of its popcount
-    // AUX_GATHER, optional: gidx[(cv*k + cu)*Wy4 + i] = index in a neighbour's
-    // row (color cu) of the value Y value i (vertex color cv) takes from it, or
-    // -1; rows k*k .. k*k+k-1 are all -1 (a slot past the end reads there)
+    // AUX_GATHER (and AUX_GATHER_T), optional: gidx[(cv*k + cu)*Wy4 + i] =
+    // index in a neighbour's row (color cu) of the value Y value i (vertex
+    // color cv) takes from it, or -1; rows k*k .. k*k+k-1 are all -1 (a slot
+    // past the end reads there)
     const int16_t* gidx;
     int32_t Wy4;             // Wy rounded up to kRegCols
     // AUX_GATHER with gidx, optional (0 = off): neighbour rows are staged in
@@ -89,6 +91,12 @@ struct SubArgs {
     const int16_t* xidx;
     int32_t Wa4;             // Wa rounded up to 4
     const int32_t* slot_begin;  // hub h's partial slots [slot_begin[h], slot_begin[h+1])
+    // AUX_GATHER_G: each neighbour's contribution x to the vertex's Y row is
+    // also contracted with dY there, and grad[e] += sum over the row of dY * x
+    // (in FP64) for the entry e
+    const T* dy;             // [n_act][ldd]
+    int32_t ldd;             // dy row stride (B * Wy)
+    double* grad;            // [nnz] internal entry order
     // main side
     const T* main;
     int32_t ldm;
@@ -167,6 +175,7 @@ struct NodeArgs {
     const uint32_t* by_aux;  // [Wy][Pa] (o << 16 | main rank)
     const uint32_t* by_main; // [Wm][Pm] (o << 16 | y rank)
     double scale;            // 1, or float(1e-12) for the scaled subtemplate
+    int32_t part;            // 0: dY and G_main, 1: dY only, 2: G_main only
 };
 template <typename T>
 hipError_t launch_bwd_node(const NodeArgs<T>& a, int num_cus, hipStream_t st);

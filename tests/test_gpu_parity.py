@@ -166,18 +166,20 @@ def test_autotune_keeps_results(gpu_ctx):
         gpu_ctx.set_option("autotune", 1)
         print("[autotune] first call (tunes)", flush=True)
         g64 = gpu_ctx.count(plan, 4, colors=cols, precision="fp64")
-        g32 = gpu_ctx.count(plan, 4, colors=cols, precision="fp32")
-        chosen = {k: gpu_ctx.get_option(k) for k in keys[1:]}
-        print(f"[autotune] chosen {chosen}", flush=True)
-        assert chosen["stage_rows"] in (0, 1) and chosen["aux_rows"] in (0, -1)
-        assert chosen["bucket_high"] in (16, 32, 64) and chosen["bucket_mid"] in (4, 8)
-        assert chosen["hub_degree"] in (128, 256, 1024) and chosen["hub_chunk"] == chosen["hub_degree"]
+        chosen64 = {k: gpu_ctx.get_option(k) for k in keys[1:]}
+        g32 = gpu_ctx.count(plan, 4, colors=cols, precision="fp32")  # its own key: tunes again
+        chosen32 = {k: gpu_ctx.get_option(k) for k in keys[1:]}
+        print(f"[autotune] chosen fp64 {chosen64} fp32 {chosen32}", flush=True)
+        for chosen in (chosen64, chosen32):
+            assert chosen["stage_rows"] in (0, 1) and chosen["aux_rows"] in (0, -1)
+            assert chosen["bucket_high"] in (16, 32, 64) and chosen["bucket_mid"] in (4, 8)
+            assert chosen["hub_degree"] in (128, 256, 1024) and chosen["hub_chunk"] == chosen["hub_degree"]
         for i in range(4):
             assert g64[i] == oracle.count_f64(op, colptr, rowidx, cols[i])
             assert _rel(g32[i], oracle.count_f32(op, colptr, rowidx, cols[i])) <= TOL32
-        again = gpu_ctx.count(plan, 4, colors=cols, precision="fp64")
+        again = gpu_ctx.count(plan, 4, colors=cols, precision="fp64")  # reuses the fp64 choice
         assert np.array_equal(again, g64)
-        assert {k: gpu_ctx.get_option(k) for k in keys[1:]} == chosen
+        assert {k: gpu_ctx.get_option(k) for k in keys[1:]} == chosen64
     finally:
         for k, v in zip(keys, saved):
             gpu_ctx.set_option(k, v)
