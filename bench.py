@@ -192,6 +192,7 @@ def s22_variant(sync, with_grad=True):
         sync()
         tune_s = time.perf_counter() - t1
         tuned = {k: ctx.get_option(k) for k in TUNED_KEYS}
+        tuned["colorings_per_pass"] = ctx.get_option("last_tile")
         t_b = _timed(lambda r=0: ctx.count(plan, B, seed=200 + r), reps, sync)
         ctx.set_option("autotune", 0)
         t_p = _timed(lambda r=0: ctx.count(plan, 1, seed=300 + r), 3, sync)
@@ -220,10 +221,15 @@ def s22_variant(sync, with_grad=True):
             else:
                 ctx.set_weights(hashed_weights(n, colptr, rowidx))
             t_f = _timed(lambda r=0: ctx.count(plan, 16, seed=400 + r), 1, sync)
+            fwd_tile = ctx.get_option("last_tile")
             t_fb = _timed(lambda r=0: ctx.count_grad(plan, 16, seed=500 + r, precision="fp32"), 1, sync)
             rec[wname] = {"fwd_colorings_per_s": 16 / t_f, "fwd_bwd_colorings_per_s": 16 / t_fb,
-                          "fwd_bwd_vs_fwd": t_f / t_fb}
-            log(f"s22 tree12 {wname}: fwd {16 / t_f:.2f}/s, fwd+bwd {16 / t_fb:.2f}/s")
+                          "fwd_bwd_vs_fwd": t_f / t_fb, "fwd_colorings_per_pass": fwd_tile,
+                          "fwd_bwd_colorings_per_pass": ctx.get_option("last_tile"),
+                          "adjoint_tables_recomputed": bool(ctx.get_option("last_recompute"))}
+            log(f"s22 tree12 {wname}: fwd {16 / t_f:.2f}/s ({fwd_tile}/pass), fwd+bwd {16 / t_fb:.2f}/s "
+                f"({rec[wname]['fwd_bwd_colorings_per_pass']}/pass, recompute "
+                f"{rec[wname]['adjoint_tables_recomputed']})")
         out["tree12_fwd_bwd"] = {"colorings_per_call": 16, **rec}
     return out
 

@@ -66,7 +66,12 @@ const char* pg_last_error(const pg_ctx* ctx);
  *   "blocks_per_cu" workgroups per CU of persistent launches, 0 = occupancy query
  *   "stage_rows"   1 = gathers stage neighbour rows in LDS, 0 = direct loads (default 0)
  *   "aux_rows"     neighbour sums row-wise with LDS atomics: 1 always, 0 never (default),
- *                  -1 when most of a neighbour row is used */
+ *                  -1 when most of a neighbour row is used
+ *   "adjoint_recompute" pg_count_grad: 0 keeps every forward table for the reverse
+ *                  sweep, 1 rebuilds each table from its subtree when the sweep needs it,
+ *                  -1 rebuilds only when that fits more colorings per pass (default)
+ * Read-only: "last_tile" (colorings per pass of the last count / gradient call) and
+ * "last_recompute" (1 if that gradient call rebuilt tables). */
 int pg_set_option(pg_ctx* ctx, const char* key, int64_t value);
 int pg_get_option(const pg_ctx* ctx, const char* key, int64_t* value);
 

@@ -135,6 +135,8 @@ struct pg_ctx {
     int64_t blocks_per_cu = 0;  // 0: resident workgroups from the occupancy query
     int64_t aux_rows = 0;       // neighbour sums row-wise: 1 always, 0 never, -1 per subtemplate
     int64_t stage_rows = 0;     // gathers read neighbour rows staged in LDS (1) or directly (0)
+    int64_t adjoint_recompute = -1;  // adjoint tables: 0 kept, 1 recomputed, -1 recomputed when that fits more
+    int64_t last_tile = 0, last_recompute = 0;  // of the last count / gradient call (read-only options)
     // graph
     bool has_graph = false, weighted = false;
     int32_t n = 0, n_act = 0, max_deg = 0;
@@ -505,6 +507,110 @@ int launch_gathering(pg_ctx* c, int aux, int main_mode, SubArgs<T> a, T* part, i
     return PG_OK;
 }
 
+// Device pointers of one pass's buffers (colors, neighbour colors, the
+// histogram H, hub partial rows, root values, results, parameters).
+template <typename T>
+struct PassBufs {
+    const PassParams* pp;
+    uint8_t* colors;
+    uint8_t* ecol;
+    T* H;
+    T* part;
+    double* rootv;
+    double* rawd;
+    double* tmp;
+};
+
+template <typename T>
+PassBufs<T> pass_bufs(pg_ctx* c, const MemPlan& mp)
+{
+    char* base = c->d_arena.as<char>();
+    return {(const PassParams*)(base + mp.off_params), (uint8_t*)(base + mp.off_colors),
+            (uint8_t*)(base + mp.off_ecol),          (T*)(base + mp.off_H),
+            (T*)(base + mp.off_part),                (double*)(base + mp.off_root),
+            (double*)(base + mp.off_rawdev),         (double*)(base + mp.off_tmp)};
+}
+
+// The arguments every launch of a pass shares.
+template <typename T>
+SubArgs<T> pass_args(const pg_ctx* c, const pg_plan* p, int B, const PassBufs<T>& pb)
+{
+    SubArgs<T> a{};
+    a.rowptr = c->d_rowptr.as<int32_t>();
+    a.col = c->d_col.as<int32_t>();
+    if (c->weighted) a.w = sizeof(T) == 4 ? (const T*)c->d_w32.p : (const T*)c->d_w64.p;
+    a.colors = pb.colors;
+    a.ecol = pb.ecol;
+    a.B = B;
+    a.k = p->k;
+    a.K = p->K;
+    a.ng = 1;
+    a.rank_tab = p->rank_tab.as<int16_t>();
+    a.slot_begin = c->d_slot_begin.as<int32_t>();
+    a.scale = 1.0e-12f;
+    return a;
+}
+
+// survey byte model: CSC streamed once per gathering launch, full-width tables
+double csc_bytes(const pg_ctx* c) { return 4.0 * (c->n_act + 1) + 4.0 * c->nnz * (c->weighted ? 2 : 1); }
+
+// One forward step: subtemplate d of B colorings from its children's tables
+// (nullptr for a leaf child) into out_tab or, at the root, into the
+// per-vertex root values and the pass results.
+template <typename T>
+int enqueue_step(pg_ctx* c, const pg_plan* p, const DevSub& d, int B, const PassBufs<T>& pb, const T* main_tab,
+                 const T* aux_tab, T* out_tab)
+{
+    const hipStream_t st = c->stream;
+    const double na = (double)c->n_act;
+    SubArgs<T> a = pass_args<T>(c, p, B, pb);
+    a.Wy = d.Wy;
+    a.Wm = d.Wm;
+    a.Wo = d.Wo;
+    a.P = d.P;
+    a.split = d.split.as<uint32_t>();
+    a.ymask = d.ymask.as<uint16_t>();
+    const int main_mode = d.mm == 1 ? MAIN_LEAF : MAIN_TABLE;
+    if (d.mm > 1) {
+        a.main = main_tab;
+        a.ldm = B * d.Wm;
+    }
+    if (d.root) {
+        a.out_mode = OUT_ROOT;
+        a.root_vals = pb.rootv;
+        a.n_root = c->n_act;
+    } else {
+        a.out = out_tab;
+        a.ldo = B * d.Wo;
+        a.out_mode = (d.scaled && sizeof(T) == 4) ? OUT_EMA_SCALED : OUT_EMA;
+    }
+    const double tab_bytes = 4.0 * na * B * ((d.aa > 1 ? d.Ca : 0) + (d.mm > 1 ? d.Cm : 0) + (d.root ? 0 : d.Cs));
+    if (d.aa == 1) {
+        const int team = team_for(std::max(B * d.Wy, B * d.Wo));
+        a.aux = pb.H;
+        a.lda = B * p->k;
+        a.Wa = p->k;
+        a.n_items = c->n_act;
+        a.item_base = 0;
+        record(c, kname<T>(team, AUX_READ, main_mode), tab_bytes, true);
+        HIP_TRY(c, launch_sub<T>(team, AUX_READ, main_mode, a, c->num_cus, c->blocks_per_cu, st));
+        record(c, "", 0, false);
+    } else {
+        a.aux = aux_tab;
+        a.Wa = d.Wa;
+        a.lda = B * d.Wa;
+        a.xmask = d.amask.as<uint16_t>();
+        a.gidx = d.gidx.as<int16_t>();
+        a.Wy4 = d.Wy4;
+        a.xidx = d.xidx.as<int16_t>();
+        a.Wa4 = d.Wa4;
+        int rc = launch_gathering<T>(c, aux_mode_for(c, p, d), main_mode, a, pb.part, B * d.Wy, csc_bytes(c) + tab_bytes);
+        if (rc) return rc;
+    }
+    if (d.root) HIP_TRY(c, launch_root_reduce(pb.rootv, c->n_act, B, pb.tmp, pb.rawd, st));
+    return PG_OK;
+}
+
 // Enqueues one pass of B colorings through the whole DP on c->stream.  The
 // pass's seed and first coloring are read on the device from mp.off_params,
 // so the same launches (captured once as a graph) serve every pass of this
@@ -513,105 +619,36 @@ template <typename T>
 int enqueue_pass(pg_ctx* c, const pg_plan* p, int B, const uint8_t* d_colors_in, const MemPlan& mp)
 {
     char* base = c->d_arena.as<char>();
-    const PassParams* pp = (const PassParams*)(base + mp.off_params);
-    uint8_t* colors = (uint8_t*)(base + mp.off_colors);
-    T* H = (T*)(base + mp.off_H);
-    T* part = (T*)(base + mp.off_part);
-    double* rootv = (double*)(base + mp.off_root);
-    double* rawd = (double*)(base + mp.off_rawdev);
-    double* tmp = (double*)(base + mp.off_tmp);
-    const T* w = nullptr;
-    if (c->weighted) w = sizeof(T) == 4 ? (const T*)c->d_w32.p : (const T*)c->d_w64.p;
+    const PassBufs<T> pb = pass_bufs<T>(c, mp);
     const hipStream_t st = c->stream;
-    const int k = p->k;
-    // survey byte model: CSC streamed once per gathering launch, full-width tables
-    const double nb_csc = 4.0 * (c->n_act + 1) + 4.0 * c->nnz * (c->weighted ? 2 : 1);
-    const double na = (double)c->n_act;
-
     if (d_colors_in)
-        HIP_TRY(c, launch_colors_gather(colors, d_colors_in, c->d_perm.as<int32_t>(), c->n_act, B, c->n, pp, st));
+        HIP_TRY(c, launch_colors_gather(pb.colors, d_colors_in, c->d_perm.as<int32_t>(), c->n_act, B, c->n, pb.pp, st));
     else
-        HIP_TRY(c, launch_colors_rng(colors, c->d_perm.as<int32_t>(), c->n_act, B, pp, k, st));
-    uint8_t* ecol = (uint8_t*)(base + mp.off_ecol);
-    HIP_TRY(c, launch_edge_colors(ecol, c->d_col.as<int32_t>(), colors, c->nnz, B, st));
-
-    SubArgs<T> base_args{};
-    base_args.rowptr = c->d_rowptr.as<int32_t>();
-    base_args.col = c->d_col.as<int32_t>();
-    base_args.w = w;
-    base_args.colors = colors;
-    base_args.ecol = ecol;
-    base_args.B = B;
-    base_args.k = k;
-    base_args.K = p->K;
-    base_args.ng = 1;
-    base_args.rank_tab = p->rank_tab.as<int16_t>();
-    base_args.slot_begin = c->d_slot_begin.as<int32_t>();
-    base_args.scale = 1.0e-12f;
+        HIP_TRY(c, launch_colors_rng(pb.colors, c->d_perm.as<int32_t>(), c->n_act, B, pb.pp, p->k, st));
+    HIP_TRY(c, launch_edge_colors(pb.ecol, c->d_col.as<int32_t>(), pb.colors, c->nnz, B, st));
 
     // ---- neighbour-color histogram H = A * leaf (shared by every leaf aux)
     bool need_H = false;
     for (auto& d : p->subs) need_H |= d.get()->aa == 1;
     if (need_H) {
-        SubArgs<T> a = base_args;
-        a.Wy = k;
-        a.Wo = k;
-        a.out = H;
-        a.ldo = (int64_t)B * k;
+        SubArgs<T> a = pass_args<T>(c, p, B, pb);
+        a.Wy = p->k;
+        a.Wo = p->k;
+        a.out = pb.H;
+        a.ldo = B * p->k;
         a.out_mode = OUT_Y;
-        int rc = launch_gathering<T>(c, AUX_COLORS, MAIN_LEAF, a, part, B * k, nb_csc + na * B);
+        int rc = launch_gathering<T>(c, AUX_COLORS, MAIN_LEAF, a, pb.part, B * p->k,
+                                     csc_bytes(c) + (double)c->n_act * B);
         if (rc) return rc;
     }
 
     // ---- subtemplates, children before parents
+    auto tab = [&](int s) { return (T*)(base + mp.off_tab[s]); };
     for (auto& dp : p->subs) {
         const DevSub& d = *dp;
-        SubArgs<T> a = base_args;
-        a.Wy = d.Wy;
-        a.Wm = d.Wm;
-        a.Wo = d.Wo;
-        a.P = d.P;
-        a.split = d.split.as<uint32_t>();
-        a.ymask = d.ymask.as<uint16_t>();
-        const int main_mode = d.mm == 1 ? MAIN_LEAF : MAIN_TABLE;
-        if (d.mm > 1) {
-            a.main = (const T*)(base + mp.off_tab[d.mi]);
-            a.ldm = (int64_t)B * d.Wm;
-        }
-        if (d.root) {
-            a.out_mode = OUT_ROOT;
-            a.root_vals = rootv;
-            a.n_root = c->n_act;
-        } else {
-            a.out = (T*)(base + mp.off_tab[d.s]);
-            a.ldo = (int64_t)B * d.Wo;
-            a.out_mode = (d.scaled && sizeof(T) == 4) ? OUT_EMA_SCALED : OUT_EMA;
-        }
-        const double tab_bytes =
-            4.0 * na * B * ((d.aa > 1 ? d.Ca : 0) + (d.mm > 1 ? d.Cm : 0) + (d.root ? 0 : d.Cs));
-        if (d.aa == 1) {
-            const int team = team_for(std::max(B * d.Wy, B * d.Wo));
-            a.aux = H;
-            a.lda = (int64_t)B * k;
-            a.Wa = k;
-            a.n_items = c->n_act;
-            a.item_base = 0;
-            record(c, kname<T>(team, AUX_READ, main_mode), tab_bytes, true);
-            HIP_TRY(c, launch_sub<T>(team, AUX_READ, main_mode, a, c->num_cus, c->blocks_per_cu, st));
-            record(c, "", 0, false);
-        } else {
-            a.aux = (const T*)(base + mp.off_tab[d.ai]);
-            a.Wa = d.Wa;
-            a.lda = (int64_t)B * d.Wa;
-            a.xmask = d.amask.as<uint16_t>();
-            a.gidx = d.gidx.as<int16_t>();
-            a.Wy4 = d.Wy4;
-            a.xidx = d.xidx.as<int16_t>();
-            a.Wa4 = d.Wa4;
-            int rc = launch_gathering<T>(c, aux_mode_for(c, p, d), main_mode, a, part, B * d.Wy, nb_csc + tab_bytes);
-            if (rc) return rc;
-        }
-        if (d.root) HIP_TRY(c, launch_root_reduce(rootv, c->n_act, B, tmp, rawd, st));
+        int rc = enqueue_step<T>(c, p, d, B, pb, d.mm > 1 ? tab(d.mi) : nullptr, d.aa > 1 ? tab(d.ai) : nullptr,
+                                 d.root ? nullptr : tab(d.s));
+        if (rc) return rc;
     }
     return PG_OK;
 }
@@ -867,6 +904,8 @@ int count_impl(pg_ctx* c, const pg_plan* p, int ncol, const uint8_t* colors, uin
         drop_graphs(c);  // graphs hold arena addresses
         HIP_TRY(c, c->d_arena.alloc(mp.total));
     }
+    c->last_tile = tile;
+    c->last_recompute = 0;
     if (c->autotune) {
         rc = autotune<T>(c, p, tile);
         if (rc) return rc;
@@ -950,177 +989,299 @@ int build_transpose(pg_ctx* c)
     return PG_OK;
 }
 
-// Tables kept for the adjoint: every forward table lives through the whole
-// pass, plus the adjoint tables G_s, recomputed Y and dY.
+// Buffers of the adjoint beside the forward pass's: the recomputed aux sums
+// Y, dY (both sized for the widest subtemplate) and the cotangents.  With
+// keep_all every forward table and every G_s has its own slot for the whole
+// pass; otherwise (recompute) the tables live in a pool above the pass-long
+// buffers, and off_pool / pool_bytes describe it.
 struct GradPlan {
     MemPlan fwd;
+    bool keep_all = true;
     std::vector<size_t> off_G;
-    size_t off_Y = 0, off_dY = 0, off_cot = 0, total = 0;
+    size_t off_Y = 0, off_dY = 0, off_cot = 0, off_pool = 0, pool_bytes = 0, total = 0;
 };
+
+size_t table_bytes(const pg_ctx* c, const DevSub& d, int B, size_t tsz)
+{
+    return align_up((size_t)c->n_act * B * d.Wo * tsz, 256);
+}
 
 GradPlan plan_memory_grad(const pg_ctx* c, const pg_plan* p, int B, size_t tsz)
 {
     GradPlan g;
     g.fwd = plan_memory(c, p, B, tsz);
     // forward plan reuses table space; give every table its own slot instead
-    const size_t na = (size_t)c->n_act;
     size_t off = g.fwd.total;
     for (auto& d : p->subs) {
         if (d->root) continue;
         g.fwd.off_tab[d->s] = off;
-        off += align_up(na * B * d->Wo * tsz, 256);
+        off += table_bytes(c, *d, B, tsz);
     }
     g.off_G.assign(p->nsub, 0);
     for (auto& d : p->subs) {
         if (d->root) continue;
         g.off_G[d->s] = off;
-        off += align_up(na * B * d->Wo * tsz, 256);
+        off += table_bytes(c, *d, B, tsz);
     }
     int maxY = 1;
     for (auto& d : p->subs) maxY = std::max(maxY, d->Wy);
     g.off_Y = off;
-    off += align_up(na * B * maxY * tsz, 256);
+    off += align_up((size_t)c->n_act * B * maxY * tsz, 256);
     g.off_dY = off;
-    off += align_up(na * B * maxY * tsz, 256);
+    off += align_up((size_t)c->n_act * B * maxY * tsz, 256);
     g.off_cot = off;
     off += align_up((size_t)B * sizeof(double), 256);
     g.total = off;
-    g.fwd.total = off;
     return g;
 }
 
-// Reverse sweep of one pass (root first): for each non-leaf s, recompute Y,
-// form dY and G_main, add dY x T_aux to every entry's gradient, and pull
-// G_aux from the neighbours' dY (the transposed SpMM).
+// The backward step of one subtemplate d: from G_s (nullptr at the root,
+// whose cotangents are cot), the main and aux children's tables Tm / Ta,
+// write G_main into Gm and G_aux into Ga (nullptr for leaf children), and add
+// the step's share of every entry's gradient.  Order: dY (from G and the
+// main row), one gather pass that recomputes Y and adds dY . T_aux-term to
+// every entry's gradient, G_main (from G and Y), then G_aux by the
+// transposed gather of the neighbours' dY.
 template <typename T>
-int enqueue_backward(pg_ctx* c, const pg_plan* p, int B, const GradPlan& gp)
+int enqueue_sub_backward(pg_ctx* c, const pg_plan* p, const DevSub& d, int B, const PassBufs<T>& pb, T* Ybuf,
+                         T* dYbuf, const double* cot, const T* Tm, const T* Ta, const T* G, T* Gm, T* Ga)
 {
-    char* base = c->d_arena.as<char>();
-    const MemPlan& mp = gp.fwd;
-    const uint8_t* colors = (const uint8_t*)(base + mp.off_colors);
-    const uint8_t* ecol = (const uint8_t*)(base + mp.off_ecol);
-    const T* H = (const T*)(base + mp.off_H);
-    T* part = (T*)(base + mp.off_part);
-    T* Ybuf = (T*)(base + gp.off_Y);
-    T* dYbuf = (T*)(base + gp.off_dY);
     const hipStream_t st = c->stream;
-    const int k = p->k;
     const bool fp32 = sizeof(T) == 4;
-    const T* w = nullptr;
-    const T* wT = nullptr;
-    if (c->weighted) {
-        w = fp32 ? (const T*)c->d_w32.p : (const T*)c->d_w64.p;
-        wT = fp32 ? (const T*)c->d_wT32.p : (const T*)c->d_wT64.p;
-    }
-    SubArgs<T> base_args{};
-    base_args.rowptr = c->d_rowptr.as<int32_t>();
-    base_args.col = c->d_col.as<int32_t>();
-    base_args.colors = colors;
-    base_args.ecol = ecol;
-    base_args.B = B;
-    base_args.k = k;
-    base_args.K = p->K;
-    base_args.ng = 1;
-    base_args.rank_tab = p->rank_tab.as<int16_t>();
-    base_args.slot_begin = c->d_slot_begin.as<int32_t>();
-    base_args.out_mode = OUT_Y;
-    for (auto it = p->subs.rbegin(); it != p->subs.rend(); ++it) {
-        const DevSub& d = **it;
-        NodeArgs<T> na{};
-        na.n_act = c->n_act;
-        na.B = B;
-        na.k = k;
-        na.colors = colors;
-        na.G = d.root ? nullptr : (const T*)(base + gp.off_G[d.s]);
-        na.root_cot = (const double*)(base + gp.off_cot);
-        na.Tm = d.mm > 1 ? (const T*)(base + mp.off_tab[d.mi]) : nullptr;
-        na.Y = d.aa > 1 ? Ybuf : H;
-        na.y_from_h = d.aa > 1 ? 0 : 1;
-        na.dY = dYbuf;
-        na.Gm = d.mm > 1 ? (T*)(base + gp.off_G[d.mi]) : nullptr;
-        na.Wo = d.Wo;
-        na.Wy = d.Wy;
-        na.Wm = d.Wm;
-        na.Pa = d.Pa;
-        na.Pm = d.Pm;
-        na.by_aux = d.by_aux.as<uint32_t>();
-        na.by_main = d.by_main.as<uint32_t>();
-        na.scale = (d.scaled && fp32) ? (double)1.0e-12f : 1.0;
-        const std::string node_name = std::string("k_bwd_node<") + (fp32 ? "float>" : "double>");
-        if (d.aa > 1) {
-            // dY (from G and the main row), then one gather pass that recomputes
-            // Y and adds dY . T_aux-term to every entry's gradient, then G_main
-            na.part = 1;
+    const SubArgs<T> base_args = pass_args<T>(c, p, B, pb);
+    NodeArgs<T> na{};
+    na.n_act = c->n_act;
+    na.B = B;
+    na.k = p->k;
+    na.colors = pb.colors;
+    na.G = G;
+    na.root_cot = cot;
+    na.Tm = d.mm > 1 ? Tm : nullptr;
+    na.Y = d.aa > 1 ? Ybuf : pb.H;
+    na.y_from_h = d.aa > 1 ? 0 : 1;
+    na.dY = dYbuf;
+    na.Gm = d.mm > 1 ? Gm : nullptr;
+    na.Wo = d.Wo;
+    na.Wy = d.Wy;
+    na.Wm = d.Wm;
+    na.Pa = d.Pa;
+    na.Pm = d.Pm;
+    na.by_aux = d.by_aux.as<uint32_t>();
+    na.by_main = d.by_main.as<uint32_t>();
+    na.scale = (d.scaled && fp32) ? (double)1.0e-12f : 1.0;
+    const std::string node_name = std::string("k_bwd_node<") + (fp32 ? "float>" : "double>");
+    if (d.aa > 1) {
+        na.part = 1;
+        record(c, node_name, 0.0, true);
+        HIP_TRY(c, launch_bwd_node<T>(na, c->num_cus, st));
+        record(c, "", 0, false);
+        SubArgs<T> a = base_args;
+        a.out_mode = OUT_Y;
+        a.aux = Ta;
+        a.Wa = d.Wa;
+        a.lda = B * d.Wa;
+        a.Wy = d.Wy;
+        a.Wo = d.Wy;
+        a.ymask = d.ymask.as<uint16_t>();
+        a.gidx = d.gidx.as<int16_t>();
+        a.Wy4 = d.Wy4;
+        a.out = Ybuf;
+        a.ldo = B * d.Wy;
+        a.dy = dYbuf;
+        a.ldd = B * d.Wy;
+        a.grad = c->d_grad.as<double>();
+        int rc = launch_gathering<T>(c, AUX_GATHER_G, MAIN_LEAF, a, pb.part, B * d.Wy, 0.0);
+        if (rc) return rc;
+        if (na.Tm) {
+            na.part = 2;
             record(c, node_name, 0.0, true);
             HIP_TRY(c, launch_bwd_node<T>(na, c->num_cus, st));
             record(c, "", 0, false);
-            SubArgs<T> a = base_args;
-            a.w = w;
-            a.aux = (const T*)(base + mp.off_tab[d.ai]);
-            a.Wa = d.Wa;
-            a.lda = (int64_t)B * d.Wa;
-            a.Wy = d.Wy;
-            a.Wo = d.Wy;
-            a.ymask = d.ymask.as<uint16_t>();
-            a.gidx = d.gidx.as<int16_t>();
-            a.Wy4 = d.Wy4;
-            a.out = Ybuf;
-            a.ldo = (int64_t)B * d.Wy;
-            a.dy = dYbuf;
-            a.ldd = B * d.Wy;
-            a.grad = c->d_grad.as<double>();
-            int rc = launch_gathering<T>(c, AUX_GATHER_G, MAIN_LEAF, a, part, B * d.Wy, 0.0);
-            if (rc) return rc;
-            if (na.Tm) {
-                na.part = 2;
-                record(c, node_name, 0.0, true);
-                HIP_TRY(c, launch_bwd_node<T>(na, c->num_cus, st));
-                record(c, "", 0, false);
-            }
-        } else {
-            // aux child a leaf: Y is the histogram H; the entry's derivative is
-            // dY at the neighbour's color
-            record(c, node_name, 0.0, true);
-            HIP_TRY(c, launch_bwd_node<T>(na, c->num_cus, st));
-            record(c, "", 0, false);
-            EdgeGradArgs<T> eg{};
-            eg.nnz = c->nnz;
-            eg.B = B;
-            eg.k = k;
-            eg.K = p->K;
-            eg.erow = c->d_erow.as<int32_t>();
-            eg.col = c->d_col.as<int32_t>();
-            eg.colors = colors;
-            eg.ecol = ecol;
-            eg.dY = dYbuf;
-            eg.Wy = d.Wy;
-            eg.Ta = nullptr;
-            eg.Wa = d.Wa;
-            eg.ymask = d.ymask.as<uint16_t>();
-            eg.rank_tab = p->rank_tab.as<int16_t>();
-            eg.grad = c->d_grad.as<double>();
-            record(c, std::string("k_edge_grad<") + (fp32 ? "float>" : "double>"), 0.0, true);
-            HIP_TRY(c, launch_edge_grad<T>(eg, st));
-            record(c, "", 0, false);
         }
-        if (d.aa > 1) {
-            SubArgs<T> a = base_args;
-            a.w = wT;
-            a.aux = dYbuf;
-            a.lda = (int64_t)B * d.Wy;
-            a.Wa = d.Wy;  // per-coloring stride of the neighbours' dY rows
-            a.Wy = d.Wa;  // values per coloring: the aux child's row
-            a.Wo = d.Wa;
-            a.ymask = d.amask.as<uint16_t>();
-            a.gidx = d.gidxT.as<int16_t>();
-            a.Wy4 = d.Wa4;
-            a.out = (T*)(base + gp.off_G[d.ai]);
-            a.ldo = (int64_t)B * d.Wa;
-            int rc = launch_gathering<T>(c, AUX_GATHER_T, MAIN_LEAF, a, part, B * d.Wa, 0.0);
-            if (rc) return rc;
-        }
+        SubArgs<T> t = base_args;
+        t.out_mode = OUT_Y;
+        if (c->weighted) t.w = fp32 ? (const T*)c->d_wT32.p : (const T*)c->d_wT64.p;
+        t.aux = dYbuf;
+        t.lda = B * d.Wy;
+        t.Wa = d.Wy;  // per-coloring stride of the neighbours' dY rows
+        t.Wy = d.Wa;  // values per coloring: the aux child's row
+        t.Wo = d.Wa;
+        t.ymask = d.amask.as<uint16_t>();
+        t.gidx = d.gidxT.as<int16_t>();
+        t.Wy4 = d.Wa4;
+        t.out = Ga;
+        t.ldo = B * d.Wa;
+        rc = launch_gathering<T>(c, AUX_GATHER_T, MAIN_LEAF, t, pb.part, B * d.Wa, 0.0);
+        if (rc) return rc;
+    } else {
+        // aux child a leaf: Y is the histogram H; the entry's derivative is
+        // dY at the neighbour's color
+        record(c, node_name, 0.0, true);
+        HIP_TRY(c, launch_bwd_node<T>(na, c->num_cus, st));
+        record(c, "", 0, false);
+        EdgeGradArgs<T> eg{};
+        eg.nnz = c->nnz;
+        eg.B = B;
+        eg.k = p->k;
+        eg.K = p->K;
+        eg.erow = c->d_erow.as<int32_t>();
+        eg.col = c->d_col.as<int32_t>();
+        eg.colors = pb.colors;
+        eg.ecol = pb.ecol;
+        eg.dY = dYbuf;
+        eg.Wy = d.Wy;
+        eg.Ta = nullptr;
+        eg.Wa = d.Wa;
+        eg.ymask = d.ymask.as<uint16_t>();
+        eg.rank_tab = p->rank_tab.as<int16_t>();
+        eg.grad = c->d_grad.as<double>();
+        record(c, std::string("k_edge_grad<") + (fp32 ? "float>" : "double>"), 0.0, true);
+        HIP_TRY(c, launch_edge_grad<T>(eg, st));
+        record(c, "", 0, false);
     }
     return PG_OK;
+}
+
+// Reverse sweep with every forward table kept (root first).
+template <typename T>
+int enqueue_backward_kept(pg_ctx* c, const pg_plan* p, int B, const GradPlan& gp)
+{
+    char* base = c->d_arena.as<char>();
+    const PassBufs<T> pb = pass_bufs<T>(c, gp.fwd);
+    auto tab = [&](int s) { return (T*)(base + gp.fwd.off_tab[s]); };
+    auto gad = [&](int s) { return (T*)(base + gp.off_G[s]); };
+    for (auto it = p->subs.rbegin(); it != p->subs.rend(); ++it) {
+        const DevSub& d = **it;
+        int rc = enqueue_sub_backward<T>(c, p, d, B, pb, (T*)(base + gp.off_Y), (T*)(base + gp.off_dY),
+                                         (const double*)(base + gp.off_cot), d.mm > 1 ? tab(d.mi) : nullptr,
+                                         d.aa > 1 ? tab(d.ai) : nullptr, d.root ? nullptr : gad(d.s),
+                                         d.mm > 1 ? gad(d.mi) : nullptr, d.aa > 1 ? gad(d.ai) : nullptr);
+        if (rc) return rc;
+    }
+    return PG_OK;
+}
+
+// First-fit allocator of byte ranges in [0, cap) of the pool (host-side
+// bookkeeping; every user is ordered on the one stream, so a range freed here
+// may be handed out again at once).  With dry = true it only records the
+// peak.
+struct PoolAlloc {
+    size_t cap = 0, peak = 0;
+    std::map<size_t, size_t> used;  // offset -> bytes
+    static constexpr size_t kNone = ~(size_t)0;
+    size_t alloc(size_t bytes)
+    {
+        size_t off = 0;
+        for (auto& u : used) {
+            if (off + bytes <= u.first) break;
+            off = std::max(off, u.first + u.second);
+        }
+        if (off + bytes > cap) return kNone;
+        used[off] = bytes;
+        peak = std::max(peak, off + bytes);
+        return off;
+    }
+    void release(size_t off)
+    {
+        if (off != kNone) used.erase(off);
+    }
+};
+
+// The adjoint with tables recomputed: the backward step of s needs the
+// tables of its children, which are rebuilt from their subtrees (forward
+// steps into pool buffers, freed as soon as their parent step has run), and
+// hands G_main / G_aux on to the children's steps.  Each table is rebuilt
+// once per ancestor level, and the live set is one root-to-leaf path of G
+// tables plus the tables of the subtree being rebuilt.
+template <typename T>
+struct Recompute {
+    pg_ctx* c;
+    const pg_plan* p;
+    int B;
+    bool dry;
+    PoolAlloc pool;
+    char* pool_base = nullptr;
+    PassBufs<T> pb{};
+    T* Ybuf = nullptr;
+    T* dYbuf = nullptr;
+    const double* cot = nullptr;
+    std::vector<const DevSub*> sub;  // by subtemplate index
+
+    T* at(size_t off) const { return off == PoolAlloc::kNone ? nullptr : (T*)(pool_base + off); }
+
+    // rebuilds the table of non-leaf subtemplate x; *out = its pool offset
+    int materialize(int x, size_t* out)
+    {
+        const DevSub& d = *sub[x];
+        size_t m = PoolAlloc::kNone, a = PoolAlloc::kNone;
+        int rc = PG_OK;
+        if (d.mm > 1) rc = materialize(d.mi, &m);
+        if (rc == PG_OK && d.aa > 1) rc = materialize(d.ai, &a);
+        if (rc) return rc;
+        *out = pool.alloc(table_bytes(c, d, B, sizeof(T)));
+        if (*out == PoolAlloc::kNone) return PG_ERR_NOMEM;
+        if (!dry) rc = enqueue_step<T>(c, p, d, B, pb, at(m), at(a), at(*out));
+        pool.release(m);
+        pool.release(a);
+        return rc;
+    }
+
+    // backward step of x with its G table at pool offset g (none at the root)
+    int backward(int x, size_t g)
+    {
+        const DevSub& d = *sub[x];
+        size_t tm = PoolAlloc::kNone, ta = PoolAlloc::kNone, gm = PoolAlloc::kNone, ga = PoolAlloc::kNone;
+        int rc = PG_OK;
+        if (d.mm > 1) rc = materialize(d.mi, &tm);
+        if (rc == PG_OK && d.aa > 1) rc = materialize(d.ai, &ta);
+        if (rc) return rc;
+        if (d.mm > 1 && (gm = pool.alloc(table_bytes(c, *sub[d.mi], B, sizeof(T)))) == PoolAlloc::kNone)
+            return PG_ERR_NOMEM;
+        if (d.aa > 1 && (ga = pool.alloc(table_bytes(c, *sub[d.ai], B, sizeof(T)))) == PoolAlloc::kNone)
+            return PG_ERR_NOMEM;
+        if (!dry) rc = enqueue_sub_backward<T>(c, p, d, B, pb, Ybuf, dYbuf, cot, at(tm), at(ta), at(g), at(gm), at(ga));
+        if (rc) return rc;
+        pool.release(tm);
+        pool.release(ta);
+        pool.release(g);
+        if (d.mm > 1 && (rc = backward(d.mi, gm))) return rc;
+        if (d.aa > 1 && (rc = backward(d.ai, ga))) return rc;
+        return PG_OK;
+    }
+};
+
+// Memory plan of the recomputing adjoint: the pass-long buffers of the
+// forward plan, then Y, dY and the cotangents, then the pool, whose peak a
+// dry run of the sweep measures.  The forward pass's own tables overlap Y,
+// dY and the pool; they are dead once the forward pass has run.
+template <typename T>
+GradPlan plan_memory_recompute(pg_ctx* c, const pg_plan* p, int B, size_t budget)
+{
+    GradPlan g;
+    g.keep_all = false;
+    g.fwd = plan_memory(c, p, B, sizeof(T));
+    // plan_memory places the pass-long buffers first, below every table, so
+    // the lowest table offset is where they end
+    size_t tab_lo = ~(size_t)0;
+    for (auto& d : p->subs)
+        if (!d->root) tab_lo = std::min(tab_lo, g.fwd.off_tab[d->s]);
+    int maxY = 1;
+    for (auto& d : p->subs) maxY = std::max(maxY, d->Wy);
+    size_t off = tab_lo == ~(size_t)0 ? g.fwd.total : tab_lo;
+    g.off_Y = off;
+    off += align_up((size_t)c->n_act * B * maxY * sizeof(T), 256);
+    g.off_dY = off;
+    off += align_up((size_t)c->n_act * B * maxY * sizeof(T), 256);
+    g.off_cot = off;
+    off += align_up((size_t)B * sizeof(double), 256);
+    g.off_pool = off;
+    Recompute<T> r{c, p, B, true};
+    r.pool.cap = budget > off ? budget - off : 0;
+    r.sub.assign(p->nsub, nullptr);
+    for (auto& d : p->subs) r.sub[d->s] = d.get();
+    g.pool_bytes = r.backward(0, PoolAlloc::kNone) == PG_OK ? r.pool.peak : ~(size_t)0;
+    g.total = g.pool_bytes == ~(size_t)0 ? ~(size_t)0 : std::max(g.fwd.total, off + g.pool_bytes);
+    return g;
 }
 
 template <typename T>
@@ -1142,20 +1303,41 @@ int count_grad_impl(pg_ctx* c, const pg_plan* p, int ncol, const uint8_t* colors
         HIP_TRY(c, hipMemGetInfo(&fr, &tot));
         budget = (size_t)(0.8 * (double)(fr + c->d_arena.bytes));
     }
-    int tile = 0;
-    GradPlan gp;
+    // the largest batch tile that fits, with every table kept and with tables
+    // recomputed; option "adjoint_recompute": 0 keep, 1 recompute, -1 recompute
+    // only when it fits more colorings per pass than keeping does
     const int cap = c->batch_tile > 0 ? (int)std::min<int64_t>(c->batch_tile, ncol) : std::min(ncol, 64);
-    for (int B = cap; B >= 1; --B) {
+    auto lds_ok = [&](int B) {
         size_t lds = 0;
         for (auto& d : p->subs) lds = std::max(lds, sub_lds_max(c, p, *d, B, tsz));
-        if (lds > kLdsLimit) continue;
-        GradPlan g = plan_memory_grad(c, p, B, tsz);
-        if (g.total > budget) continue;
-        tile = B;
-        gp = g;
-        break;
-    }
+        return lds <= kLdsLimit;
+    };
+    int tile_keep = 0, tile_rec = 0;
+    GradPlan g_keep, g_rec;
+    if (c->adjoint_recompute != 1)
+        for (int B = cap; B >= 1; --B) {
+            if (!lds_ok(B)) continue;
+            GradPlan g = plan_memory_grad(c, p, B, tsz);
+            if (g.total > budget) continue;
+            tile_keep = B;
+            g_keep = g;
+            break;
+        }
+    if (c->adjoint_recompute == 1 || (c->adjoint_recompute == -1 && tile_keep < cap))
+        for (int B = cap; B > tile_keep; --B) {
+            if (!lds_ok(B)) continue;
+            GradPlan g = plan_memory_recompute<T>(c, p, B, budget);
+            if (g.total > budget) continue;
+            tile_rec = B;
+            g_rec = g;
+            break;
+        }
+    const bool use_rec = tile_rec > 0;
+    const int tile = use_rec ? tile_rec : tile_keep;
+    const GradPlan gp = use_rec ? g_rec : g_keep;
     if (tile == 0) return set_err(c, PG_ERR_NOMEM, "adjoint tables of one coloring do not fit the device");
+    c->last_tile = tile;
+    c->last_recompute = use_rec ? 1 : 0;
     if (gp.total > c->d_arena.bytes) {
         drop_graphs(c);
         HIP_TRY(c, c->d_arena.alloc(gp.total));
@@ -1188,16 +1370,37 @@ int count_grad_impl(pg_ctx* c, const pg_plan* p, int ncol, const uint8_t* colors
     std::vector<double> raw;
     for (int first = 0; first < ncol && rc == PG_OK; first += tile) {
         const int B = std::min(tile, ncol - first);
-        const GradPlan g = B == tile ? gp : plan_memory_grad(c, p, B, tsz);
+        const GradPlan g = B == tile ? gp : use_rec ? plan_memory_recompute<T>(c, p, B, gp.total)
+                                                    : plan_memory_grad(c, p, B, tsz);
+        if (g.total > gp.total) {
+            rc = set_err(c, PG_ERR_NOMEM, "adjoint plan of the last pass exceeds the first");
+            break;
+        }
         const PassParams hp{seed, first};
         HIP_TRY(c, hipMemcpyAsync(base + g.fwd.off_params, &hp, sizeof hp, hipMemcpyHostToDevice, c->stream));
         std::vector<double> cot(B, 1.0);
         if (cotangent) std::copy(cotangent + first, cotangent + first + B, cot.begin());
-        HIP_TRY(c, hipMemcpyAsync(base + g.off_cot, cot.data(), sizeof(double) * B, hipMemcpyHostToDevice, c->stream));
         std::vector<LaunchRec> recs;  // filled when option "timing" is on
         c->cur_recs = &recs;
         rc = enqueue_pass<T>(c, p, B, d_in, g.fwd);
-        if (rc == PG_OK) rc = enqueue_backward<T>(c, p, B, g);
+        // the cotangents go in after the forward pass, whose tables may share their space
+        if (rc == PG_OK && hipMemcpyAsync(base + g.off_cot, cot.data(), sizeof(double) * B, hipMemcpyHostToDevice,
+                                          c->stream) != hipSuccess)
+            rc = set_err(c, PG_ERR_HIP, "cotangent upload failed");
+        if (rc == PG_OK && g.keep_all) rc = enqueue_backward_kept<T>(c, p, B, g);
+        if (rc == PG_OK && !g.keep_all) {
+            Recompute<T> r{c, p, B, false};
+            r.pool.cap = g.pool_bytes;
+            r.pool_base = base + g.off_pool;
+            r.pb = pass_bufs<T>(c, g.fwd);
+            r.Ybuf = (T*)(base + g.off_Y);
+            r.dYbuf = (T*)(base + g.off_dY);
+            r.cot = (const double*)(base + g.off_cot);
+            r.sub.assign(p->nsub, nullptr);
+            for (auto& d : p->subs) r.sub[d->s] = d.get();
+            rc = r.backward(0, PoolAlloc::kNone);
+            if (rc == PG_ERR_NOMEM) set_err(c, rc, "adjoint recompute pool exhausted");
+        }
         c->cur_recs = nullptr;
         if (rc) {
             destroy_recs(recs);
@@ -1392,6 +1595,9 @@ extern "C" int pg_set_option(pg_ctx* c, const char* key, int64_t v)
             drop_graphs(c);
             c->aux_rows = v;
         }
+    } else if (k == "adjoint_recompute") {
+        if (v < -1 || v > 1) return set_err(c, PG_ERR_INVALID, "adjoint_recompute must be -1, 0 or 1");
+        c->adjoint_recompute = v;
     } else if (k == "stage_rows") {
         if (v != 0 && v != 1) return set_err(c, PG_ERR_INVALID, "stage_rows must be 0 or 1");
         if (v != c->stage_rows) {
@@ -1425,6 +1631,9 @@ extern "C" int pg_get_option(const pg_ctx* c, const char* key, int64_t* v)
     else if (k == "blocks_per_cu") *v = c->blocks_per_cu;
     else if (k == "aux_rows") *v = c->aux_rows;
     else if (k == "stage_rows") *v = c->stage_rows;
+    else if (k == "adjoint_recompute") *v = c->adjoint_recompute;
+    else if (k == "last_tile") *v = c->last_tile;
+    else if (k == "last_recompute") *v = c->last_recompute;
     else if (k == "graphs") *v = c->use_graphs ? 1 : 0;
     else return PG_ERR_INVALID;
     return PG_OK;

@@ -162,3 +162,44 @@ def test_autograd_function(gpu_ctx):
     r2, g2 = gpu_ctx.count_grad(plan, 4, seed=2, precision="fp64", cotangent=coef.cpu().numpy())
     assert np.allclose(raw.detach().cpu().numpy(), r2, rtol=1e-12)
     assert np.allclose(w.grad.cpu().numpy(), g2.astype(np.float32), rtol=1e-6, atol=1e-6)
+
+
+TREE10 = [(0, 1), (1, 2), (2, 3), (3, 4), (1, 5), (5, 6), (5, 7), (7, 8), (7, 9)]
+TREE12 = [(0, 1), (1, 2), (2, 3), (3, 4), (4, 5), (2, 6), (6, 7), (7, 8), (6, 9), (9, 10), (10, 11)]
+
+
+@pytest.mark.parametrize("tv,te", [(7, TREE7), (10, TREE10), (12, TREE12)])
+def test_recompute_matches_kept_tables(gpu_ctx, tv, te):
+    """The adjoint that rebuilds tables from their subtrees (option
+    "adjoint_recompute" = 1) equals the one that keeps every table: counts
+    bit for bit, gradients exactly for integer contributions (unit weights,
+    FP64) and to FP64 rounding otherwise (the two sweeps visit the
+    subtemplates in different orders)."""
+    rng = np.random.default_rng(tv)
+    n = 1 << 10
+    s, d = sg.rmat_edges(10, 8, seed=tv)
+    colptr, rowidx = sg.csc_from_edges(n, s, d)
+    rows = np.repeat(np.arange(n), np.diff(colptr))
+    lo, hi = np.minimum(rows, rowidx), np.maximum(rows, rowidx)
+    w = (0.5 + ((lo * 7919 + hi * 104729) % 1000) / 1000.0).astype(np.float32)  # symmetric
+    saved = gpu_ctx.get_option("adjoint_recompute")
+    try:
+        for weighted in (False, True):
+            gpu_ctx.upload_graph(colptr, rowidx, vals=w if weighted else None)
+            plan = gpu_ctx.plan(tv, te)
+            for prec in ("fp64", "fp32"):
+                out = {}
+                for mode in (0, 1):
+                    gpu_ctx.set_option("adjoint_recompute", mode)
+                    print(f"[recompute] k={tv} weighted={weighted} {prec} mode={mode}", flush=True)
+                    out[mode] = gpu_ctx.count_grad(plan, 5, seed=tv, precision=prec)
+                    assert gpu_ctx.get_option("last_recompute") == mode
+                (r0, g0), (r1, g1) = out[0], out[1]
+                assert np.array_equal(r0, r1)
+                if prec == "fp64" and not weighted and np.abs(g0).max() < 2.0 ** 50:
+                    assert np.array_equal(g0, g1)  # integer partial sums: exact in any order
+                else:
+                    assert np.allclose(g0, g1, rtol=1e-12, atol=0.0)
+                assert np.any(g0 != 0)
+    finally:
+        gpu_ctx.set_option("adjoint_recompute", saved)
