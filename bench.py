@@ -83,6 +83,22 @@ def cpu_baseline(n_decl, src, dst, seed, ncol, gpu_raw_fn):
             "seconds_per_coloring": sum(secs) / ncol, "parity_max_rel_err": err}
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_plain_rmat_s20_ef8_tree7.json")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` on this workload from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE runs (calibrated against a copy kernel
+    of known size), or None when that file does not cover the kernel."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            rec = json.load(f)
+    except (OSError, ValueError):
+        return None
+    k = rec.get("kernels", {}).get(kernel)
+    return None if k is None else k["hbm_bytes_per_launch"]
+
+
 def _timed(fn, reps, sync):
     fn()
     sync()
@@ -243,6 +259,9 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--cpu-colorings", type=int, default=12)
     ap.add_argument("--variants", default="grad,s22", help="comma list of extra measurements: grad, s22, none")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="first run one elementwise kernel that reads and writes MIB MiB (the known-size "
+                         "kernel rocprofv3 FETCH_SIZE / WRITE_SIZE readings are scaled by)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print(json.dumps({"error": "this build measures one GPU: bench.py supports --gpus 1 only; "
@@ -250,6 +269,15 @@ def main():
         sys.exit(2)
 
     import subgraph_amd as sg
+
+    if args.copy_calibration > 0:
+        import torch
+        src_t = torch.ones(args.copy_calibration << 18, dtype=torch.float32, device="cuda")
+        dst_t = torch.empty_like(src_t)
+        torch.add(src_t, 1.0, out=dst_t)  # reads and writes copy_calibration MiB
+        torch.cuda.synchronize()
+        log(f"copy calibration: {args.copy_calibration} MiB read + {args.copy_calibration} MiB written")
+        del src_t, dst_t
 
     scale, ef = 20, 8
     t0 = time.perf_counter()
@@ -315,7 +343,7 @@ def main():
                    "template": "7-vertex tree 0-1,1-2,2-3,1-4,4-5,4-6",
                    "colorings_per_step": 1, "n": int(n), "n_active": st["n_active"], "nnz": int(len(rowidx))},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": kern,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(kern), "kernel": kern,
                      "launches": klaunch, "avg_launch_ms": kms / klaunch,
                      "algorithmic_bytes_per_launch": kbytes / klaunch},
     }
