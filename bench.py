@@ -176,7 +176,7 @@ def dominant_kernel(ctx, run):
             "achieved_gbs": nbytes / (ms * 1e-3) / 1e9, "frac": nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
 
 
-TUNED_KEYS = ("stage_rows", "aux_rows", "bucket_high", "bucket_mid", "hub_degree", "batch_tile")
+TUNED_KEYS = ("reg_cols", "stage_rows", "aux_rows", "bucket_high", "bucket_mid", "hub_degree", "batch_tile")
 
 
 def s22_variant(sync, with_grad=True):

@@ -58,13 +58,15 @@ const char* pg_last_error(const pg_ctx* ctx);
  *   "batch_tile"   colorings per pass, 0 = automatic (default 0)
  *   "mem_budget_mb" device bytes the tables may use, 0 = 90% of free memory
  *   "autotune"     1 = before the first pass of each (plan, batch tile, precision), time
- *                  one pass per candidate of stage_rows, aux_rows, bucket_high,
+ *                  one pass per candidate of reg_cols, stage_rows, aux_rows, bucket_high,
  *                  bucket_mid and hub_degree (hub_chunk follows hub_degree) and keep
  *                  the fastest; pg_get_option then reports the choice (default 0)
  *   "timing"       1 = event pair around every launch, passes run eagerly (default 0)
  *   "graphs"       1 = each pass replays a captured hipGraph (default 1)
  *   "blocks_per_cu" workgroups per CU of persistent launches, 0 = occupancy query
  *   "stage_rows"   1 = gathers stage neighbour rows in LDS, 0 = direct loads (default 0)
+ *   "reg_cols"     values per lane of gathers: 0 = one for Y rows narrower than four values
+ *                  (the root), else four (default); 1 or 4 = that for every gather
  *   "aux_rows"     neighbour sums row-wise with LDS atomics: 1 always, 0 never (default),
  *                  -1 when most of a neighbour row is used
  *   "adjoint_recompute" pg_count_grad: 0 keeps every forward table for the reverse

@@ -135,6 +135,7 @@ struct pg_ctx {
     int64_t blocks_per_cu = 0;  // 0: resident workgroups from the occupancy query
     int64_t aux_rows = 0;       // neighbour sums row-wise: 1 always, 0 never, -1 per subtemplate
     int64_t stage_rows = 0;     // gathers read neighbour rows staged in LDS (1) or directly (0)
+    int64_t reg_cols = 0;       // values per lane of gathers: 0 one for rows under kRegCols, else 1 or 4
     int64_t adjoint_recompute = -1;  // adjoint tables: 0 kept, 1 recomputed, -1 recomputed when that fits more
     int64_t last_tile = 0, last_recompute = 0;  // of the last count / gradient call (read-only options)
     // graph
@@ -199,8 +200,8 @@ int pow2_at_least(int x)
 
 // column lanes for `cols` value slots (1..64)
 int cl_for(int cols) { return std::min(64, pow2_at_least(std::max(cols, 1))); }
-// value slots of a neighbour-summing launch: kRegCols values of one coloring each
-int slots_for(int B, int W) { return B * ((W + kRegCols - 1) / kRegCols); }
+// value slots of a neighbour-summing launch: rc values of one coloring each
+int slots_for(int B, int W, int rc) { return B * ((W + rc - 1) / rc); }
 // lanes of a team that only streams rows (no neighbour loop)
 int team_for(int cols) { return std::max(8, cl_for(cols)); }
 
@@ -320,11 +321,24 @@ int aux_mode_for(const pg_ctx* c, const pg_plan* p, const DevSub& d)
     return 2 * (p->K - d.aa + 1) >= p->K ? AUX_ROWS : AUX_GATHER;
 }
 
+// Values per lane of a neighbour-summing launch: kRegCols, except for gathers
+// whose Y row per coloring is narrower than that (the root's single value,
+// where kRegCols per lane would load three unused values per neighbour), which
+// take one.  Option "reg_cols": 0 that rule, 1 or kRegCols for every gather.
+int rc_for(const pg_ctx* c, int aux, int B, int Wy)
+{
+    (void)B;
+    const bool gather = aux == AUX_GATHER || aux == AUX_GATHER_G || aux == AUX_GATHER_T;
+    if (!gather) return kRegCols;
+    if (c->reg_cols != 0) return (int)c->reg_cols;
+    return Wy < kRegCols ? 1 : kRegCols;
+}
+
 // column lanes of a neighbour-summing launch
-int cl_of(int aux, int B, int k, int Wy, int Wa)
+int cl_of(int aux, int B, int k, int Wy, int Wa, int rc)
 {
     if (aux == AUX_ROWS) return cl_for(B * Wa);
-    return cl_for(slots_for(B, aux == AUX_COLORS ? k : Wy));
+    return cl_for(slots_for(B, aux == AUX_COLORS ? k : Wy, rc));
 }
 
 // LDS row staging of a gathering launch (AUX_GATHER with a gather index
@@ -382,8 +396,9 @@ size_t sub_lds_max_t(const pg_ctx* c, const pg_plan* p, const DevSub& d, int B)
     size_t m = lds_of<T>(fin_team, d.aa == 1 ? AUX_READ : AUX_PARTIALS, mm, 1, out, B, p, d);
     if (d.aa == 1) return m;
     const int aux = aux_mode_for(c, p, d);
-    const int cl = cl_of(aux, B, p->k, d.Wy, d.Wa);
-    const int sp = stage_pad(c, aux, d.gidx.p != nullptr, B, d.Wy, d.Wa, cl, sizeof(T));
+    const int rc = rc_for(c, aux, B, d.Wy);
+    const int cl = cl_of(aux, B, p->k, d.Wy, d.Wa, rc);
+    const int sp = rc == kRegCols ? stage_pad(c, aux, d.gidx.p != nullptr, B, d.Wy, d.Wa, cl, sizeof(T)) : 0;
     const int aux_l = sp > 0 ? AUX_STAGED : aux;
     m = std::max(m, lds_of<T>(64, aux_l, mm, 64 / cl, OUT_PARTIAL, B, p, d, sp));
     for (int lanes : {64, 16, 8}) {
@@ -440,10 +455,11 @@ void destroy_recs(std::vector<LaunchRec>& recs)
 }
 
 template <typename T>
-std::string kname(int team, int aux, int main_mode)
+std::string kname(int team, int aux, int main_mode, int rc = kRegCols)
 {
     char buf[96];
-    snprintf(buf, sizeof buf, "k_sub<%s,%d,%d,%d>", sizeof(T) == 4 ? "float" : "double", team, aux, main_mode);
+    snprintf(buf, sizeof buf, "k_sub<%s,%d,%d,%d%s>", sizeof(T) == 4 ? "float" : "double", team, aux, main_mode,
+             rc == kRegCols ? "" : ",1");
     return buf;
 }
 
@@ -456,8 +472,9 @@ std::string kname(int team, int aux, int main_mode)
 template <typename T>
 int launch_gathering(pg_ctx* c, int aux, int main_mode, SubArgs<T> a, T* part, int y_cols, double bytes)
 {
-    const int cl = cl_of(aux, a.B, a.k, a.Wy, a.Wa);
-    a.segpad = stage_pad(c, aux, a.gidx != nullptr, a.B, a.Wy, a.Wa, cl, sizeof(T));
+    const int rc = rc_for(c, aux, a.B, a.Wy);
+    const int cl = cl_of(aux, a.B, a.k, a.Wy, a.Wa, rc);
+    a.segpad = rc == kRegCols ? stage_pad(c, aux, a.gidx != nullptr, a.B, a.Wy, a.Wa, cl, sizeof(T)) : 0;
     if (a.segpad > 0) aux = AUX_STAGED;
     const hipStream_t st = c->stream;
     const int cus = c->num_cus;
@@ -472,8 +489,8 @@ int launch_gathering(pg_ctx* c, int aux, int main_mode, SubArgs<T> a, T* part, i
         h.n_items = c->n_hub_items;
         h.item_base = 0;
         h.ng = 64 / cl;
-        record(c, kname<T>(64, aux, main_mode), share(c->h_rowptr[c->n_hub], 0), true);
-        HIP_TRY(c, launch_sub<T>(64, aux, main_mode, h, cus, c->blocks_per_cu, st));
+        record(c, kname<T>(64, aux, main_mode, rc), share(c->h_rowptr[c->n_hub], 0), true);
+        HIP_TRY(c, launch_sub<T>(64, aux, main_mode, rc, h, cus, c->blocks_per_cu, st));
         record(c, "", 0, false);
     }
     struct Bucket { int32_t lo, hi; int lanes; };
@@ -486,9 +503,9 @@ int launch_gathering(pg_ctx* c, int aux, int main_mode, SubArgs<T> a, T* part, i
         r.item_base = bk.lo;
         r.n_items = bk.hi - bk.lo;
         r.ng = team / cl;
-        record(c, kname<T>(team, aux, main_mode),
+        record(c, kname<T>(team, aux, main_mode, rc),
                share((double)c->h_rowptr[bk.hi] - c->h_rowptr[bk.lo], bk.hi - bk.lo), true);
-        HIP_TRY(c, launch_sub<T>(team, aux, main_mode, r, cus, c->blocks_per_cu, st));
+        HIP_TRY(c, launch_sub<T>(team, aux, main_mode, rc, r, cus, c->blocks_per_cu, st));
         record(c, "", 0, false);
     }
     if (c->n_hub > 0) {
@@ -501,7 +518,7 @@ int launch_gathering(pg_ctx* c, int aux, int main_mode, SubArgs<T> a, T* part, i
         f.ng = 1;
         const int team = team_for(std::max(y_cols, a.B * a.Wo));
         record(c, kname<T>(team, AUX_PARTIALS, main_mode), share(0, c->n_hub), true);
-        HIP_TRY(c, launch_sub<T>(team, AUX_PARTIALS, main_mode, f, cus, c->blocks_per_cu, st));
+        HIP_TRY(c, launch_sub<T>(team, AUX_PARTIALS, main_mode, kRegCols, f, cus, c->blocks_per_cu, st));
         record(c, "", 0, false);
     }
     return PG_OK;
@@ -593,7 +610,7 @@ int enqueue_step(pg_ctx* c, const pg_plan* p, const DevSub& d, int B, const Pass
         a.n_items = c->n_act;
         a.item_base = 0;
         record(c, kname<T>(team, AUX_READ, main_mode), tab_bytes, true);
-        HIP_TRY(c, launch_sub<T>(team, AUX_READ, main_mode, a, c->num_cus, c->blocks_per_cu, st));
+        HIP_TRY(c, launch_sub<T>(team, AUX_READ, main_mode, kRegCols, a, c->num_cus, c->blocks_per_cu, st));
         record(c, "", 0, false);
     } else {
         a.aux = aux_tab;
@@ -791,6 +808,7 @@ struct Knob {
 const std::vector<Knob>& tune_knobs()
 {
     static const std::vector<Knob> knobs = {
+        {"reg_cols", &pg_ctx::reg_cols, {0, 1}},
         {"stage_rows", &pg_ctx::stage_rows, {0, 1}},
         {"aux_rows", &pg_ctx::aux_rows, {0, -1}},
         {"bucket_high", &pg_ctx::bucket_high, {16, 32, 64}},
@@ -1598,6 +1616,13 @@ extern "C" int pg_set_option(pg_ctx* c, const char* key, int64_t v)
     } else if (k == "adjoint_recompute") {
         if (v < -1 || v > 1) return set_err(c, PG_ERR_INVALID, "adjoint_recompute must be -1, 0 or 1");
         c->adjoint_recompute = v;
+    } else if (k == "reg_cols") {
+        if (v != 0 && v != 1 && v != kRegCols)
+            return set_err(c, PG_ERR_INVALID, "reg_cols must be 0, 1 or %d", kRegCols);
+        if (v != c->reg_cols) {
+            drop_graphs(c);
+            c->reg_cols = v;
+        }
     } else if (k == "stage_rows") {
         if (v != 0 && v != 1) return set_err(c, PG_ERR_INVALID, "stage_rows must be 0 or 1");
         if (v != c->stage_rows) {
@@ -1631,6 +1656,7 @@ extern "C" int pg_get_option(const pg_ctx* c, const char* key, int64_t* v)
     else if (k == "blocks_per_cu") *v = c->blocks_per_cu;
     else if (k == "aux_rows") *v = c->aux_rows;
     else if (k == "stage_rows") *v = c->stage_rows;
+    else if (k == "reg_cols") *v = c->reg_cols;
     else if (k == "adjoint_recompute") *v = c->adjoint_recompute;
     else if (k == "last_tile") *v = c->last_tile;
     else if (k == "last_recompute") *v = c->last_recompute;

@@ -117,7 +117,7 @@ size_t sub_lds_bytes(int team, int aux, int main_mode, const SubArgs<T>& a)
 template size_t sub_lds_bytes<float>(int, int, int, const SubArgs<float>&);
 template size_t sub_lds_bytes<double>(int, int, int, const SubArgs<double>&);
 
-template <typename T, int TEAM, int AUX, int MAIN>
+template <typename T, int TEAM, int AUX, int MAIN, int RC>
 __global__ __launch_bounds__(kBlock) void k_sub(const SubArgs<T> a)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -137,6 +137,7 @@ __global__ __launch_bounds__(kBlock) void k_sub(const SubArgs<T> a)
     const bool use_gidx =
         (AUX == AUX_GATHER || AUX == AUX_STAGED || AUX == AUX_GATHER_T || AUX == AUX_GATHER_G) && a.gidx != nullptr;
     constexpr bool staged = AUX == AUX_STAGED;
+    static_assert(RC == kRegCols || gathers_rows(AUX), "one value per lane is a gather layout");
     const bool y_in_lds = !(mode == OUT_Y || mode == OUT_PARTIAL) || AUX == AUX_ROWS;
     const int gstride = staged ? stage_group_stride(a.segpad, TEAM / ng) : 0;
     const int nstage = ng * gstride;
@@ -271,26 +272,26 @@ __global__ __launch_bounds__(kBlock) void k_sub(const SubArgs<T> a)
                 else sY[t] = sum;
             }
         } else if (gathers_rows(AUX) || AUX == AUX_COLORS) {
-            // a slot is kRegCols consecutive Y values of one coloring; column
+            // a slot is RC consecutive Y values of one coloring; column
             // lane cln of every group owns slot s0 + cln of each chunk
             const int W = gathers_rows(AUX) ? a.Wy : a.k;
-            const int spb = (W + kRegCols - 1) / kRegCols;
+            const int spb = (W + RC - 1) / RC;
             const int nslots = B * spb;
             for (int s0 = 0; s0 < nslots; s0 += cl) {
                 const int sid = s0 + cln;
                 const bool sv = active && sid < nslots;
                 const int b = sv ? sid / spb : 0;
-                const int i0 = (sid - b * spb) * kRegCols;
-                T acc[kRegCols];
+                const int i0 = (sid - b * spb) * RC;
+                T acc[RC];
                 // k-bit color set of each value (GATHER: avoids col(v), 0 = unused;
                 // GATHER_T: holds col(v), all ones = unused); COLORS: the value's color
-                uint32_t amask[kRegCols];
+                uint32_t amask[RC];
                 const uint32_t cv = sv && gathers_rows(AUX) ? a.colors[(int64_t)v * B + b] : 0u;
-                // gidx row of this slot's kRegCols values for neighbour color 0
+                // gidx row of this slot's RC values for neighbour color 0
                 // (a slot past the end reads the all -1 rows)
                 const int gbase = sv ? ((int)cv * a.k) * a.Wy4 + i0 : a.k * a.k * a.Wy4;
 #pragma unroll
-                for (int r = 0; r < kRegCols; ++r) {
+                for (int r = 0; r < RC; ++r) {
                     acc[r] = T(0);
                     const int i = i0 + r;
                     const bool ok = sv && i < W;
@@ -301,17 +302,19 @@ __global__ __launch_bounds__(kBlock) void k_sub(const SubArgs<T> a)
                 }
                 const int aoff = b * a.Wa;
                 // adjoint: this slot's dY values (zero past the row's end)
-                T dyv[kRegCols];
+                T dyv[RC];
                 constexpr bool edge_grad = AUX == AUX_GATHER_G;
                 if (edge_grad)
 #pragma unroll
-                    for (int r = 0; r < kRegCols; ++r)
+                    for (int r = 0; r < RC; ++r)
                         dyv[r] = sv && i0 + r < W ? a.dy[(int64_t)v * a.ldd + b * W + i0 + r] : T(0);
                 // a neighbour past the end reads as the vertex itself with a
                 // color that adds nothing (col(v) is never in a GATHER set and
                 // always in a GATHER_T set)
                 const uint32_t dead = gathers_rows(AUX) ? cv : 0xffu;
-                if (staged && active) {
+                if constexpr (AUX == AUX_STAGED) {
+                    static_assert(RC == 4, "the staged path reads a slot's 4 indices at once");
+                    if (active) {
                     // Rows staged in LDS: the group copies, per neighbour, the
                     // segment of its row that holds this chunk's colorings
                     // (coalesced), then each lane reads its values there.
@@ -384,7 +387,8 @@ __global__ __launch_bounds__(kBlock) void k_sub(const SubArgs<T> a)
                             uB[q] = uC[q]; cB[q] = cC[q]; wB[q] = wC[q]; oB[q] = oC[q];
                         }
                     }
-                } else if (!staged && active) {
+                    }
+                } else if (active) {
                     const int step = ng;
                     int e = e0 + grp;
                     // software pipeline: the ids, colors and weights of the
@@ -422,20 +426,27 @@ __global__ __launch_bounds__(kBlock) void k_sub(const SubArgs<T> a)
                             ok[q] = okn[q];
                             fetch(e + (4 + q) * step, q);
                         }
-                        T x[4][kRegCols];
+                        T x[4][RC];
                         if (use_gidx) {
-                            static_assert(kRegCols == 4, "one 8-byte LDS read holds a slot's indices");
 #pragma unroll
                             for (int q = 0; q < 4; ++q) {
                                 const int gb = ok[q] ? gbase : sbase;
-                                const int2 ids =
-                                    *reinterpret_cast<const int2*>(s_gidx + gb + (int)__umul24(cu[q], a.Wy4));
-                                const int id[4] = {(int)(int16_t)(ids.x & 0xffff), ids.x >> 16,
-                                                   (int)(int16_t)(ids.y & 0xffff), ids.y >> 16};
+                                const int16_t* gp = s_gidx + gb + (int)__umul24(cu[q], a.Wy4);
+                                int id[RC];
+                                if constexpr (RC == 4) {  // one 8-byte LDS read holds the slot's indices
+                                    const int2 ids = *reinterpret_cast<const int2*>(gp);
+                                    id[0] = (int)(int16_t)(ids.x & 0xffff);
+                                    id[1] = ids.x >> 16;
+                                    id[2] = (int)(int16_t)(ids.y & 0xffff);
+                                    id[3] = ids.y >> 16;
+                                } else {
+#pragma unroll
+                                    for (int r = 0; r < RC; ++r) id[r] = gp[r];
+                                }
                                 const T* row = rowbase + (int64_t)u[q] * a.lda;
                                 // every lane reads (index 0 for "no value") and selects
 #pragma unroll
-                                for (int r = 0; r < kRegCols; ++r) {
+                                for (int r = 0; r < RC; ++r) {
                                     const T val = row[(unsigned)(id[r] < 0 ? 0 : id[r])];
                                     x[q][r] = id[r] < 0 ? T(0) : val;
                                 }
@@ -446,14 +457,14 @@ __global__ __launch_bounds__(kBlock) void k_sub(const SubArgs<T> a)
                                 const int uq = ok[q] ? u[q] : v;
                                 const uint32_t cq = ok[q] ? cu[q] : dead;
 #pragma unroll
-                                for (int r = 0; r < kRegCols; ++r)
+                                for (int r = 0; r < RC; ++r)
                                     x[q][r] = aux_term<T, AUX>(a, s_rank, amask[r], aoff, uq, cq);
                             }
                         }
 #pragma unroll
                         for (int q = 0; q < 4; ++q)
 #pragma unroll
-                            for (int r = 0; r < kRegCols; ++r)
+                            for (int r = 0; r < RC; ++r)
                                 acc[r] = a.w ? acc[r] + wv[q] * x[q][r] : acc[r] + x[q][r];
                         if (edge_grad) {
                             // d Y[v] / d w_e = x_e: contract with dY over the group's
@@ -467,7 +478,7 @@ __global__ __launch_bounds__(kBlock) void k_sub(const SubArgs<T> a)
                             for (int q = 0; q < 4; ++q) {
                                 dot[q] = T(0);
 #pragma unroll
-                                for (int r = 0; r < kRegCols; ++r) dot[q] = dot[q] + dyv[r] * x[q][r];
+                                for (int r = 0; r < RC; ++r) dot[q] = dot[q] + dyv[r] * x[q][r];
                             }
                             if (cl >= 4) {
                                 const bool h1 = cln & 1, h2 = (cln >> 1) & 1;
@@ -491,10 +502,10 @@ __global__ __launch_bounds__(kBlock) void k_sub(const SubArgs<T> a)
                 // fold the neighbour groups: fixed xor tree, identical in every lane
                 for (int st = cl; st < TEAM; st <<= 1)
 #pragma unroll
-                    for (int r = 0; r < kRegCols; ++r) acc[r] = acc[r] + __shfl_xor(acc[r], st, TEAM);
+                    for (int r = 0; r < RC; ++r) acc[r] = acc[r] + __shfl_xor(acc[r], st, TEAM);
                 if (sv && grp == 0) {
 #pragma unroll
-                    for (int r = 0; r < kRegCols; ++r) {
+                    for (int r = 0; r < RC; ++r) {
                         const int i = i0 + r;
                         if (i >= W) continue;
                         const int cc = b * W + i;
@@ -594,7 +605,7 @@ __global__ __launch_bounds__(kBlock) void k_sub(const SubArgs<T> a)
 // Persistent grid: as many workgroups as can be resident at once (occupancy
 // for this instantiation and LDS size, or the caller's blocks_per_cu, times
 // the CU count), never more than there are items.
-template <typename T, int TEAM, int AUX, int MAIN>
+template <typename T, int TEAM, int AUX, int MAIN, int RC>
 static hipError_t launch_one(const SubArgs<T>& a, int num_cus, int blocks_per_cu, hipStream_t st)
 {
     const size_t lds = sub_lds_bytes<T>(TEAM, AUX, MAIN, a);
@@ -608,7 +619,7 @@ static hipError_t launch_one(const SubArgs<T>& a, int num_cus, int blocks_per_cu
             per_cu = it->second;
         } else {
             hipError_t e =
-                hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sub<T, TEAM, AUX, MAIN>, kBlock, lds);
+                hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sub<T, TEAM, AUX, MAIN, RC>, kBlock, lds);
             if (e != hipSuccess) return e;
             if (per_cu < 1) return hipErrorInvalidConfiguration;
             per_cu_cache[lds] = per_cu;
@@ -616,54 +627,66 @@ static hipError_t launch_one(const SubArgs<T>& a, int num_cus, int blocks_per_cu
     }
     const int64_t cap = (int64_t)per_cu * num_cus;
     const int grid = (int)(need < cap ? need : cap);
-    hipLaunchKernelGGL((k_sub<T, TEAM, AUX, MAIN>), dim3(grid), dim3(kBlock), lds, st, a);
+    hipLaunchKernelGGL((k_sub<T, TEAM, AUX, MAIN, RC>), dim3(grid), dim3(kBlock), lds, st, a);
     return hipGetLastError();
 }
 
+// gathers come with one or kRegCols values per lane, every other mode with kRegCols
+template <typename T, int TEAM, int AUX, int MAIN>
+static hipError_t launch_rc(int rc, const SubArgs<T>& a, int num_cus, int per_cu, hipStream_t st)
+{
+    if constexpr (AUX == AUX_GATHER || AUX == AUX_GATHER_G || AUX == AUX_GATHER_T) {
+        if (rc == 1) return launch_one<T, TEAM, AUX, MAIN, 1>(a, num_cus, per_cu, st);
+    }
+    if (rc != kRegCols) return hipErrorInvalidValue;
+    return launch_one<T, TEAM, AUX, MAIN, kRegCols>(a, num_cus, per_cu, st);
+}
+
 template <typename T, int TEAM>
-static hipError_t launch_team(int aux, int main_mode, const SubArgs<T>& a, int num_cus, int per_cu, hipStream_t st)
+static hipError_t launch_team(int aux, int main_mode, int rc, const SubArgs<T>& a, int num_cus, int per_cu,
+                              hipStream_t st)
 {
     if (main_mode == MAIN_LEAF) {
         switch (aux) {
-        case AUX_GATHER: return launch_one<T, TEAM, AUX_GATHER, MAIN_LEAF>(a, num_cus, per_cu, st);
-        case AUX_ROWS: return launch_one<T, TEAM, AUX_ROWS, MAIN_LEAF>(a, num_cus, per_cu, st);
-        case AUX_STAGED: return launch_one<T, TEAM, AUX_STAGED, MAIN_LEAF>(a, num_cus, per_cu, st);
-        case AUX_GATHER_G: return launch_one<T, TEAM, AUX_GATHER_G, MAIN_LEAF>(a, num_cus, per_cu, st);
-        case AUX_GATHER_T: return launch_one<T, TEAM, AUX_GATHER_T, MAIN_LEAF>(a, num_cus, per_cu, st);
-        case AUX_COLORS: return launch_one<T, TEAM, AUX_COLORS, MAIN_LEAF>(a, num_cus, per_cu, st);
-        case AUX_READ: return launch_one<T, TEAM, AUX_READ, MAIN_LEAF>(a, num_cus, per_cu, st);
-        case AUX_PARTIALS: return launch_one<T, TEAM, AUX_PARTIALS, MAIN_LEAF>(a, num_cus, per_cu, st);
+        case AUX_GATHER: return launch_rc<T, TEAM, AUX_GATHER, MAIN_LEAF>(rc, a, num_cus, per_cu, st);
+        case AUX_ROWS: return launch_rc<T, TEAM, AUX_ROWS, MAIN_LEAF>(rc, a, num_cus, per_cu, st);
+        case AUX_STAGED: return launch_rc<T, TEAM, AUX_STAGED, MAIN_LEAF>(rc, a, num_cus, per_cu, st);
+        case AUX_GATHER_G: return launch_rc<T, TEAM, AUX_GATHER_G, MAIN_LEAF>(rc, a, num_cus, per_cu, st);
+        case AUX_GATHER_T: return launch_rc<T, TEAM, AUX_GATHER_T, MAIN_LEAF>(rc, a, num_cus, per_cu, st);
+        case AUX_COLORS: return launch_rc<T, TEAM, AUX_COLORS, MAIN_LEAF>(rc, a, num_cus, per_cu, st);
+        case AUX_READ: return launch_rc<T, TEAM, AUX_READ, MAIN_LEAF>(rc, a, num_cus, per_cu, st);
+        case AUX_PARTIALS: return launch_rc<T, TEAM, AUX_PARTIALS, MAIN_LEAF>(rc, a, num_cus, per_cu, st);
         default: return hipErrorInvalidValue;
         }
     }
     switch (aux) {
-    case AUX_GATHER: return launch_one<T, TEAM, AUX_GATHER, MAIN_TABLE>(a, num_cus, per_cu, st);
-    case AUX_ROWS: return launch_one<T, TEAM, AUX_ROWS, MAIN_TABLE>(a, num_cus, per_cu, st);
-    case AUX_STAGED: return launch_one<T, TEAM, AUX_STAGED, MAIN_TABLE>(a, num_cus, per_cu, st);
-    case AUX_READ: return launch_one<T, TEAM, AUX_READ, MAIN_TABLE>(a, num_cus, per_cu, st);
-    case AUX_PARTIALS: return launch_one<T, TEAM, AUX_PARTIALS, MAIN_TABLE>(a, num_cus, per_cu, st);
+    case AUX_GATHER: return launch_rc<T, TEAM, AUX_GATHER, MAIN_TABLE>(rc, a, num_cus, per_cu, st);
+    case AUX_ROWS: return launch_rc<T, TEAM, AUX_ROWS, MAIN_TABLE>(rc, a, num_cus, per_cu, st);
+    case AUX_STAGED: return launch_rc<T, TEAM, AUX_STAGED, MAIN_TABLE>(rc, a, num_cus, per_cu, st);
+    case AUX_READ: return launch_rc<T, TEAM, AUX_READ, MAIN_TABLE>(rc, a, num_cus, per_cu, st);
+    case AUX_PARTIALS: return launch_rc<T, TEAM, AUX_PARTIALS, MAIN_TABLE>(rc, a, num_cus, per_cu, st);
     default: return hipErrorInvalidValue;
     }
 }
 
 template <typename T>
-hipError_t launch_sub(int team, int aux, int main_mode, const SubArgs<T>& a, int num_cus, int blocks_per_cu,
+hipError_t launch_sub(int team, int aux, int main_mode, int rc, const SubArgs<T>& a, int num_cus, int blocks_per_cu,
                       hipStream_t st)
 {
     if (a.n_items <= 0) return hipSuccess;
     if (a.ng < 1 || team % a.ng != 0 || (a.ng & (a.ng - 1)) != 0) return hipErrorInvalidValue;
     if (uses_rank(aux) && (a.K < 1 || a.K > kMaxK - 1)) return hipErrorInvalidValue;
     switch (team) {
-    case 8: return launch_team<T, 8>(aux, main_mode, a, num_cus, blocks_per_cu, st);
-    case 16: return launch_team<T, 16>(aux, main_mode, a, num_cus, blocks_per_cu, st);
-    case 32: return launch_team<T, 32>(aux, main_mode, a, num_cus, blocks_per_cu, st);
-    case 64: return launch_team<T, 64>(aux, main_mode, a, num_cus, blocks_per_cu, st);
+    case 8: return launch_team<T, 8>(aux, main_mode, rc, a, num_cus, blocks_per_cu, st);
+    case 16: return launch_team<T, 16>(aux, main_mode, rc, a, num_cus, blocks_per_cu, st);
+    case 32: return launch_team<T, 32>(aux, main_mode, rc, a, num_cus, blocks_per_cu, st);
+    case 64: return launch_team<T, 64>(aux, main_mode, rc, a, num_cus, blocks_per_cu, st);
     default: return hipErrorInvalidValue;
     }
 }
 
-template hipError_t launch_sub<float>(int, int, int, const SubArgs<float>&, int, int, hipStream_t);
-template hipError_t launch_sub<double>(int, int, int, const SubArgs<double>&, int, int, hipStream_t);
+template hipError_t launch_sub<float>(int, int, int, int, const SubArgs<float>&, int, int, hipStream_t);
+template hipError_t launch_sub<double>(int, int, int, int, const SubArgs<double>&, int, int, hipStream_t);
 
 // ------------------------------------------------------------ utilities ---
 

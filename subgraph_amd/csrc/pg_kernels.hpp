@@ -124,12 +124,13 @@ __host__ __device__ constexpr int stage_group_stride(int segpad, int cl)
 template <typename T>
 size_t sub_lds_bytes(int team, int aux, int main_mode, const SubArgs<T>& a);
 
-// Launches k_sub<T, team, aux, main> as a persistent grid of
+// Launches k_sub<T, team, aux, main, rc> as a persistent grid of
 // blocks_per_cu x num_cus workgroups (blocks_per_cu = 0: the occupancy
-// query's resident count), never more than there are items; returns
+// query's resident count), never more than there are items; rc = values per
+// lane (1 for gathers whose row fits one chunk, else kRegCols); returns
 // hipErrorInvalidValue for an unsupported combination.
 template <typename T>
-hipError_t launch_sub(int team, int aux, int main_mode, const SubArgs<T>& a, int num_cus, int blocks_per_cu,
+hipError_t launch_sub(int team, int aux, int main_mode, int rc, const SubArgs<T>& a, int num_cus, int blocks_per_cu,
                       hipStream_t st);
 
 // Per-pass values read from device memory, so that one captured graph serves

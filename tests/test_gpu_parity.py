@@ -102,24 +102,28 @@ def test_random_graphs_and_trees(gpu_ctx, hub):
         gpu_ctx.set_option("hub_chunk", saved[1])
 
 
-VARIANTS = [  # (aux_rows, stage_rows): every neighbour-sum kernel the library can pick
-    (0, 0),   # direct gathers through the gather index table
-    (0, 1),   # gathers from rows staged in LDS
-    (1, 0),   # row-wise sums with LDS atomics
-    (-1, 1),  # per-subtemplate choice
+VARIANTS = [  # (aux_rows, stage_rows, reg_cols): every neighbour-sum kernel the library can pick
+    (0, 0, 0),   # direct gathers, one value per lane for the root's single value
+    (0, 0, 1),   # direct gathers, one value per lane
+    (0, 0, 4),   # direct gathers, four values per lane
+    (0, 1, 4),   # gathers from rows staged in LDS
+    (1, 0, 0),   # row-wise sums with LDS atomics
+    (-1, 1, 4),  # per-subtemplate choice
 ]
 
 
-@pytest.mark.parametrize("aux_rows,stage_rows", VARIANTS)
+@pytest.mark.parametrize("aux_rows,stage_rows,reg_cols", VARIANTS)
 @pytest.mark.parametrize("B", [1, 5, 16])
-def test_kernel_variants_match_oracle(gpu_ctx, aux_rows, stage_rows, B):
+def test_kernel_variants_match_oracle(gpu_ctx, aux_rows, stage_rows, reg_cols, B):
     """Every neighbour-sum variant, at several batch widths (segments of
     one or several colorings per chunk), against the oracle: FP64 exact,
     FP32 within the tolerance; the hub split path included."""
-    rng = np.random.default_rng(500 + 10 * B + 3 * aux_rows + stage_rows)
-    saved = [gpu_ctx.get_option(k) for k in ("aux_rows", "stage_rows", "hub_degree", "hub_chunk")]
+    rng = np.random.default_rng(500 + 10 * B + 3 * aux_rows + stage_rows + reg_cols)
+    keys = ("aux_rows", "stage_rows", "reg_cols", "hub_degree", "hub_chunk")
+    saved = [gpu_ctx.get_option(k) for k in keys]
     gpu_ctx.set_option("aux_rows", aux_rows)
     gpu_ctx.set_option("stage_rows", stage_rows)
+    gpu_ctx.set_option("reg_cols", reg_cols)
     gpu_ctx.set_option("hub_degree", 24)
     gpu_ctx.set_option("hub_chunk", 24)
     try:
@@ -130,8 +134,8 @@ def test_kernel_variants_match_oracle(gpu_ctx, aux_rows, stage_rows, B):
             s, d = _simple_graph(rng, n, int(rng.integers(4 * n, 10 * n)))
             colptr, rowidx = sg.csc_from_edges(n, s, d)
             gpu_ctx.upload_graph(colptr, rowidx)
-            print(f"[variants] aux_rows={aux_rows} stage_rows={stage_rows} B={B} k={k} n={n} nnz={len(rowidx)}",
-                  flush=True)
+            print(f"[variants] aux_rows={aux_rows} stage_rows={stage_rows} reg_cols={reg_cols} B={B} k={k} n={n} "
+                  f"nnz={len(rowidx)}", flush=True)
             plan = gpu_ctx.plan(k, te)
             op = oracle.Plan(k, te)
             cols = rng.integers(0, k, size=(B, n)).astype(np.uint8)
@@ -144,7 +148,7 @@ def test_kernel_variants_match_oracle(gpu_ctx, aux_rows, stage_rows, B):
                     o32 = oracle.count_f32(op, colptr, rowidx, cols[i])
                     assert _rel(g32[i], o32) <= TOL32, (trial, i, g32[i], o32)
     finally:
-        for key, v in zip(("aux_rows", "stage_rows", "hub_degree", "hub_chunk"), saved):
+        for key, v in zip(keys, saved):
             gpu_ctx.set_option(key, v)
 
 
@@ -153,7 +157,7 @@ def test_autotune_keeps_results(gpu_ctx):
     within the tolerance), the chosen settings are among the candidates,
     and the second call of the same shape reuses them."""
     rng = np.random.default_rng(77)
-    keys = ("autotune", "stage_rows", "aux_rows", "bucket_high", "bucket_mid", "hub_degree", "hub_chunk")
+    keys = ("autotune", "reg_cols", "stage_rows", "aux_rows", "bucket_high", "bucket_mid", "hub_degree", "hub_chunk")
     saved = [gpu_ctx.get_option(k) for k in keys]
     s, d = sg.rmat_edges(12, 8, seed=4)
     n = 1 << 12
@@ -171,7 +175,7 @@ def test_autotune_keeps_results(gpu_ctx):
         chosen32 = {k: gpu_ctx.get_option(k) for k in keys[1:]}
         print(f"[autotune] chosen fp64 {chosen64} fp32 {chosen32}", flush=True)
         for chosen in (chosen64, chosen32):
-            assert chosen["stage_rows"] in (0, 1) and chosen["aux_rows"] in (0, -1)
+            assert chosen["stage_rows"] in (0, 1) and chosen["aux_rows"] in (0, -1) and chosen["reg_cols"] in (0, 1)
             assert chosen["bucket_high"] in (16, 32, 64) and chosen["bucket_mid"] in (4, 8)
             assert chosen["hub_degree"] in (128, 256, 1024) and chosen["hub_chunk"] == chosen["hub_degree"]
         for i in range(4):
