@@ -347,6 +347,26 @@ def main():
                      "launches": klaunch, "avg_launch_ms": kms / klaunch,
                      "algorithmic_bytes_per_launch": kbytes / klaunch},
     }
+    # the reference's cost models (CountMat::estimate*, GB/Gflop = 2^30) over
+    # the measured time per coloring, beside this library's own byte model
+    est = sg.estimate_model(n, len(rowidx), 7, TREE7)
+    fp = ctx.footprint(plan, 1)
+    sec = elapsed / args.steps
+    gib = float(1 << 30)
+    result["estimates"] = {
+        "reference_model": {k: est[k] for k in ("peak_mem_gb", "pgbsc_spmv_gbytes", "pgbsc_fma_gbytes",
+                                                "pgbsc_spmv_gflop", "pgbsc_fma_gflop")},
+        "reference_model_gbs": (est["pgbsc_spmv_gbytes"] + est["pgbsc_fma_gbytes"]) * gib / sec / 1e9,
+        "reference_model_gflops": (est["pgbsc_spmv_gflop"] + est["pgbsc_fma_gflop"]) * gib / sec / 1e9,
+        "algorithmic_bytes_per_coloring": fp["algorithmic_bytes_per_coloring"],
+        "algorithmic_gbs": fp["algorithmic_bytes_per_coloring"] / sec / 1e9,
+        "device_bytes_one_coloring_pass": fp["device_bytes"],
+        "note": "reference_model_*: the reference's PGBSC SpMV + eMA byte and flop models per coloring over the "
+                "measured ms_per_step (decimal GB/s and Gflop/s); algorithmic_*: each table written and read "
+                "once, CSC once per gathering launch"}
+    log(f"models: reference PGBSC {result['estimates']['reference_model_gbs']:.0f} GB/s, "
+        f"{result['estimates']['reference_model_gflops']:.0f} Gflop/s; algorithmic "
+        f"{result['estimates']['algorithmic_gbs']:.0f} GB/s over the whole pass")
 
     if not args.no_batched:
         B = 64

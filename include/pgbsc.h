@@ -121,6 +121,34 @@ int pg_template_tables(int nv, int ne, const int32_t* src, const int32_t* dst, i
                        int32_t* main_idx, int32_t* aux_idx, int32_t* comb_to_count,
                        int32_t* split_main, int32_t* split_aux);
 
+/* ---- analytic estimators (replace CountMat::estimatePeakMemUsage,
+ * estimateFlops{PGBSC,Fascia,PrunedFascia}, estimateMemComm{PGBSC,Fascia,
+ * PrunedFascia} and estimateTemplate, CountMat.cpp:1067-1446) ----
+ * The reference's models, in its units: "GB", "GBytes" and "Gflop" are
+ * 2^30, and the estimateTemplate figures are unscaled.  Inputs are the
+ * graph's vertex count (holes removed, as the reference's CSC holds it) and
+ * directed nnz, and the subtemplate chain as pg_plan_create takes it
+ * (sub_size, main_idx, aux_idx; leaves -1).  Host only. */
+typedef struct pg_estimate {
+    double peak_mem_gb;                            /* estimatePeakMemUsage */
+    double pgbsc_spmv_gflop, pgbsc_fma_gflop;      /* estimateFlopsPGBSC */
+    double pgbsc_spmv_gbytes, pgbsc_fma_gbytes;    /* estimateMemCommPGBSC */
+    double fascia_gflop, fascia_gbytes;            /* estimate{Flops,MemComm}Fascia */
+    double pruned_fascia_gflop, pruned_fascia_gbytes; /* estimate{Flops,MemComm}PrunedFascia */
+    double workload_n, workload_nnz;               /* estimateTemplate */
+    double memload_n, memload_nnz;
+} pg_estimate;
+int pg_estimate_model(int64_t n, int64_t nnz, int k, int nsub, const int32_t* sub_size,
+                      const int32_t* main_idx, const int32_t* aux_idx, pg_estimate* out);
+/* This library's own figures for a pass of `colorings_per_pass` colorings of
+ * `plan` on the uploaded graph (no reference counterpart; the device side of
+ * estimatePeakMemUsage): device bytes of the pass's memory plan (count
+ * tables placed over their lifetimes, per-pass buffers, graph), and the
+ * algorithmic HBM bytes per coloring the roofline figures use (each table
+ * written once and read once, the CSC streamed once per gathering launch). */
+int pg_plan_footprint(pg_ctx* ctx, const pg_plan* plan, int colorings_per_pass, int precision,
+                      int64_t* device_bytes, double* algorithmic_bytes_per_coloring);
+
 /* ---- colorings (replaces CountMat::colorInit, CountMat.cpp:979-991) ----
  * Color of vertex v (caller numbering) in coloring c of stream `seed`:
  *   h = splitmix64(seed ^ splitmix64(c * 0x9E3779B97F4A7C15 + v));

@@ -16,6 +16,7 @@ from __future__ import annotations
 import ctypes as C
 import os
 import subprocess
+import time
 
 import numpy as np
 
@@ -195,6 +196,56 @@ def run_ref(graph_path: str, tmpl_path: str, colors_path: str, ncol: int, thread
             raws.append(float(f[4]))
             secs.append(float(f[6]))
     return raws, secs
+
+
+def estimate_ref(graph_path: str, tmpl_path: str, workdir: str) -> dict:
+    """The reference's cost models (CountMat.cpp:1067-1400) for a graph /
+    template pair: the figures at full precision, the printed peak-memory
+    line's value, and degDistri.data as [[degree, count], ...]."""
+    out = subprocess.run([REF_BIN, "estimate", graph_path, tmpl_path], check=True, cwd=workdir,
+                         capture_output=True, text=True, env=dict(os.environ, OMP_NUM_THREADS="1")).stdout
+    res: dict = {}
+    for line in out.splitlines():
+        if line.startswith("PGREF est "):
+            f = line.split()
+            res[f[2]] = float(f[3])
+        elif line.startswith("PGREF n "):
+            f = line.split()
+            res["n"], res["nnz"] = int(f[2]), int(f[4])
+        elif line.startswith("Peak memory usage estimated : "):
+            res["peak_mem_gb_printed"] = line.split(":")[1].split()[0]
+        elif line.startswith("maxDeg: "):
+            res["max_deg"] = int(line.split()[1])
+    with open(os.path.join(workdir, "degDistri.data")) as f:
+        res["deg_distribution"] = [[int(a) for a in ln.split()] for ln in f if ln.strip()]
+    return res
+
+
+def estimate_template_ref(graph_path: str, tmpl_path: str, timeout: float = 60.0) -> dict:
+    """estimateTemplate's two printed lines (CountMat.cpp:1448-1496).  The
+    function lacks a return statement and, built at -O3, does not return:
+    the child is ended once both lines have been read."""
+    proc = subprocess.Popen([REF_BIN, "estimate_template", graph_path, tmpl_path], stdout=subprocess.PIPE,
+                            stderr=subprocess.DEVNULL, text=True, env=dict(os.environ, OMP_NUM_THREADS="1"))
+    res: dict = {}
+    deadline = time.monotonic() + timeout
+    try:
+        for line in proc.stdout:
+            if line.startswith("Workload N: "):
+                a, b = line.split(":", 1)[1].split(",")
+                res["workload_n"], res["workload_nnz"] = float(a), float(b.split(":")[1])
+            elif line.startswith("Memload N: "):
+                a, b = line.split(":", 1)[1].split(",")
+                res["memload_n"], res["memload_nnz"] = float(a), float(b.split(":")[1])
+                break
+            if time.monotonic() > deadline:
+                break
+    finally:
+        proc.kill()
+        proc.wait()
+    if len(res) != 4:
+        raise RuntimeError("estimateTemplate printed no figures")
+    return res
 
 
 def dump_ref(graph_path: str, tmpl_path: str) -> dict:

@@ -31,6 +31,19 @@
 //   its subtemplate chain and index tables ("PGREF sub s size main aux",
 //   "PGREF c2c s ...", "PGREF smain s ...", "PGREF saux s ...") with leaves'
 //   child links as -1.
+//
+//        pg_ref estimate <graph.el> <template.fascia>
+//   runs the reference's cost models on the compute(t, true) setup:
+//   estimatePeakMemUsage (its own "Peak memory usage estimated" line),
+//   estimate{Flops,MemComm}{PGBSC,Fascia,PrunedFascia}, then
+//   "PGREF est <name> <%.17g>" per figure; degreeDistribution writes
+//   degDistri.data in the working directory.
+//
+//        pg_ref estimate_template <graph.el> <template.fascia>
+//   runs estimateTemplate (CountMat.cpp:1448-1496) on the same setup.  That
+//   function has no return statement; built at -O3 it runs on past its two
+//   lines ("Workload N: ...", "Memload N: ...") and does not return, so the
+//   caller reads those two lines and then ends the process.
 
 #include <cstdint>
 #include <cstdio>
@@ -140,10 +153,67 @@ static int dump(const std::string& graph_name, const std::string& tmpl_name)
     return 0;
 }
 
+static int estimate(const std::string& graph_name, const std::string& tmpl_name)
+{
+    EdgeList elist(graph_name);
+    CSCGraph<int32_t, float>* g = new CSCGraph<int32_t, float>();
+    g->createFromEdgeListFile(elist.getNumVertices(), elist.getNumEdges(), elist.getSrcList(), elist.getDstList(), false);
+    g->splitCSC(4);
+    Graph tmpl;
+    tmpl.read_enlist(tmpl_name);
+    CountMat ex;
+    ex.initialization(nullptr, g, 1, 1, 1, 1, -1, false);
+    ex.compute(tmpl, true);
+    printf("PGREF n %d nnz %lld\n", g->getNumVertices(), (long long)g->getNNZ());
+    fflush(stdout);
+    ex.estimatePeakMemUsage();
+    const double pgbsc_flop = ex.estimateFlopsPGBSC();
+    const double pgbsc_bytes = ex.estimateMemCommPGBSC();
+    const double fascia_flop = ex.estimateFlopsFascia();
+    const double fascia_bytes = ex.estimateMemCommFascia();
+    const double pruned_flop = ex.estimateFlopsPrunedFascia();
+    const double pruned_bytes = ex.estimateMemCommPrunedFascia();
+    printf("PGREF est pgbsc_spmv_gflop %.17g\n", ex._spmvFlops);
+    printf("PGREF est pgbsc_fma_gflop %.17g\n", ex._fmaFlops);
+    printf("PGREF est pgbsc_spmv_gbytes %.17g\n", ex._spmvMemBytes);
+    printf("PGREF est pgbsc_fma_gbytes %.17g\n", ex._fmaMemBytes);
+    printf("PGREF est pgbsc_gflop %.17g\n", pgbsc_flop);
+    printf("PGREF est pgbsc_gbytes %.17g\n", pgbsc_bytes);
+    printf("PGREF est fascia_gflop %.17g\n", fascia_flop);
+    printf("PGREF est fascia_gbytes %.17g\n", fascia_bytes);
+    printf("PGREF est pruned_fascia_gflop %.17g\n", pruned_flop);
+    printf("PGREF est pruned_fascia_gbytes %.17g\n", pruned_bytes);
+    fflush(stdout);
+    ex.degreeDistribution();
+    fflush(stdout);
+    return 0;
+}
+
+static int estimate_template(const std::string& graph_name, const std::string& tmpl_name)
+{
+    EdgeList elist(graph_name);
+    CSCGraph<int32_t, float>* g = new CSCGraph<int32_t, float>();
+    g->createFromEdgeListFile(elist.getNumVertices(), elist.getNumEdges(), elist.getSrcList(), elist.getDstList(), false);
+    g->splitCSC(4);
+    Graph tmpl;
+    tmpl.read_enlist(tmpl_name);
+    CountMat ex;
+    ex.initialization(nullptr, g, 1, 1, 1, 1, -1, false);
+    ex.compute(tmpl, true);
+    fflush(stdout);
+    ex.estimateTemplate();
+    fflush(stdout);
+    return 0;
+}
+
 int main(int argc, char** argv)
 {
     if (argc == 4 && std::string(argv[1]) == "dump")
         return dump(argv[2], argv[3]);
+    if (argc == 4 && std::string(argv[1]) == "estimate")
+        return estimate(argv[2], argv[3]);
+    if (argc == 4 && std::string(argv[1]) == "estimate_template")
+        return estimate_template(argv[2], argv[3]);
     if (argc < 6) {
         fprintf(stderr, "usage: %s graph.el template.fascia colors.u8|- ncolorings threads [seed]\n", argv[0]);
         return 2;

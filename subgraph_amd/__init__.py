@@ -17,10 +17,11 @@ import math
 
 import numpy as np
 
-from ._lib import (PG_FLAG_FIX_2VERTEX, PG_FP32, PG_FP64, PGError, exported_symbols, lib)
+from ._lib import (ESTIMATE_FIELDS, PG_FLAG_FIX_2VERTEX, PG_FP32, PG_FP64, PGError, PgEstimate,
+                   exported_symbols, lib)
 
 __all__ = ["Context", "Plan", "CountMat", "PGError", "rmat_edges", "csc_from_edges", "colors_fill",
-           "remove_holes", "exported_symbols", "PG_FP32", "PG_FP64"]
+           "remove_holes", "exported_symbols", "estimate_model", "PG_FP32", "PG_FP64"]
 
 _PREC = {"fp32": PG_FP32, "fp64": PG_FP64, PG_FP32: PG_FP32, PG_FP64: PG_FP64}
 
@@ -86,6 +87,20 @@ def template_tables(nv: int, edges) -> dict:
     out["split_main"] = out["split_main"][: nsplit.value]
     out["split_aux"] = out["split_aux"][: nsplit.value]
     return out
+
+
+def estimate_model(n: int, nnz: int, nv: int, edges) -> dict:
+    """The reference's cost models (CountMat::estimatePeakMemUsage,
+    estimateFlops*/estimateMemComm* and estimateTemplate, CountMat.cpp:
+    1067-1496) for a graph of n vertices and nnz directed entries and a
+    template tree: GB / GBytes / Gflop are 2^30.  Host only."""
+    t = template_tables(nv, edges)
+    e = PgEstimate()
+    rc = lib().pg_estimate_model(int(n), int(nnz), int(nv), len(t["sub_size"]), t["sub_size"], t["main_idx"],
+                                 t["aux_idx"], C.byref(e))
+    if rc:
+        raise PGError(rc, "pg_estimate_model")
+    return {f: getattr(e, f) for f in ESTIMATE_FIELDS}
 
 
 def colors_fill(seed: int, first: int, ncolorings: int, n: int, k: int) -> np.ndarray:
@@ -160,6 +175,16 @@ class Context:
         else:
             vals = np.ascontiguousarray(vals, np.float32)
             self._check(lib().pg_graph_set_weights(self.ptr, vals.ctypes.data, 0), "pg_graph_set_weights")
+
+    def footprint(self, plan: Plan, colorings_per_pass: int = 1, precision="fp32") -> dict:
+        """Device bytes of a pass of `colorings_per_pass` colorings (memory
+        plan + graph) and the algorithmic HBM bytes per coloring that the
+        roofline figures use."""
+        dev = C.c_int64()
+        alg = C.c_double()
+        self._check(lib().pg_plan_footprint(self.ptr, plan.ptr, int(colorings_per_pass), _PREC[precision],
+                                            C.byref(dev), C.byref(alg)), "pg_plan_footprint")
+        return {"device_bytes": dev.value, "algorithmic_bytes_per_coloring": alg.value}
 
     def stats(self) -> dict:
         v = [C.c_int64() for _ in range(4)]
@@ -274,10 +299,12 @@ class CountMat:
                        device: int = 0, precision="fp32", seed: int = 0):
         self.ctx = Context(device)
         self.ctx.upload_graph(colptr, rowidx)
+        self.n, self.nnz = len(colptr) - 1, len(rowidx)
         self.itr_num, self.precision, self.seed = itr_num, precision, seed
         self.calculate_automorphisms = calculate_automorphisms
 
     def compute(self, nv: int, edges, isEstimate: bool = False) -> float:
+        self.template = (nv, edges)
         plan = self.ctx.plan(nv, edges)
         if isEstimate:
             return 0.0
@@ -286,6 +313,11 @@ class CountMat:
         prob = math.factorial(k) / float(k) ** k  # CountMat.cpp:248-249
         aut = automorphisms(nv, edges) if self.calculate_automorphisms else 1
         return math.floor(float(np.mean(self.raw)) / (prob * aut) + 0.5)
+
+    def estimates(self) -> dict:
+        """Every figure of the reference's estimate* models (CountMat.cpp:
+        1067-1496) for the graph and the template of the last compute()."""
+        return estimate_model(self.n, self.nnz, *self.template)
 
 
 def automorphisms(nv: int, edges) -> int:

@@ -30,6 +30,16 @@ _vp = C.c_void_p
 _lib = None
 
 
+ESTIMATE_FIELDS = ("peak_mem_gb", "pgbsc_spmv_gflop", "pgbsc_fma_gflop", "pgbsc_spmv_gbytes", "pgbsc_fma_gbytes",
+                   "fascia_gflop", "fascia_gbytes", "pruned_fascia_gflop", "pruned_fascia_gbytes",
+                   "workload_n", "workload_nnz", "memload_n", "memload_nnz")
+
+
+class PgEstimate(C.Structure):
+    """struct pg_estimate (pgbsc.h): thirteen doubles in declaration order."""
+    _fields_ = [(f, C.c_double) for f in ESTIMATE_FIELDS]
+
+
 class PGError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"{msg} [{ERRORS.get(code, code)}]")
@@ -94,6 +104,10 @@ def lib() -> C.CDLL:
         "pg_template_sizes": (C.c_int, [C.c_int, C.c_int, _i32p, _i32p, C.POINTER(C.c_int32),
                                         C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
         "pg_template_tables": (C.c_int, [C.c_int, C.c_int, _i32p, _i32p] + [_i32p] * 6),
+        "pg_estimate_model": (C.c_int, [C.c_int64, C.c_int64, C.c_int, C.c_int, _i32p, _i32p, _i32p,
+                                        C.POINTER(PgEstimate)]),
+        "pg_plan_footprint": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.POINTER(C.c_int64),
+                                        C.POINTER(C.c_double)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -110,4 +124,4 @@ def exported_symbols() -> list[str]:
             "pg_plan_from_edges", "pg_plan_destroy", "pg_plan_info", "pg_colors_fill", "pg_count",
             "pg_count_multi", "pg_count_grad", "pg_last_timing", "pg_last_timing_kernel",
             "pg_last_timing_entry", "pg_rmat_edges", "pg_csc_from_edges", "pg_edges_remove_holes",
-            "pg_template_sizes", "pg_template_tables"]
+            "pg_template_sizes", "pg_template_tables", "pg_estimate_model", "pg_plan_footprint"]

@@ -1,10 +1,12 @@
 // CountMat driver over the libpgbsc C ABI (reference: CountMat.cpp:21-266,
-// 979-999, 1498-1572).
+// 979-999, 1067-1572).
 #include "countmat.hpp"
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <fstream>
 #include <functional>
 
 namespace pgbsc {
@@ -23,6 +25,7 @@ int CountMat::initialization(const CSCGraph* g, int thd_num, int itr_num, int is
     }
     (void)isPruned;
     (void)useSPMM;
+    graph_ = g;
     itr_ = itr_num;
     thd_ = thd_num;
     autom_ = calculate_automorphisms;
@@ -49,7 +52,7 @@ double CountMat::compute(const Graph& t, bool isEstimate)
         err_ = "compute before initialization";
         return PG_ERR_STATE;
     }
-    FlatPlan fp;
+    FlatPlan& fp = fp_;
     int rc = flatten_plan(t, &fp, &err_);
     if (rc) return rc;
     pg_plan* plan = nullptr;
@@ -59,14 +62,36 @@ double CountMat::compute(const Graph& t, bool isEstimate)
         err_ = pg_last_error(ctx_);
         return rc;
     }
+    pg_estimate est{};
+    if (verbose_) {
+        // CountMat.cpp:132-162
+        estimatePeakMemUsage();
+        int64_t dev_bytes = 0;
+        double alg = 0;
+        if (pg_plan_footprint(ctx_, plan, 1, precision_, &dev_bytes, &alg) == PG_OK)
+            printf("Device memory of a one-coloring pass : %f GB; algorithmic HBM bytes per coloring : %f GB\n",
+                   dev_bytes / 1073741824.0, alg / 1073741824.0);
+        const double flops = estimateFlopsPGBSC(), bytes = estimateMemCommPGBSC();
+        printf("PGBSC Arith Intensity: %f\n", flops / bytes);
+        const double ff = estimateFlopsFascia(), fb = estimateMemCommFascia();
+        printf("Fascia Arith Intensity: %f\n", ff / fb);
+        const double pf = estimateFlopsPrunedFascia(), pb = estimateMemCommPrunedFascia();
+        printf("Pruned Fascia Arith Intensity: %f\n", pf / pb);
+        degreeDistribution();
+        estimateTemplate();
+        fflush(stdout);
+        model(&est);
+    }
     if (isEstimate) {
         pg_plan_destroy(plan);
         return 0.0;
     }
     raw_.assign(itr_, 0.0);
+    if (verbose_) pg_set_option(ctx_, "timing", 1);
     auto t0 = std::chrono::steady_clock::now();
     rc = pg_count(ctx_, plan, itr_, nullptr, seed_, precision_, 0u, raw_.data());
     auto t1 = std::chrono::steady_clock::now();
+    if (verbose_) pg_set_option(ctx_, "timing", 0);
     pg_plan_destroy(plan);
     if (rc) {
         err_ = pg_last_error(ctx_);
@@ -74,6 +99,28 @@ double CountMat::compute(const Graph& t, bool isEstimate)
     }
     sec_per_col_ = std::chrono::duration<double>(t1 - t0).count() / itr_;
     printf("\nTime for count per iter: %9.6lf seconds\n", sec_per_col_);
+    if (verbose_) {
+        // CountMat.cpp:216-242.  SpMM and eMA run fused in one kernel, so the
+        // model's two parts are reported over the one device time.
+        double total_ms = 0;
+        pg_last_timing(ctx_, &total_ms, nullptr, nullptr, nullptr);
+        const double sec = total_ms * 1e-3;
+        printf("Device kernel time per iter : %f seconds\n", sec / itr_);
+        char name[128];
+        double ms = 0, kb = 0;
+        int64_t nl = 0;
+        const int nk = pg_last_timing_entry(ctx_, 0, nullptr, 0, nullptr, nullptr, nullptr);
+        for (int i = 0; i < nk; ++i)
+            if (pg_last_timing_entry(ctx_, i, name, (int)sizeof name, &ms, &nl, &kb) >= 0)
+                printf("Kernel %s : %f ms in %lld launches, %f GB/s algorithmic\n", name, ms, (long long)nl,
+                       ms > 0 ? kb / (ms * 1e-3) / 1e9 : 0.0);
+        if (sec > 0) {
+            printf("Total Memory bandwidth is : %f GBytes per second\n",
+                   ((est.pgbsc_spmv_gbytes + est.pgbsc_fma_gbytes) * itr_) / sec);
+            printf("Total Throughput is : %f Gflops per second\n",
+                   ((est.pgbsc_spmv_gflop + est.pgbsc_fma_gflop) * itr_) / sec);
+        }
+    }
     double mean = 0;
     for (double r : raw_) mean += r;
     mean /= itr_;
@@ -118,6 +165,111 @@ int CountMat::automorphismNum() const
     };
     rec(0);
     return count;
+}
+
+bool CountMat::model(pg_estimate* e)
+{
+    if (!graph_ || fp_.nsub == 0) {
+        err_ = "the estimators need initialization() and compute() first";
+        return false;
+    }
+    if (pg_estimate_model(graph_->getNumVertices(), graph_->getNNZ(), fp_.k, fp_.nsub, fp_.sub_size.data(),
+                          fp_.main_idx.data(), fp_.aux_idx.data(), e) != PG_OK) {
+        err_ = "invalid subtemplate chain";
+        return false;
+    }
+    return true;
+}
+
+void CountMat::estimatePeakMemUsage()
+{
+    pg_estimate e;
+    if (!model(&e)) return;
+    printf("Peak memory usage estimated : %f GB \n", e.peak_mem_gb);
+    fflush(stdout);
+}
+
+double CountMat::estimateFlopsPGBSC()
+{
+    pg_estimate e;
+    if (!model(&e)) return 0.0;
+    printf("|V| is: %d, |E| nnz is: %lld \n", graph_->getNumVertices(), (long long)graph_->getNNZ());
+    printf("Flops PGBSC estimated : SpMV %f Gflop, FMA %f Gflop \n", e.pgbsc_spmv_gflop, e.pgbsc_fma_gflop);
+    fflush(stdout);
+    return e.pgbsc_spmv_gflop + e.pgbsc_fma_gflop;
+}
+
+double CountMat::estimateMemCommPGBSC()
+{
+    pg_estimate e;
+    if (!model(&e)) return 0.0;
+    printf("Comm Bytes estimated PGBSC : SpMV %f GBytes, FMA %f GBytes \n", e.pgbsc_spmv_gbytes, e.pgbsc_fma_gbytes);
+    fflush(stdout);
+    return e.pgbsc_spmv_gbytes + e.pgbsc_fma_gbytes;
+}
+
+double CountMat::estimateFlopsFascia()
+{
+    pg_estimate e;
+    if (!model(&e)) return 0.0;
+    printf("|V| is: %d, |E| nnz is: %lld \n", graph_->getNumVertices(), (long long)graph_->getNNZ());
+    printf("Flops estimated fascia: %f Gflop\n", e.fascia_gflop);
+    fflush(stdout);
+    return e.fascia_gflop;
+}
+
+double CountMat::estimateMemCommFascia()
+{
+    pg_estimate e;
+    if (!model(&e)) return 0.0;
+    printf("Comm Bytes Fascia estimated:  %f GBytes\n", e.fascia_gbytes);
+    fflush(stdout);
+    return e.fascia_gbytes;
+}
+
+double CountMat::estimateFlopsPrunedFascia()
+{
+    pg_estimate e;
+    if (!model(&e)) return 0.0;
+    printf("|V| is: %d, |E| nnz is: %lld \n", graph_->getNumVertices(), (long long)graph_->getNNZ());
+    printf("Flops estimated Pruned Fascia: %f Gflop\n", e.pruned_fascia_gflop);
+    fflush(stdout);
+    return e.pruned_fascia_gflop;
+}
+
+double CountMat::estimateMemCommPrunedFascia()
+{
+    pg_estimate e;
+    if (!model(&e)) return 0.0;
+    printf("Comm Bytes Pruned Fascia estimated:  %f GBytes\n", e.pruned_fascia_gbytes);
+    fflush(stdout);
+    return e.pruned_fascia_gbytes;
+}
+
+void CountMat::degreeDistribution()
+{
+    if (!graph_) return;
+    const int32_t n = graph_->getNumVertices();
+    const int32_t* deg = graph_->getDegList();
+    int64_t max_deg = 0;
+    for (int32_t i = 0; i < n; ++i) max_deg = std::max<int64_t>(max_deg, deg[i]);
+    printf("maxDeg: %lld\n", (long long)max_deg);
+    fflush(stdout);
+    std::vector<int64_t> hist((size_t)max_deg + 1, 0);
+    for (int32_t i = 0; i < n; ++i) hist[deg[i]]++;
+    std::ofstream out("degDistri.data");
+    for (size_t d = 1; d < hist.size(); ++d)
+        if (hist[d] > 0) out << d << " " << hist[d] << "\n";
+}
+
+double CountMat::estimateTemplate()
+{
+    pg_estimate e;
+    if (!model(&e)) return 0.0;
+    printf("Workload N: %f, NNZ: %f \n", e.workload_n, e.workload_nnz);
+    printf("Memload N: %f, NNZ: %f \n", e.memload_n, e.memload_nnz);
+    fflush(stdout);
+    return 0.0;
 }
 
 }  // namespace pgbsc

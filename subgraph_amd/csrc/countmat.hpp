@@ -27,14 +27,44 @@ public:
     // CountMat.cpp:1498-1572: number of automorphisms of the template.
     int automorphismNum() const;
 
+    // The reference's analytic estimators (CountMat.cpp:1067-1496), printing
+    // its lines.  They need the subtemplate chain, so they work after
+    // compute() (compute(t, true) sets it up without counting), like the
+    // reference's, which read the tables compute() builds.  GB / GBytes /
+    // Gflop are 2^30.  Figures come from pg_estimate_model.
+    void estimatePeakMemUsage();
+    double estimateFlopsPGBSC();
+    double estimateMemCommPGBSC();
+    double estimateFlopsFascia();
+    double estimateMemCommFascia();
+    double estimateFlopsPrunedFascia();
+    double estimateMemCommPrunedFascia();
+    // Writes "degree count" lines to degDistri.data in the working directory
+    // and prints the largest degree (CountMat.cpp:1374-1400).
+    void degreeDistribution();
+    // Prints the raw workload / memload sums (CountMat.cpp:1448-1496; the
+    // reference's version has no return statement, this one returns 0).
+    double estimateTemplate();
+
+    // VERBOSE-style report (the reference's compile-time VERBOSE blocks,
+    // CountMat.cpp:132-162, 216-242): with it on, compute() prints every
+    // estimate before counting, then the per-kernel breakdown of the device
+    // time and the model's bandwidth and throughput over the measured time.
+    void set_verbose(bool on) { verbose_ = on; }
+
     const std::vector<double>& raw_counts() const { return raw_; }
     double seconds_per_coloring() const { return sec_per_col_; }
     const std::string& error() const { return err_; }
     pg_ctx* context() { return ctx_; }
 
 private:
+    bool model(pg_estimate* e);
+
     pg_ctx* ctx_ = nullptr;
+    const CSCGraph* graph_ = nullptr;
     const Graph* tmpl_ = nullptr;
+    FlatPlan fp_;
+    bool verbose_ = false;
     int itr_ = 1, thd_ = 1, precision_ = PG_FP32;
     bool autom_ = false;
     uint64_t seed_ = 0;

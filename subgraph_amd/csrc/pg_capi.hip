@@ -2047,6 +2047,46 @@ extern "C" int pg_plan_info(const pg_plan* p, int* k, int* nsub, int* n_nonleaf,
     return PG_OK;
 }
 
+// Device side of CountMat::estimatePeakMemUsage (CountMat.cpp:1067-1115):
+// what a forward pass of B colorings holds, and the byte model of the
+// per-kernel timing records (enqueue_pass / enqueue_step) per coloring.
+extern "C" int pg_plan_footprint(pg_ctx* c, const pg_plan* p, int B, int precision, int64_t* device_bytes,
+                                 double* algorithmic_bytes_per_coloring)
+{
+    if (!c) return PG_ERR_INVALID;
+    if (!p || p->ctx != c || B <= 0) return set_err(c, PG_ERR_INVALID, "invalid footprint arguments");
+    if (precision != PG_FP32 && precision != PG_FP64) return set_err(c, PG_ERR_INVALID, "precision must be PG_FP32 or PG_FP64");
+    if (!c->has_graph) return set_err(c, PG_ERR_STATE, "no graph uploaded");
+    int rc = build_buckets(c);
+    if (rc) return rc;
+    const size_t tsz = precision == PG_FP64 ? 8 : 4;
+    if (device_bytes) {
+        size_t b = plan_memory(c, p, B, tsz).total;
+        const DevBuf* graph_bufs[] = {&c->d_rowptr, &c->d_col, &c->d_ent, &c->d_perm, &c->d_w32, &c->d_w64,
+                                      &c->d_wsrc, &c->d_hub_items, &c->d_slot_begin, &p->rank_tab};
+        for (const DevBuf* g : graph_bufs)
+            b += g->bytes;
+        for (auto& d : p->subs)
+            for (const DevBuf* t : {&d->split, &d->ymask, &d->by_aux, &d->by_main, &d->amask, &d->gidx, &d->xidx,
+                                    &d->gidxT})
+                b += t->bytes;
+        *device_bytes = (int64_t)b;
+    }
+    if (algorithmic_bytes_per_coloring) {
+        const double na = (double)c->n_act;
+        double bytes = 0;
+        bool need_H = false;
+        for (auto& d : p->subs) {
+            need_H |= d->aa == 1;
+            bytes += 4.0 * na * B * ((d->aa > 1 ? d->Ca : 0) + (d->mm > 1 ? d->Cm : 0) + (d->root ? 0 : d->Cs));
+            if (d->aa > 1) bytes += csc_bytes(c);
+        }
+        if (need_H) bytes += csc_bytes(c) + na * B;
+        *algorithmic_bytes_per_coloring = bytes / B;
+    }
+    return PG_OK;
+}
+
 extern "C" int pg_count(pg_ctx* c, const pg_plan* p, int ncol, const uint8_t* colors, uint64_t seed, int precision,
                         uint32_t flags, double* raw_out)
 {

@@ -9,6 +9,12 @@
 // "fp64" = exact counts).  <graph> may also be "rmat:<scale>:<edge_factor>[:seed]"
 // for a synthetic R-MAT graph (a=.57 b=.19 c=.19).  The reference's start-up
 // memory-bandwidth probe (sc-main.cpp:1606) is not run.
+//
+// The reference switches its estimate report on by rebuilding (the VERBOSE
+// macro, and isEstimate at sc-main.cpp:1581-1584); here the environment does:
+//   PG_VERBOSE=1   print the cost-model estimates before counting and the
+//                  per-kernel device times and model bandwidth after it
+//   PG_ESTIMATE=1  stop after the estimates (compute(t, isEstimate = true))
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -115,7 +121,11 @@ int main(int argc, char** argv)
         fprintf(stderr, "error: %s\n", executor.error().c_str());
         return 1;
     }
-    double res = executor.compute(tmpl, false);
+    const char* verbose = getenv("PG_VERBOSE");
+    const char* estimate = getenv("PG_ESTIMATE");
+    const bool is_estimate = estimate && atoi(estimate) != 0;
+    executor.set_verbose(is_estimate || (verbose && atoi(verbose) != 0));
+    double res = executor.compute(tmpl, is_estimate);
     if (res < 0) {
         fprintf(stderr, "error: %s\n", executor.error().c_str());
         return 1;
