@@ -159,7 +159,7 @@ struct pg_ctx {
     bool use_graphs = true;
     uint64_t generation = 0;
     std::map<GraphKey, PassGraph> graphs;
-    std::map<std::tuple<const pg_plan*, int, int>, std::vector<int64_t>> tuned;  // (plan, B, tsize) -> settings
+    std::map<std::tuple<const pg_plan*, int, int>, std::vector<int64_t>> tuned;  // (plan, B, tsize) -> settings, tile
     // timing of the last call
     std::vector<LaunchRec>* cur_recs = nullptr;
     std::map<std::string, KernelTotals> agg;
@@ -868,13 +868,20 @@ int time_pass(pg_ctx* c, const pg_plan* p, int B, double* ms)
     return rc;
 }
 
+// Coordinate descent over the knobs at the largest tile B that fits, then
+// the column tile itself: B/2 and B/4 colorings per pass are timed with the
+// chosen knobs and kept when a coloring costs 2% less (wide templates can
+// run better with fewer, narrower passes).  A tile fixed by option
+// batch_tile is kept.  *tile receives the tile to run.
 template <typename T>
-int autotune(pg_ctx* c, const pg_plan* p, int B)
+int autotune(pg_ctx* c, const pg_plan* p, int B, int* tile)
 {
+    *tile = B;
     const auto key = std::make_tuple(p, B, (int)sizeof(T));
     auto it = c->tuned.find(key);
     if (it != c->tuned.end()) {
         apply_settings(c, it->second);
+        *tile = (int)it->second.back();
         return build_buckets(c);
     }
     const int64_t timing = c->timing;
@@ -900,11 +907,28 @@ int autotune(pg_ctx* c, const pg_plan* p, int B)
             }
         }
     }
+    int best_tile = B;
+    if (rc == PG_OK && c->batch_tile <= 0 && best_ms < INFINITY) {
+        apply_settings(c, best);
+        double best_per = best_ms / B;
+        for (int t = B / 2; t >= 1 && t >= B / 4; t /= 2) {
+            double ms = INFINITY;
+            rc = time_pass<T>(c, p, t, &ms);
+            if (rc) break;
+            if (ms / t < 0.98 * best_per) {
+                best_per = ms / t;
+                best_tile = t;
+            }
+        }
+    }
     c->timing = timing;
     c->use_graphs = graphs;
     apply_settings(c, best);
     if (rc) return rc;
-    c->tuned[key] = best;
+    std::vector<int64_t> rec = best;
+    rec.push_back(best_tile);
+    c->tuned[key] = rec;
+    *tile = best_tile;
     return build_buckets(c);
 }
 
@@ -925,9 +949,12 @@ int count_impl(pg_ctx* c, const pg_plan* p, int ncol, const uint8_t* colors, uin
     c->last_tile = tile;
     c->last_recompute = 0;
     if (c->autotune) {
-        rc = autotune<T>(c, p, tile);
+        int tuned_tile = tile;
+        rc = autotune<T>(c, p, tile, &tuned_tile);
         if (rc) return rc;
         // the chosen settings fit this tile (tuning skipped those that did not)
+        tile = tuned_tile;
+        c->last_tile = tile;
         mp = plan_memory(c, p, tile, sizeof(T));
     }
     const uint8_t* d_in = nullptr;

@@ -171,6 +171,8 @@ def test_autotune_keeps_results(gpu_ctx):
         print("[autotune] first call (tunes)", flush=True)
         g64 = gpu_ctx.count(plan, 4, colors=cols, precision="fp64")
         chosen64 = {k: gpu_ctx.get_option(k) for k in keys[1:]}
+        tile64 = gpu_ctx.get_option("last_tile")
+        assert tile64 in (1, 2, 4)  # the column tile is tuned too
         g32 = gpu_ctx.count(plan, 4, colors=cols, precision="fp32")  # its own key: tunes again
         chosen32 = {k: gpu_ctx.get_option(k) for k in keys[1:]}
         print(f"[autotune] chosen fp64 {chosen64} fp32 {chosen32}", flush=True)
@@ -184,6 +186,7 @@ def test_autotune_keeps_results(gpu_ctx):
         again = gpu_ctx.count(plan, 4, colors=cols, precision="fp64")  # reuses the fp64 choice
         assert np.array_equal(again, g64)
         assert {k: gpu_ctx.get_option(k) for k in keys[1:]} == chosen64
+        assert gpu_ctx.get_option("last_tile") == tile64
     finally:
         for k, v in zip(keys, saved):
             gpu_ctx.set_option(k, v)
