@@ -243,9 +243,16 @@ def s22_variant(sync, with_grad=True):
                           "fwd_bwd_vs_fwd": t_f / t_fb, "fwd_colorings_per_pass": fwd_tile,
                           "fwd_bwd_colorings_per_pass": ctx.get_option("last_tile"),
                           "adjoint_tables_recomputed": bool(ctx.get_option("last_recompute"))}
+            # the same with every table rebuilt from its subtree in the reverse sweep
+            ctx.set_option("adjoint_recompute", 1)
+            t_rc = _timed(lambda r=0: ctx.count_grad(plan, 16, seed=600 + r, precision="fp32"), 1, sync)
+            rec[wname]["fwd_bwd_recompute_colorings_per_s"] = 16 / t_rc
+            rec[wname]["fwd_bwd_recompute_colorings_per_pass"] = ctx.get_option("last_tile")
+            ctx.set_option("adjoint_recompute", -1)
             log(f"s22 tree12 {wname}: fwd {16 / t_f:.2f}/s ({fwd_tile}/pass), fwd+bwd {16 / t_fb:.2f}/s "
                 f"({rec[wname]['fwd_bwd_colorings_per_pass']}/pass, recompute "
-                f"{rec[wname]['adjoint_tables_recomputed']})")
+                f"{rec[wname]['adjoint_tables_recomputed']}); tables rebuilt: {16 / t_rc:.2f}/s "
+                f"({rec[wname]['fwd_bwd_recompute_colorings_per_pass']}/pass)")
         out["tree12_fwd_bwd"] = {"colorings_per_call": 16, **rec}
     return out
 

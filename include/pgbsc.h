@@ -73,7 +73,8 @@ const char* pg_last_error(const pg_ctx* ctx);
  *                  -1 when most of a neighbour row is used
  *   "adjoint_recompute" pg_count_grad: 0 keeps every forward table for the reverse
  *                  sweep, 1 rebuilds each table from its subtree when the sweep needs it,
- *                  -1 rebuilds only when that fits more colorings per pass (default)
+ *                  -1 rebuilds only when that fits four times as many colorings per pass,
+ *                  or when kept tables do not fit one coloring (default)
  * Read-only: "last_tile" (colorings per pass of the last count / gradient call) and
  * "last_recompute" (1 if that gradient call rebuilt tables). */
 int pg_set_option(pg_ctx* ctx, const char* key, int64_t value);

@@ -116,13 +116,15 @@ class Plan:
     def __init__(self, ctx: "Context", ptr: int, nv: int):
         self._ctx = ctx
         self.ptr = ptr
+        # bound now: at interpreter exit the module's globals may already be gone
+        self._destroy = lib().pg_plan_destroy
         k, nsub, nonleaf, scaled = (C.c_int() for _ in range(4))
         lib().pg_plan_info(ptr, C.byref(k), C.byref(nsub), C.byref(nonleaf), C.byref(scaled))
         self.k, self.nsub, self.n_nonleaf, self.scaled_sub = k.value, nsub.value, nonleaf.value, scaled.value
 
     def __del__(self):
         if getattr(self, "ptr", None):
-            lib().pg_plan_destroy(self.ptr)
+            self._destroy(self.ptr)
             self.ptr = None
 
 
@@ -136,13 +138,14 @@ class Context:
         if rc:
             raise PGError(rc, f"pg_create(device={device}): no usable HIP device")
         self.ptr = p.value
+        self._destroy = L.pg_destroy  # bound now, as for Plan
         self.n = self.nnz = 0
         for k, v in options.items():
             self.set_option(k, v)
 
     def __del__(self):
         if getattr(self, "ptr", None):
-            lib().pg_destroy(self.ptr)
+            self._destroy(self.ptr)
             self.ptr = None
 
     def _check(self, rc: int, what: str):

@@ -136,7 +136,7 @@ struct pg_ctx {
     int64_t aux_rows = 0;       // neighbour sums row-wise: 1 always, 0 never, -1 per subtemplate
     int64_t stage_rows = 0;     // gathers read neighbour rows staged in LDS (1) or directly (0)
     int64_t reg_cols = 0;       // values per lane of gathers: 0 one for rows under kRegCols, else 1 or 4
-    int64_t adjoint_recompute = -1;  // adjoint tables: 0 kept, 1 recomputed, -1 recomputed when that fits more
+    int64_t adjoint_recompute = -1;  // adjoint tables: 0 kept, 1 recomputed, -1 recomputed when that fits 4x more
     int64_t last_tile = 0, last_recompute = 0;  // of the last count / gradient call (read-only options)
     // graph
     bool has_graph = false, weighted = false;
@@ -1349,8 +1349,13 @@ int count_grad_impl(pg_ctx* c, const pg_plan* p, int ncol, const uint8_t* colors
         budget = (size_t)(0.8 * (double)(fr + c->d_arena.bytes));
     }
     // the largest batch tile that fits, with every table kept and with tables
-    // recomputed; option "adjoint_recompute": 0 keep, 1 recompute, -1 recompute
-    // only when it fits more colorings per pass than keeping does
+    // recomputed; option "adjoint_recompute": 0 keep, 1 recompute, -1 keep
+    // unless recomputing fits kRecomputeGain times as many colorings per pass
+    // (or keeping fits none).  Rebuilding a table repeats its subtree's
+    // forward sweep at every level that needs it (measured: 1.9x the time per
+    // coloring for the 12-vertex tree at R-MAT scale 22, same tile), while a
+    // wider pass gains less than that per coloring beyond a few colorings.
+    constexpr int kRecomputeGain = 4;
     const int cap = c->batch_tile > 0 ? (int)std::min<int64_t>(c->batch_tile, ncol) : std::min(ncol, 64);
     auto lds_ok = [&](int B) {
         size_t lds = 0;
@@ -1368,8 +1373,9 @@ int count_grad_impl(pg_ctx* c, const pg_plan* p, int ncol, const uint8_t* colors
             g_keep = g;
             break;
         }
-    if (c->adjoint_recompute == 1 || (c->adjoint_recompute == -1 && tile_keep < cap))
-        for (int B = cap; B > tile_keep; --B) {
+    const int rec_lo = c->adjoint_recompute == 1 ? 1 : tile_keep == 0 ? 1 : tile_keep * kRecomputeGain;
+    if (c->adjoint_recompute != 0)
+        for (int B = cap; B >= rec_lo; --B) {
             if (!lds_ok(B)) continue;
             GradPlan g = plan_memory_recompute<T>(c, p, B, budget);
             if (g.total > budget) continue;

@@ -203,3 +203,38 @@ def test_recompute_matches_kept_tables(gpu_ctx, tv, te):
                 assert np.any(g0 != 0)
     finally:
         gpu_ctx.set_option("adjoint_recompute", saved)
+
+
+def test_default_adjoint_mode(gpu_ctx):
+    """Option "adjoint_recompute" = -1 (default) keeps the tables while they
+    fit, and rebuilds them once a memory budget leaves no room to keep one
+    coloring's tables; both give the kept-table results."""
+    n = 1 << 16
+    s, d = sg.rmat_edges(16, 8, seed=3)
+    colptr, rowidx = sg.csc_from_edges(n, s, d)
+    keys = ("adjoint_recompute", "mem_budget_mb")
+    saved = [gpu_ctx.get_option(k) for k in keys]
+    try:
+        gpu_ctx.upload_graph(colptr, rowidx)
+        plan = gpu_ctx.plan(10, TREE10)
+        gpu_ctx.set_option("adjoint_recompute", -1)
+        print("[default-adjoint] ample memory", flush=True)
+        r_keep, g_keep = gpu_ctx.count_grad(plan, 2, seed=9, precision="fp64")
+        assert gpu_ctx.get_option("last_recompute") == 0
+        chosen = None
+        for mb in (512, 400, 320, 256, 200, 160, 128, 100, 80, 64, 50, 40, 32):
+            gpu_ctx.set_option("mem_budget_mb", mb)
+            print(f"[default-adjoint] budget {mb} MB", flush=True)
+            try:
+                r, g = gpu_ctx.count_grad(plan, 2, seed=9, precision="fp64")
+            except sg.PGError:
+                break  # not even the rebuilt tables fit
+            if gpu_ctx.get_option("last_recompute") == 1:
+                chosen = mb
+                assert np.array_equal(r, r_keep)
+                assert np.allclose(g, g_keep, rtol=1e-12, atol=0.0)
+                break
+        assert chosen is not None, "no budget made the default rebuild tables"
+    finally:
+        for k, v in zip(keys, saved):
+            gpu_ctx.set_option(k, v)
