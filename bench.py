@@ -129,17 +129,28 @@ def fwd_bwd_variant(sync):
     ctx.upload_graph(colptr, rowidx, vals=w)
     out = {"graph": "R-MAT scale 18 edge factor 16 (seed 1), weights U(0.5,1.5) symmetric",
            "nnz": int(len(rowidx)), "colorings_per_call": 16, "dtype": "fp32", "templates": {}}
+    import torch
+    grad = torch.empty(len(rowidx), dtype=torch.float32, device="cuda")  # the autograd path's output
     for name, te in (("star5", [(0, 1), (0, 2), (0, 3), (0, 4)]), ("u5", [(0, 1), (1, 2), (2, 3), (3, 4)])):
         plan = ctx.plan(5, te)
         t_plain = _timed(lambda r=0: ctx.count(plan, 1, seed=50 + r), 10, sync)
         t_fwd = _timed(lambda r=0: ctx.count(plan, 16, seed=60 + r), 5, sync)
-        t_fb = _timed(lambda r=0: ctx.count_grad(plan, 16, seed=70 + r, precision="fp32"), 3, sync)
+        t_fb1 = _timed(lambda r=0: ctx.count_grad_device(plan, 1, grad.data_ptr(), seed=80 + r), 10, sync)
+        t_fb = _timed(lambda r=0: ctx.count_grad_device(plan, 16, grad.data_ptr(), seed=70 + r), 5, sync)
+        t_fbh = _timed(lambda r=0: ctx.count_grad(plan, 16, seed=90 + r, precision="fp32"), 3, sync)
         rec = {"plain_fwd_colorings_per_s": 1.0 / t_plain, "batched_fwd_colorings_per_s": 16 / t_fwd,
-               "fwd_bwd_colorings_per_s": 16 / t_fb}
+               "plain_fwd_bwd_colorings_per_s": 1.0 / t_fb1, "fwd_bwd_colorings_per_s": 16 / t_fb,
+               "fwd_bwd_host_grad_colorings_per_s": 16 / t_fbh}
+        rec["fwd_bwd_vs_plain_fwd_bwd"] = rec["fwd_bwd_colorings_per_s"] / rec["plain_fwd_bwd_colorings_per_s"]
         rec["fwd_bwd_vs_plain_fwd"] = rec["fwd_bwd_colorings_per_s"] / rec["plain_fwd_colorings_per_s"]
         out["templates"][name] = rec
         log(f"fwd+bwd s18 {name}: plain fwd {rec['plain_fwd_colorings_per_s']:.0f}/s, batched fwd "
-            f"{rec['batched_fwd_colorings_per_s']:.0f}/s, fwd+bwd {rec['fwd_bwd_colorings_per_s']:.0f}/s")
+            f"{rec['batched_fwd_colorings_per_s']:.0f}/s, plain fwd+bwd {rec['plain_fwd_bwd_colorings_per_s']:.0f}/s, "
+            f"fwd+bwd {rec['fwd_bwd_colorings_per_s']:.0f}/s (gradient to host memory: "
+            f"{rec['fwd_bwd_host_grad_colorings_per_s']:.0f}/s)")
+    out["note"] = ("fwd+bwd: the gradient stays on the device as float32 (the torch.autograd path); "
+                   "host_grad: the same call returning FP64 gradients in host memory (a PCIe copy of 8 B per "
+                   "entry per call)")
     return out
 
 
@@ -228,6 +239,7 @@ def s22_variant(sync, with_grad=True):
                     "colorings_per_s": sum(counts) / t_m, "seconds": t_m}
     log(f"s22 mixed {counts}: {sum(counts) / t_m:.1f} colorings/s")
     if with_grad:
+        import torch
         ctx.set_option("autotune", 0)
         plan = ctx.plan(12, TREE12)
         rec = {}
@@ -238,14 +250,15 @@ def s22_variant(sync, with_grad=True):
                 ctx.set_weights(hashed_weights(n, colptr, rowidx))
             t_f = _timed(lambda r=0: ctx.count(plan, 16, seed=400 + r), 1, sync)
             fwd_tile = ctx.get_option("last_tile")
-            t_fb = _timed(lambda r=0: ctx.count_grad(plan, 16, seed=500 + r, precision="fp32"), 1, sync)
+            gbuf = torch.empty(len(rowidx), dtype=torch.float32, device="cuda")  # the autograd path's output
+            t_fb = _timed(lambda r=0: ctx.count_grad_device(plan, 16, gbuf.data_ptr(), seed=500 + r), 1, sync)
             rec[wname] = {"fwd_colorings_per_s": 16 / t_f, "fwd_bwd_colorings_per_s": 16 / t_fb,
                           "fwd_bwd_vs_fwd": t_f / t_fb, "fwd_colorings_per_pass": fwd_tile,
                           "fwd_bwd_colorings_per_pass": ctx.get_option("last_tile"),
                           "adjoint_tables_recomputed": bool(ctx.get_option("last_recompute"))}
             # the same with every table rebuilt from its subtree in the reverse sweep
             ctx.set_option("adjoint_recompute", 1)
-            t_rc = _timed(lambda r=0: ctx.count_grad(plan, 16, seed=600 + r, precision="fp32"), 1, sync)
+            t_rc = _timed(lambda r=0: ctx.count_grad_device(plan, 16, gbuf.data_ptr(), seed=600 + r), 1, sync)
             rec[wname]["fwd_bwd_recompute_colorings_per_s"] = 16 / t_rc
             rec[wname]["fwd_bwd_recompute_colorings_per_pass"] = ctx.get_option("last_tile")
             ctx.set_option("adjoint_recompute", -1)
