@@ -171,8 +171,13 @@ int pg_count(pg_ctx* ctx, const pg_plan* plan, int ncolorings, const uint8_t* co
              uint64_t seed, int precision, uint32_t flags, double* raw_out);
 
 /* Mixed batch: plans[p] is counted for ncolorings[p] colorings drawn on the
- * device from seeds[p]; raw_out is the concatenation of the per-plan results.
- * No reference counterpart (the reference counts one template per run). */
+ * device from seeds[p]; raw_out is the concatenation of the per-plan results,
+ * equal to what pg_count gives each plan with the same seed.  With graphs on
+ * and timing off, each round of passes (one pass of every plan with
+ * colorings left) is one launch of a captured graph with a branch per plan,
+ * when the plans fit side by side in memory and share bucket settings;
+ * otherwise the plans run one after another.  No reference counterpart (the
+ * reference counts one template per run). */
 int pg_count_multi(pg_ctx* ctx, int nplans, const pg_plan* const* plans,
                    const int32_t* ncolorings, const uint64_t* seeds, int precision,
                    uint32_t flags, double* raw_out);

@@ -159,6 +159,13 @@ struct pg_ctx {
     bool use_graphs = true;
     uint64_t generation = 0;
     std::map<GraphKey, PassGraph> graphs;
+    // mixed batches (pg_count_multi): one graph per set of (plan, colorings,
+    // settings) with a branch per plan, the side streams its branches are
+    // captured on, and the fork / join events
+    std::map<std::vector<int64_t>, PassGraph> multi_graphs;
+    std::vector<hipStream_t> side;
+    std::vector<hipEvent_t> ev_join;
+    hipEvent_t ev_fork = nullptr;
     std::map<std::tuple<const pg_plan*, int, int>, std::vector<int64_t>> tuned;  // (plan, B, tsize) -> settings, tile
     // timing of the last call
     std::vector<LaunchRec>* cur_recs = nullptr;
@@ -738,6 +745,15 @@ void drop_graphs(pg_ctx* c, const pg_plan* only)
         (void)hipGraphExecDestroy(it->second.exec);
         it = c->graphs.erase(it);
     }
+    for (auto it = c->multi_graphs.begin(); it != c->multi_graphs.end();) {
+        const auto& k = it->first;
+        if (only && std::find(k.begin(), k.end(), (int64_t)(intptr_t)only) == k.end()) {
+            ++it;
+            continue;
+        }
+        (void)hipGraphExecDestroy(it->second.exec);
+        it = c->multi_graphs.erase(it);
+    }
 }
 
 // One pass: through its cached graph when graphs are on and timing is off
@@ -989,6 +1005,217 @@ int count_impl(pg_ctx* c, const pg_plan* p, int ncol, const uint8_t* colors, uin
     (void)hipEventDestroy(t1);
     finish_timing(c);
     return rc;
+}
+
+// ------------------------------------------------------------ mixed batch
+//
+// pg_count_multi with graphs on: each plan gets its tile (and, with option
+// autotune, its tuned kernel variants and tile), its own region of the arena,
+// and a branch of ONE captured graph per round of passes.  The branches are
+// captured on side streams forked from and joined back into the context's
+// stream, so a single graph launch runs a pass of every template and the
+// device may overlap them.  The branches share the degree-bucket work lists,
+// so every plan must use the same bucket settings; otherwise (and with timing
+// on, or graphs off) the plans run one after another through count_impl.
+
+// the knob values without dropping captured graphs (the bucket knobs, which
+// size the shared work lists, are equal for every plan of a mixed graph)
+void put_settings(pg_ctx* c, const std::vector<int64_t>& v)
+{
+    const auto& knobs = tune_knobs();
+    for (size_t i = 0; i < knobs.size(); ++i) c->*knobs[i].field = v[i];
+}
+
+bool same_buckets(const std::vector<int64_t>& a, const std::vector<int64_t>& b)
+{
+    const auto& knobs = tune_knobs();
+    for (size_t i = 0; i < knobs.size(); ++i) {
+        const std::string name = knobs[i].name;
+        if ((name == "bucket_high" || name == "bucket_mid" || name == "hub_degree") && a[i] != b[i]) return false;
+    }
+    return true;
+}
+
+MemPlan shifted(MemPlan m, size_t d)
+{
+    for (size_t* o : {&m.off_H, &m.off_part, &m.off_root, &m.off_colors, &m.off_ecol, &m.off_rawdev, &m.off_tmp,
+                      &m.off_params})
+        *o += d;
+    for (auto& o : m.off_tab) o += d;
+    m.total += d;
+    return m;
+}
+
+int ensure_branches(pg_ctx* c, int n)
+{
+    if (!c->ev_fork) HIP_TRY(c, hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+    while ((int)c->side.size() < n) {
+        hipStream_t s = nullptr;
+        HIP_TRY(c, hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        c->side.push_back(s);
+        hipEvent_t e = nullptr;
+        HIP_TRY(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        c->ev_join.push_back(e);
+    }
+    return PG_OK;
+}
+
+// *done = false: not handled here (the caller runs the plans one by one)
+template <typename T>
+int count_multi_impl(pg_ctx* c, int np, const pg_plan* const* plans, const int32_t* ncol, const uint64_t* seeds,
+                     uint32_t flags, double* raw_out, bool* done)
+{
+    *done = false;
+    if (!c->use_graphs || c->timing || np < 2) return PG_OK;
+    int rc = build_buckets(c);
+    if (rc) return rc;
+    const size_t tsz = sizeof(T);
+    const std::vector<int64_t> entry = current_settings(c);
+    std::vector<int> tile(np);
+    std::vector<std::vector<int64_t>> knob(np, entry);
+    for (int i = 0; i < np; ++i) {
+        MemPlan mp;
+        rc = choose_tile(c, plans[i], ncol[i], tsz, &tile[i], &mp);
+        if (rc) return rc;
+        if (!c->autotune) continue;
+        const auto key = std::make_tuple(plans[i], tile[i], (int)tsz);
+        auto it = c->tuned.find(key);
+        if (it == c->tuned.end()) {
+            if (mp.total > c->d_arena.bytes) {
+                drop_graphs(c);
+                HIP_TRY(c, c->d_arena.alloc(mp.total));
+            }
+            int t = tile[i];
+            rc = autotune<T>(c, plans[i], tile[i], &t);
+            if (rc) return rc;
+            it = c->tuned.find(key);
+        }
+        knob[i].assign(it->second.begin(), it->second.begin() + (long)tune_knobs().size());
+        tile[i] = (int)it->second.back();
+    }
+    if (c->autotune) {  // without it every plan runs the caller's settings as they are
+        for (int i = 1; i < np; ++i)
+            if (!same_buckets(knob[i], knob[0])) {
+                apply_settings(c, entry);
+                return build_buckets(c);
+            }
+        apply_settings(c, knob[0]);  // the shared bucket settings
+        rc = build_buckets(c);
+        if (rc) return rc;
+    }
+    // arena regions, one per plan
+    size_t budget;
+    if (c->mem_budget_mb > 0) {
+        budget = (size_t)c->mem_budget_mb << 20;
+    } else {
+        size_t fr = 0, tot = 0;
+        HIP_TRY(c, hipMemGetInfo(&fr, &tot));
+        budget = (size_t)(0.9 * (double)(fr + c->d_arena.bytes));
+    }
+    std::vector<MemPlan> mp(np);
+    std::vector<size_t> base(np);
+    size_t total = 0;
+    for (int i = 0; i < np; ++i) {
+        mp[i] = plan_memory(c, plans[i], tile[i], tsz);
+        base[i] = total;
+        total += align_up(mp[i].total, 256);
+    }
+    if (total > budget) return PG_OK;  // the plans do not fit side by side: one after another
+    if (total > c->d_arena.bytes) {
+        drop_graphs(c);
+        HIP_TRY(c, c->d_arena.alloc(total));
+    }
+    rc = ensure_branches(c, np);
+    if (rc) return rc;
+    *done = true;
+    char* arena = c->d_arena.as<char>();
+    std::vector<int64_t> first(np, 0), out_off(np, 0);
+    for (int i = 1; i < np; ++i) out_off[i] = out_off[i - 1] + ncol[i - 1];
+    c->agg.clear();
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    HIP_TRY(c, hipEventCreate(&t0));
+    HIP_TRY(c, hipEventCreate(&t1));
+    HIP_TRY(c, hipEventRecord(t0, c->stream));
+    std::vector<double> raw;
+    for (;;) {
+        std::vector<int> B(np, 0);
+        std::vector<MemPlan> m(np);
+        std::vector<int64_t> key = {(int64_t)tsz, (int64_t)(intptr_t)c->d_arena.p, (int64_t)c->generation};
+        bool any = false;
+        for (int i = 0; i < np; ++i) {
+            B[i] = (int)std::min<int64_t>(tile[i], ncol[i] - first[i]);
+            if (B[i] <= 0) continue;
+            any = true;
+            m[i] = shifted(B[i] == tile[i] ? mp[i] : plan_memory(c, plans[i], B[i], tsz), base[i]);
+            key.push_back((int64_t)(intptr_t)plans[i]);
+            key.push_back(B[i]);
+            key.insert(key.end(), knob[i].begin(), knob[i].end());
+            const PassParams hp{seeds[i], first[i]};
+            // pageable source: the copy completes before the call returns
+            HIP_TRY(c, hipMemcpyAsync(arena + m[i].off_params, &hp, sizeof hp, hipMemcpyHostToDevice, c->stream));
+        }
+        if (!any) break;
+        auto it = c->multi_graphs.find(key);
+        if (it == c->multi_graphs.end()) {
+            const hipStream_t main = c->stream;
+            HIP_TRY(c, hipStreamBeginCapture(main, hipStreamCaptureModeThreadLocal));
+            hipError_t e = hipEventRecord(c->ev_fork, main);
+            for (int i = 0; i < np && rc == PG_OK && e == hipSuccess; ++i) {
+                if (B[i] <= 0) continue;
+                e = hipStreamWaitEvent(c->side[i], c->ev_fork, 0);
+                if (e != hipSuccess) break;
+                put_settings(c, knob[i]);
+                c->stream = c->side[i];
+                rc = enqueue_pass<T>(c, plans[i], B[i], nullptr, m[i]);
+                c->stream = main;
+                if (rc) break;
+                e = hipEventRecord(c->ev_join[i], c->side[i]);
+                if (e == hipSuccess) e = hipStreamWaitEvent(main, c->ev_join[i], 0);
+            }
+            put_settings(c, knob[0]);
+            hipGraph_t graph = nullptr;
+            const hipError_t ee = hipStreamEndCapture(main, &graph);
+            if (rc || e != hipSuccess || ee != hipSuccess) {
+                if (graph) (void)hipGraphDestroy(graph);
+                if (rc) return rc;
+                return set_err(c, PG_ERR_HIP, "mixed-batch graph capture failed: %s",
+                               hipGetErrorString(e != hipSuccess ? e : ee));
+            }
+            PassGraph g;
+            e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (e != hipSuccess)
+                return set_err(c, PG_ERR_HIP, "graph instantiation failed: %s", hipGetErrorString(e));
+            it = c->multi_graphs.emplace(key, std::move(g)).first;
+        }
+        HIP_TRY(c, hipGraphLaunch(it->second.exec, c->stream));
+        raw.assign(std::accumulate(B.begin(), B.end(), 0), 0.0);
+        size_t o = 0;
+        for (int i = 0; i < np; ++i) {
+            if (B[i] <= 0) continue;
+            HIP_TRY(c, hipMemcpyAsync(raw.data() + o, arena + m[i].off_rawdev, sizeof(double) * B[i],
+                                      hipMemcpyDeviceToHost, c->stream));
+            o += B[i];
+        }
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        o = 0;
+        for (int i = 0; i < np; ++i) {
+            if (B[i] <= 0) continue;
+            for (int b = 0; b < B[i]; ++b)
+                raw_out[out_off[i] + first[i] + b] = finish_raw(plans[i], sizeof(T) == 4, flags, raw[o + b]);
+            o += B[i];
+            first[i] += B[i];
+        }
+    }
+    (void)hipEventRecord(t1, c->stream);
+    (void)hipEventSynchronize(t1);
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, t0, t1);
+    c->last_total_ms = ms;
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    c->last_tile = tile[0];
+    return PG_OK;
 }
 
 // ---------------------------------------------------------------- adjoint
@@ -1606,6 +1833,9 @@ extern "C" void pg_destroy(pg_ctx* c)
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     drop_graphs(c);
+    for (hipStream_t s : c->side) (void)hipStreamDestroy(s);
+    for (hipEvent_t e : c->ev_join) (void)hipEventDestroy(e);
+    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
     (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -2149,6 +2379,22 @@ extern "C" int pg_count_multi(pg_ctx* c, int nplans, const pg_plan* const* plans
     if (!c) return PG_ERR_INVALID;
     if (nplans <= 0 || !plans || !ncolorings || !seeds || !raw_out)
         return set_err(c, PG_ERR_INVALID, "invalid multi-count arguments");
+    for (int i = 0; i < nplans; ++i) {
+        if (!plans[i] || plans[i]->ctx != c || ncolorings[i] <= 0)
+            return set_err(c, PG_ERR_INVALID, "invalid plan or coloring count at %d", i);
+    }
+    if (precision != PG_FP32 && precision != PG_FP64)
+        return set_err(c, PG_ERR_INVALID, "precision must be PG_FP32 or PG_FP64");
+    if (!c->has_graph) return set_err(c, PG_ERR_STATE, "no graph uploaded");
+    HIP_TRY(c, hipSetDevice(c->device));
+    if (c->n_act > 0) {
+        bool done = false;
+        int rc = precision == PG_FP32
+                     ? count_multi_impl<float>(c, nplans, plans, ncolorings, seeds, flags, raw_out, &done)
+                     : count_multi_impl<double>(c, nplans, plans, ncolorings, seeds, flags, raw_out, &done);
+        if (rc) return rc;
+        if (done) return PG_OK;
+    }
     int64_t off = 0;
     double total = 0;
     for (int i = 0; i < nplans; ++i) {

@@ -375,3 +375,48 @@ def test_full_size_properties_s20(gpu_ctx):
     finally:
         gpu_ctx.set_option("hub_degree", saved)
     assert np.array_equal(nohub, g64)
+
+
+def test_mixed_batch_matches_single_counts(gpu_ctx):
+    """pg_count_multi runs a pass of every template in one graph launch (a
+    branch per template).  Its counts equal each template counted alone with
+    the same seeds, bit for bit in both precisions: over several rounds of
+    passes (a small batch tile), on replays of the cached graph, with the
+    autotuner, and with the oracle for one coloring of each."""
+    keys = ("batch_tile", "autotune", "reg_cols", "stage_rows", "aux_rows", "bucket_high", "bucket_mid",
+            "hub_degree", "hub_chunk")
+    saved = [gpu_ctx.get_option(k) for k in keys]
+    n = 1 << 12
+    s, d = sg.rmat_edges(12, 8, seed=6)
+    colptr, rowidx = sg.csc_from_edges(n, s, d)
+    u5 = [(0, 1), (1, 2), (2, 3), (3, 4)]
+    tree10 = [(0, 1), (1, 2), (2, 3), (3, 4), (1, 5), (5, 6), (5, 7), (7, 8), (7, 9)]
+    try:
+        gpu_ctx.upload_graph(colptr, rowidx)
+        plans = [gpu_ctx.plan(5, u5), gpu_ctx.plan(7, TREE7), gpu_ctx.plan(10, tree10)]
+        counts, seeds = [5, 3, 2], [11, 12, 13]
+        for tile, tune in ((0, 0), (2, 0), (0, 1)):
+            gpu_ctx.set_option("batch_tile", tile)
+            gpu_ctx.set_option("autotune", tune)
+            for prec in ("fp64", "fp32"):
+                print(f"[mixed] tile={tile} autotune={tune} {prec}", flush=True)
+                multi = gpu_ctx.count_multi(plans, counts, seeds, precision=prec)
+                again = gpu_ctx.count_multi(plans, counts, seeds, precision=prec)  # the cached graph
+                for i, p in enumerate(plans):
+                    single = gpu_ctx.count(p, counts[i], seed=seeds[i], precision=prec)
+                    assert np.array_equal(multi[i], single), (tile, tune, prec, i, multi[i], single)
+                    assert np.array_equal(again[i], multi[i])
+        # the oracle on the first coloring of each template (the color stream of its seed)
+        gpu_ctx.set_option("batch_tile", 0)
+        gpu_ctx.set_option("autotune", 0)
+        multi = gpu_ctx.count_multi(plans, counts, seeds, precision="fp64")
+        for i, (k, te) in enumerate(((5, u5), (7, TREE7), (10, tree10))):
+            cols = sg.colors_fill(seeds[i], 0, 1, n, k)
+            assert multi[i][0] == oracle.count_f64(oracle.Plan(k, te), colptr, rowidx, cols[0])
+        # the same plan twice in one batch: two branches, distinct seeds
+        two = gpu_ctx.count_multi([plans[1], plans[1]], [2, 2], [21, 22], precision="fp64")
+        assert np.array_equal(two[0], gpu_ctx.count(plans[1], 2, seed=21, precision="fp64"))
+        assert np.array_equal(two[1], gpu_ctx.count(plans[1], 2, seed=22, precision="fp64"))
+    finally:
+        for k, v in zip(keys, saved):
+            gpu_ctx.set_option(k, v)
