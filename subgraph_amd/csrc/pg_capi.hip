@@ -1956,14 +1956,6 @@ extern "C" int pg_graph_upload(pg_ctx* c, int32_t n, int64_t nnz, const int32_t*
             seen[v] = 1;
         }
     }
-    std::vector<int32_t> perm;
-    perm.reserve(n);
-    for (int32_t v = 0; v < n; ++v)
-        if (seen[v]) perm.push_back(v);
-    std::stable_sort(perm.begin(), perm.end(), [&](int32_t a, int32_t b) { return rdeg[a] > rdeg[b]; });
-    const int32_t na = (int32_t)perm.size();
-    std::vector<int32_t> inv(n, -1);
-    for (int32_t i = 0; i < na; ++i) inv[perm[i]] = i;
     // row-major view in caller ids (ascending column within each row)
     std::vector<int64_t> rptr((size_t)n + 1, 0);
     for (int32_t v = 0; v < n; ++v) rptr[v + 1] = rptr[v] + rdeg[v];
@@ -1978,6 +1970,53 @@ extern "C" int pg_graph_upload(pg_ctx* c, int32_t n, int64_t nnz, const int32_t*
                 pos[v]++;
             }
     }
+    // Internal order: degree classes, highest first, whose boundaries are
+    // every bucket threshold the autotuner tries (so each tuned bucket is a
+    // contiguous range), and within a class breadth-first order from the
+    // largest hubs, so that vertices processed side by side share neighbour
+    // rows in L2.
+    std::vector<int32_t> perm;
+    perm.reserve(n);
+    for (int32_t v = 0; v < n; ++v)
+        if (seen[v]) perm.push_back(v);
+    std::stable_sort(perm.begin(), perm.end(), [&](int32_t a, int32_t b) { return rdeg[a] > rdeg[b]; });
+    const int32_t na = (int32_t)perm.size();
+    {
+        std::vector<int32_t> bfs(n, -1);
+        std::vector<int32_t> queue;
+        queue.reserve(na);
+        int32_t next = 0;
+        for (int32_t root : perm) {
+            if (bfs[root] >= 0) continue;
+            bfs[root] = next++;
+            queue.assign(1, root);
+            for (size_t h = 0; h < queue.size(); ++h) {
+                const int32_t v = queue[h];
+                for (int64_t e = rptr[v]; e < rptr[v + 1]; ++e) {
+                    const int32_t u = rcol[e];
+                    if (bfs[u] < 0) {
+                        bfs[u] = next++;
+                        queue.push_back(u);
+                    }
+                }
+            }
+        }
+        static const int64_t bounds[] = {1025, 257, 129, 64, 32, 16, 8, 4};  // degree >= bound
+        auto cls = [&](int32_t v) {
+            int k = 0;
+            for (int64_t b : bounds) {
+                if (rdeg[v] >= b) break;
+                ++k;
+            }
+            return k;
+        };
+        std::stable_sort(perm.begin(), perm.end(), [&](int32_t a, int32_t b) {
+            const int ca = cls(a), cb = cls(b);
+            return ca != cb ? ca < cb : bfs[a] < bfs[b];
+        });
+    }
+    std::vector<int32_t> inv(n, -1);
+    for (int32_t i = 0; i < na; ++i) inv[perm[i]] = i;
     std::vector<int32_t> hrow((size_t)na + 1, 0), hcol((size_t)nnz), hent((size_t)nnz);
     for (int32_t i = 0; i < na; ++i) hrow[i + 1] = hrow[i] + rdeg[perm[i]];
 #pragma omp parallel for schedule(dynamic, 4096)
@@ -1993,7 +2032,8 @@ extern "C" int pg_graph_upload(pg_ctx* c, int32_t n, int64_t nnz, const int32_t*
     c->n = n;
     c->nnz = nnz;
     c->n_act = na;
-    c->max_deg = na > 0 ? rdeg[perm[0]] : 0;
+    c->max_deg = 0;
+    for (int32_t v : perm) c->max_deg = std::max(c->max_deg, rdeg[v]);
     c->h_rowptr = hrow;
     c->h_col = hcol;
     c->h_perm = perm;

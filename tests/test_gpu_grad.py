@@ -231,7 +231,12 @@ def test_default_adjoint_mode(gpu_ctx):
                 break  # not even the rebuilt tables fit
             if gpu_ctx.get_option("last_recompute") == 1:
                 chosen = mb
-                assert np.array_equal(r, r_keep)
+                # the two runs may differ in colorings per pass, hence in the
+                # order of their FP64 sums: exact only below 2^53
+                if np.abs(r_keep).max() < 2.0 ** 53:
+                    assert np.array_equal(r, r_keep)
+                else:
+                    assert np.allclose(r, r_keep, rtol=1e-12, atol=0.0)
                 assert np.allclose(g, g_keep, rtol=1e-12, atol=0.0)
                 break
         assert chosen is not None, "no budget made the default rebuild tables"

@@ -23,6 +23,15 @@ def _rel(a, b):
     return abs(a - b) / max(abs(b), 1e-300) if b != 0 else abs(a)
 
 
+def _same64(got, want) -> bool:
+    """FP64 counts summed in different orders: equal below 2^53 (integers
+    there are exact in any order), within 1e-12 relative above."""
+    got, want = np.atleast_1d(np.asarray(got, np.float64)), np.atleast_1d(np.asarray(want, np.float64))
+    small = np.abs(want) < 2.0 ** 53
+    return bool(np.array_equal(got[small], want[small]) and
+                np.allclose(got[~small], want[~small], rtol=1e-12, atol=0.0))
+
+
 def _simple_graph(rng, n, m):
     s = rng.integers(0, n, m)
     d = rng.integers(0, n, m)
@@ -96,7 +105,7 @@ def test_random_graphs_and_trees(gpu_ctx, hub):
                 o32 = oracle.count_f32(op, colptr, rowidx, cols[i])
                 o64 = oracle.count_f64(op, colptr, rowidx, cols[i])
                 assert _rel(g32[i], o32) <= TOL32, (trial, i, g32[i], o32)
-                assert g64[i] == o64, (trial, i, g64[i], o64)
+                assert _same64(g64[i], o64), (trial, i, g64[i], o64)
     finally:
         gpu_ctx.set_option("hub_degree", saved[0])
         gpu_ctx.set_option("hub_chunk", saved[1])
@@ -143,7 +152,7 @@ def test_kernel_variants_match_oracle(gpu_ctx, aux_rows, stage_rows, reg_cols, B
             g64 = gpu_ctx.count(plan, B, colors=cols, precision="fp64")
             for i in range(B):
                 o64 = oracle.count_f64(op, colptr, rowidx, cols[i])
-                assert g64[i] == o64, (trial, i, g64[i], o64)
+                assert _same64(g64[i], o64), (trial, i, g64[i], o64)
                 if i < 3:
                     o32 = oracle.count_f32(op, colptr, rowidx, cols[i])
                     assert _rel(g32[i], o32) <= TOL32, (trial, i, g32[i], o32)
@@ -181,7 +190,7 @@ def test_autotune_keeps_results(gpu_ctx):
             assert chosen["bucket_high"] in (16, 32, 64) and chosen["bucket_mid"] in (4, 8)
             assert chosen["hub_degree"] in (128, 256, 1024) and chosen["hub_chunk"] == chosen["hub_degree"]
         for i in range(4):
-            assert g64[i] == oracle.count_f64(op, colptr, rowidx, cols[i])
+            assert _same64(g64[i], oracle.count_f64(op, colptr, rowidx, cols[i]))
             assert _rel(g32[i], oracle.count_f32(op, colptr, rowidx, cols[i])) <= TOL32
         again = gpu_ctx.count(plan, 4, colors=cols, precision="fp64")  # reuses the fp64 choice
         assert np.array_equal(again, g64)
@@ -227,12 +236,12 @@ def test_batch_tiles_agree(gpu_ctx):
         g32 = gpu_ctx.count(plan, 37, colors=cols, precision="fp32")
         if ref64 is None:
             ref64, ref32 = g64, g32
-        assert np.array_equal(g64, ref64)
+        assert _same64(g64, ref64)  # other tiles sum in other orders
         assert np.all(np.abs(g32 - ref32) <= 1e-6 * np.abs(ref64))
         assert np.all(np.abs(g32 - g64) <= TOL32 * np.abs(g64))
     gpu_ctx.set_option("batch_tile", 0)
     one = gpu_ctx.count(plan, 1, colors=cols[3:4], precision="fp64")
-    assert one[0] == ref64[3]
+    assert _same64(one[0], ref64[3])
     again = gpu_ctx.count(plan, 37, colors=cols, precision="fp32")
     assert np.array_equal(again, ref32)  # run to run: bit-identical
 
@@ -354,7 +363,9 @@ def test_isolated_vertices_and_empty_graph(gpu_ctx):
 
 def test_full_size_properties_s20(gpu_ctx):
     """s20/ef8 7-tree (the drop-in workload): color renaming leaves every count
-    unchanged (FP64, exact), FP32 within 1e-5 of FP64, hubs split or not."""
+    unchanged (FP64, same summation order: exact), FP32 within 1e-5 of FP64,
+    hubs split or not (another summation order: the raw counts are ~1e21, far
+    above 2^53, so FP64 agrees to rounding)."""
     n = 1 << 20
     s, d = sg.rmat_edges(20, 8, seed=1)
     colptr, rowidx = sg.csc_from_edges(n, s, d)
@@ -374,7 +385,10 @@ def test_full_size_properties_s20(gpu_ctx):
         nohub = gpu_ctx.count(plan, 2, colors=cols, precision="fp64")
     finally:
         gpu_ctx.set_option("hub_degree", saved)
-    assert np.array_equal(nohub, g64)
+    if np.abs(g64).max() < 2.0 ** 53:
+        assert np.array_equal(nohub, g64)
+    else:
+        assert np.allclose(nohub, g64, rtol=1e-12, atol=0.0)
 
 
 def test_mixed_batch_matches_single_counts(gpu_ctx):
@@ -412,7 +426,7 @@ def test_mixed_batch_matches_single_counts(gpu_ctx):
         multi = gpu_ctx.count_multi(plans, counts, seeds, precision="fp64")
         for i, (k, te) in enumerate(((5, u5), (7, TREE7), (10, tree10))):
             cols = sg.colors_fill(seeds[i], 0, 1, n, k)
-            assert multi[i][0] == oracle.count_f64(oracle.Plan(k, te), colptr, rowidx, cols[0])
+            assert _same64(multi[i][0], oracle.count_f64(oracle.Plan(k, te), colptr, rowidx, cols[0]))
         # the same plan twice in one batch: two branches, distinct seeds
         two = gpu_ctx.count_multi([plans[1], plans[1]], [2, 2], [21, 22], precision="fp64")
         assert np.array_equal(two[0], gpu_ctx.count(plans[1], 2, seed=21, precision="fp64"))
