@@ -72,10 +72,12 @@ struct GraphKey {
     const void* colors_in;  // caller colors buffer, nullptr = device RNG
     const void* arena;
     uint64_t generation;    // graph / weights / bucket version
+    int kind = 0;           // 0 count pass, 1 gradient pass (kept tables), 2 gradient pass (rebuilt tables)
+    const void* grad = nullptr;  // gradient accumulator of a gradient pass
     bool operator<(const GraphKey& o) const
     {
-        return std::tie(plan, B, tsize, colors_in, arena, generation) <
-               std::tie(o.plan, o.B, o.tsize, o.colors_in, o.arena, o.generation);
+        return std::tie(plan, B, tsize, colors_in, arena, generation, kind, grad) <
+               std::tie(o.plan, o.B, o.tsize, o.colors_in, o.arena, o.generation, o.kind, o.grad);
     }
 };
 
@@ -1544,15 +1546,21 @@ GradPlan plan_memory_recompute(pg_ctx* c, const pg_plan* p, int B, size_t budget
     off += align_up((size_t)c->n_act * B * maxY * sizeof(T), 256);
     g.off_dY = off;
     off += align_up((size_t)c->n_act * B * maxY * sizeof(T), 256);
-    g.off_cot = off;
-    off += align_up((size_t)B * sizeof(double), 256);
     g.off_pool = off;
+    const size_t cot_bytes = align_up((size_t)B * sizeof(double), 256);
     Recompute<T> r{c, p, B, true};
-    r.pool.cap = budget > off ? budget - off : 0;
+    r.pool.cap = budget > off + cot_bytes ? budget - off - cot_bytes : 0;
     r.sub.assign(p->nsub, nullptr);
     for (auto& d : p->subs) r.sub[d->s] = d.get();
     g.pool_bytes = r.backward(0, PoolAlloc::kNone) == PG_OK ? r.pool.peak : ~(size_t)0;
-    g.total = g.pool_bytes == ~(size_t)0 ? ~(size_t)0 : std::max(g.fwd.total, off + g.pool_bytes);
+    if (g.pool_bytes == ~(size_t)0) {
+        g.total = ~(size_t)0;
+        return g;
+    }
+    // the cotangents above everything else: they are uploaded before the
+    // pass, while the forward pass's tables overlap Y, dY and the pool
+    g.off_cot = align_up(std::max(g.fwd.total, off + g.pool_bytes), 256);
+    g.total = g.off_cot + cot_bytes;
     return g;
 }
 
@@ -1654,32 +1662,61 @@ int count_grad_impl(pg_ctx* c, const pg_plan* p, int ncol, const uint8_t* colors
             rc = set_err(c, PG_ERR_NOMEM, "adjoint plan of the last pass exceeds the first");
             break;
         }
+        // pass parameters and cotangents (slots of their own), then the
+        // forward pass and the reverse sweep: replayed from a captured graph
+        // per shape, or launched eagerly while timing or with graphs off
         const PassParams hp{seed, first};
         HIP_TRY(c, hipMemcpyAsync(base + g.fwd.off_params, &hp, sizeof hp, hipMemcpyHostToDevice, c->stream));
         std::vector<double> cot(B, 1.0);
         if (cotangent) std::copy(cotangent + first, cotangent + first + B, cot.begin());
+        HIP_TRY(c, hipMemcpyAsync(base + g.off_cot, cot.data(), sizeof(double) * B, hipMemcpyHostToDevice, c->stream));
+        auto enqueue_grad_pass = [&]() -> int {
+            int r0 = enqueue_pass<T>(c, p, B, d_in, g.fwd);
+            if (r0 == PG_OK && g.keep_all) r0 = enqueue_backward_kept<T>(c, p, B, g);
+            if (r0 == PG_OK && !g.keep_all) {
+                Recompute<T> r{c, p, B, false};
+                r.pool.cap = g.pool_bytes;
+                r.pool_base = base + g.off_pool;
+                r.pb = pass_bufs<T>(c, g.fwd);
+                r.Ybuf = (T*)(base + g.off_Y);
+                r.dYbuf = (T*)(base + g.off_dY);
+                r.cot = (const double*)(base + g.off_cot);
+                r.sub.assign(p->nsub, nullptr);
+                for (auto& d : p->subs) r.sub[d->s] = d.get();
+                r0 = r.backward(0, PoolAlloc::kNone);
+                if (r0 == PG_ERR_NOMEM) set_err(c, r0, "adjoint recompute pool exhausted");
+            }
+            return r0;
+        };
         std::vector<LaunchRec> recs;  // filled when option "timing" is on
-        c->cur_recs = &recs;
-        rc = enqueue_pass<T>(c, p, B, d_in, g.fwd);
-        // the cotangents go in after the forward pass, whose tables may share their space
-        if (rc == PG_OK && hipMemcpyAsync(base + g.off_cot, cot.data(), sizeof(double) * B, hipMemcpyHostToDevice,
-                                          c->stream) != hipSuccess)
-            rc = set_err(c, PG_ERR_HIP, "cotangent upload failed");
-        if (rc == PG_OK && g.keep_all) rc = enqueue_backward_kept<T>(c, p, B, g);
-        if (rc == PG_OK && !g.keep_all) {
-            Recompute<T> r{c, p, B, false};
-            r.pool.cap = g.pool_bytes;
-            r.pool_base = base + g.off_pool;
-            r.pb = pass_bufs<T>(c, g.fwd);
-            r.Ybuf = (T*)(base + g.off_Y);
-            r.dYbuf = (T*)(base + g.off_dY);
-            r.cot = (const double*)(base + g.off_cot);
-            r.sub.assign(p->nsub, nullptr);
-            for (auto& d : p->subs) r.sub[d->s] = d.get();
-            rc = r.backward(0, PoolAlloc::kNone);
-            if (rc == PG_ERR_NOMEM) set_err(c, rc, "adjoint recompute pool exhausted");
+        if (!c->use_graphs || c->timing) {
+            c->cur_recs = &recs;
+            rc = enqueue_grad_pass();
+            c->cur_recs = nullptr;
+        } else {
+            GraphKey key{p, B, (int)tsz, d_in, c->d_arena.p, c->generation};
+            key.kind = use_rec ? 2 : 1;
+            key.grad = c->d_grad.p;
+            auto it = c->graphs.find(key);
+            if (it == c->graphs.end()) {
+                HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+                rc = enqueue_grad_pass();
+                hipGraph_t graph = nullptr;
+                const hipError_t e = hipStreamEndCapture(c->stream, &graph);
+                if (rc == PG_OK && e != hipSuccess)
+                    rc = set_err(c, PG_ERR_HIP, "gradient graph capture failed: %s", hipGetErrorString(e));
+                if (rc == PG_OK) {
+                    PassGraph pg;
+                    const hipError_t ei = hipGraphInstantiate(&pg.exec, graph, nullptr, nullptr, 0);
+                    if (ei != hipSuccess)
+                        rc = set_err(c, PG_ERR_HIP, "graph instantiation failed: %s", hipGetErrorString(ei));
+                    else
+                        it = c->graphs.emplace(key, std::move(pg)).first;
+                }
+                if (graph) (void)hipGraphDestroy(graph);
+            }
+            if (rc == PG_OK) HIP_TRY(c, hipGraphLaunch(it->second.exec, c->stream));
         }
-        c->cur_recs = nullptr;
         if (rc) {
             destroy_recs(recs);
             break;
@@ -2065,11 +2102,16 @@ extern "C" int pg_graph_set_weights(pg_ctx* c, const float* vals, int on_device)
     if (!c) return PG_ERR_INVALID;
     if (!c->has_graph) return set_err(c, PG_ERR_STATE, "no graph uploaded");
     HIP_TRY(c, hipSetDevice(c->device));
+    // captured passes read the weights through d_w32 / d_w64: new values need
+    // no new capture, a change of those buffers or of weighted-ness does
     if (!vals) {
+        if (c->weighted) c->generation++;
         c->weighted = false;
-        c->generation++;
         return PG_OK;
     }
+    const void* old32 = c->d_w32.p;
+    const void* old64 = c->d_w64.p;
+    const bool was_weighted = c->weighted;
     const float* src = vals;
     if (!on_device) {
         HIP_TRY(c, c->d_wsrc.alloc(std::max<int64_t>(c->nnz, 1) * 4));
@@ -2082,7 +2124,7 @@ extern "C" int pg_graph_set_weights(pg_ctx* c, const float* vals, int on_device)
     HIP_TRY(c, launch_gather_weights_f64(c->d_w64.as<double>(), src, c->d_ent.as<int32_t>(), c->nnz, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     c->weighted = true;
-    c->generation++;
+    if (!was_weighted || old32 != c->d_w32.p || old64 != c->d_w64.p) c->generation++;
     return PG_OK;
 }
 

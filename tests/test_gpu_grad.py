@@ -243,3 +243,42 @@ def test_default_adjoint_mode(gpu_ctx):
     finally:
         for k, v in zip(keys, saved):
             gpu_ctx.set_option(k, v)
+
+
+def test_grad_graph_replays_match_eager(gpu_ctx):
+    """Gradient passes (forward + reverse sweep) replay from a captured graph
+    per shape: every replay equals the eager launches bit for bit, in both
+    table modes and precisions, and after new weight values (the same
+    buffers, no new capture) the count and gradient replays use them."""
+    n = 1 << 11
+    s, d = sg.rmat_edges(11, 8, seed=31)
+    colptr, rowidx = sg.csc_from_edges(n, s, d)
+    rows = np.repeat(np.arange(n), np.diff(colptr))
+    lo, hi = np.minimum(rows, rowidx), np.maximum(rows, rowidx)
+
+    def weights(salt):
+        return (0.5 + ((lo * 7919 + hi * 104729 + salt) % 1000) / 1000.0).astype(np.float32)
+
+    keys = ("graphs", "adjoint_recompute")
+    saved = [gpu_ctx.get_option(k) for k in keys]
+    try:
+        gpu_ctx.upload_graph(colptr, rowidx, vals=weights(0))
+        plan = gpu_ctx.plan(7, TREE7)
+        for mode in (0, 1):
+            gpu_ctx.set_option("adjoint_recompute", mode)
+            for prec in ("fp64", "fp32"):
+                for salt in (0, 17):
+                    gpu_ctx.set_weights(weights(salt))
+                    gpu_ctx.set_option("graphs", 0)
+                    r_e, g_e = gpu_ctx.count_grad(plan, 3, seed=5, precision=prec)
+                    c_e = gpu_ctx.count(plan, 3, seed=5, precision=prec)
+                    gpu_ctx.set_option("graphs", 1)
+                    for rep in range(3):
+                        print(f"[grad-graphs] mode={mode} {prec} salt={salt} replay {rep}", flush=True)
+                        r_g, g_g = gpu_ctx.count_grad(plan, 3, seed=5, precision=prec)
+                        assert np.array_equal(r_g, r_e) and np.array_equal(g_g, g_e)
+                        assert np.array_equal(gpu_ctx.count(plan, 3, seed=5, precision=prec), c_e)
+                    assert np.any(g_e != 0)
+    finally:
+        for k, v in zip(keys, saved):
+            gpu_ctx.set_option(k, v)
