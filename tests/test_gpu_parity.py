@@ -138,7 +138,10 @@ def test_kernel_variants_match_oracle(gpu_ctx, aux_rows, stage_rows, reg_cols, B
     try:
         for trial, (k, te) in enumerate([(7, TREE7), (5, [(0, 1), (1, 2), (2, 3), (3, 4)]),
                                          (10, [(0, 1), (1, 2), (2, 3), (3, 4), (1, 5), (5, 6), (5, 7), (7, 8),
-                                               (7, 9)])]):
+                                               (7, 9)]),
+                                         # 462-value rows: slots padded to whole chunks, wide staging
+                                         (12, [(0, 1), (1, 2), (2, 3), (3, 4), (4, 5), (2, 6), (6, 7), (7, 8),
+                                               (6, 9), (9, 10), (10, 11)])]):
             n = int(rng.integers(60, 300))
             s, d = _simple_graph(rng, n, int(rng.integers(4 * n, 10 * n)))
             colptr, rowidx = sg.csc_from_edges(n, s, d)
