@@ -71,6 +71,10 @@ const char* pg_last_error(const pg_ctx* ctx);
  *                  (the root), else four (default); 1 or 4 = that for every gather
  *   "aux_rows"     neighbour sums row-wise with LDS atomics: 1 always, 0 never (default),
  *                  -1 when most of a neighbour row is used
+ *   "vertex_order" internal vertex order set at the next pg_graph_upload: 0 = descending
+ *                  degree (default); 1 = degree classes aligned with the bucket thresholds,
+ *                  breadth-first within a class (one-coloring passes of smaller graphs gain
+ *                  a few percent, tuned batched passes of larger ones lose; results equal)
  *   "adjoint_recompute" pg_count_grad: 0 keeps every forward table for the reverse
  *                  sweep, 1 rebuilds each table from its subtree when the sweep needs it,
  *                  -1 rebuilds only when that fits four times as many colorings per pass,

@@ -139,6 +139,7 @@ struct pg_ctx {
     int64_t stage_rows = 0;     // gathers read neighbour rows staged in LDS (1) or directly (0)
     int64_t reg_cols = 0;       // values per lane of gathers: 0 one for rows under kRegCols, else 1 or 4
     int64_t adjoint_recompute = -1;  // adjoint tables: 0 kept, 1 recomputed, -1 recomputed when that fits 4x more
+    int64_t vertex_order = 0;   // internal order at the next upload: 0 descending degree, 1 degree classes + BFS
     int64_t last_tile = 0, last_recompute = 0;  // of the last count / gradient call (read-only options)
     // graph
     bool has_graph = false, weighted = false;
@@ -1916,6 +1917,9 @@ extern "C" int pg_set_option(pg_ctx* c, const char* key, int64_t v)
     } else if (k == "adjoint_recompute") {
         if (v < -1 || v > 1) return set_err(c, PG_ERR_INVALID, "adjoint_recompute must be -1, 0 or 1");
         c->adjoint_recompute = v;
+    } else if (k == "vertex_order") {
+        if (v != 0 && v != 1) return set_err(c, PG_ERR_INVALID, "vertex_order must be 0 or 1");
+        c->vertex_order = v;  // takes effect at the next graph upload
     } else if (k == "reg_cols") {
         if (v != 0 && v != 1 && v != kRegCols)
             return set_err(c, PG_ERR_INVALID, "reg_cols must be 0, 1 or %d", kRegCols);
@@ -1958,6 +1962,7 @@ extern "C" int pg_get_option(const pg_ctx* c, const char* key, int64_t* v)
     else if (k == "stage_rows") *v = c->stage_rows;
     else if (k == "reg_cols") *v = c->reg_cols;
     else if (k == "adjoint_recompute") *v = c->adjoint_recompute;
+    else if (k == "vertex_order") *v = c->vertex_order;
     else if (k == "last_tile") *v = c->last_tile;
     else if (k == "last_recompute") *v = c->last_recompute;
     else if (k == "graphs") *v = c->use_graphs ? 1 : 0;
@@ -2007,18 +2012,19 @@ extern "C" int pg_graph_upload(pg_ctx* c, int32_t n, int64_t nnz, const int32_t*
                 pos[v]++;
             }
     }
-    // Internal order: degree classes, highest first, whose boundaries are
-    // every bucket threshold the autotuner tries (so each tuned bucket is a
-    // contiguous range), and within a class breadth-first order from the
-    // largest hubs, so that vertices processed side by side share neighbour
-    // rows in L2.
+    // Internal order: descending degree (hubs first, each degree bucket an
+    // index range).  Option vertex_order = 1 instead takes degree classes,
+    // highest first, whose boundaries are every bucket threshold the
+    // autotuner tries (so each tuned bucket is still an index range), and
+    // within a class breadth-first order from the largest hubs, so that
+    // vertices processed side by side share neighbour rows in L2.
     std::vector<int32_t> perm;
     perm.reserve(n);
     for (int32_t v = 0; v < n; ++v)
         if (seen[v]) perm.push_back(v);
     std::stable_sort(perm.begin(), perm.end(), [&](int32_t a, int32_t b) { return rdeg[a] > rdeg[b]; });
     const int32_t na = (int32_t)perm.size();
-    {
+    if (c->vertex_order == 1) {
         std::vector<int32_t> bfs(n, -1);
         std::vector<int32_t> queue;
         queue.reserve(na);

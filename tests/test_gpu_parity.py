@@ -437,3 +437,45 @@ def test_mixed_batch_matches_single_counts(gpu_ctx):
     finally:
         for k, v in zip(keys, saved):
             gpu_ctx.set_option(k, v)
+
+
+def test_vertex_order_option(gpu_ctx):
+    """Option "vertex_order" (read at upload) changes only the internal order:
+    degree classes with breadth-first order inside give the counts of the
+    default descending-degree order (FP64 equal below 2^53, FP32 within the
+    tolerance), also with the hub split on and weights set."""
+    n = 1 << 12
+    s, d = sg.rmat_edges(12, 16, seed=9)
+    colptr, rowidx = sg.csc_from_edges(n, s, d)
+    cols = sg.colors_fill(4, 0, 3, n, 7)
+    rows = np.repeat(np.arange(n), np.diff(colptr))
+    lo, hi = np.minimum(rows, rowidx), np.maximum(rows, rowidx)
+    w = (0.5 + ((lo * 31 + hi * 17) % 100) / 100.0).astype(np.float32)
+    keys = ("vertex_order", "hub_degree", "hub_chunk")
+    saved = [gpu_ctx.get_option(k) for k in keys]
+    try:
+        gpu_ctx.set_option("hub_degree", 64)
+        gpu_ctx.set_option("hub_chunk", 64)
+        out = {}
+        for order in (0, 1):
+            gpu_ctx.set_option("vertex_order", order)
+            gpu_ctx.upload_graph(colptr, rowidx)
+            plan = gpu_ctx.plan(7, TREE7)
+            print(f"[vertex-order] {order}", flush=True)
+            r64 = gpu_ctx.count(plan, 3, colors=cols, precision="fp64")
+            r32 = gpu_ctx.count(plan, 3, colors=cols, precision="fp32")
+            gpu_ctx.set_weights(w)
+            w64 = gpu_ctx.count(plan, 3, colors=cols, precision="fp64")
+            gpu_ctx.set_weights(None)
+            out[order] = (r64, r32, w64)
+        assert _same64(out[1][0], out[0][0])
+        assert np.all(np.abs(out[1][1] - out[0][0]) <= TOL32 * np.abs(out[0][0]))
+        assert np.allclose(out[1][2], out[0][2], rtol=1e-12, atol=0.0)
+        op = oracle.Plan(7, TREE7)
+        assert _same64(out[1][0][0], oracle.count_f64(op, colptr, rowidx, cols[0]))
+        with pytest.raises(sg.PGError):
+            gpu_ctx.set_option("vertex_order", 2)
+    finally:
+        for k, v in zip(keys, saved):
+            gpu_ctx.set_option(k, v)
+        gpu_ctx.upload_graph(colptr, rowidx)
