@@ -184,7 +184,18 @@ def dominant_kernel(ctx, run):
         ctx.set_option("timing", 0)
     name, (ms, nbytes, nl) = max(kern.items(), key=lambda kv: kv[1][0])
     return {"kernel": name, "launches": nl, "avg_launch_ms": ms / nl,
-            "achieved_gbs": nbytes / (ms * 1e-3) / 1e9, "frac": nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+            "achieved_gbs": nbytes / (ms * 1e-3) / 1e9, "frac": nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "call_bytes": sum(v[1] for v in kern.values())}
+
+
+def call_roofline(ctx, run, seconds_per_call):
+    """Whole-call figure: the algorithmic bytes of every launch of one call
+    (recorded in an eager run with per-launch events) over the call's time
+    when replayed from its captured graphs, and the dominant kernel."""
+    dom = dominant_kernel(ctx, run)
+    gbs = dom["call_bytes"] / seconds_per_call / 1e9
+    return {"algorithmic_bytes_per_call": dom["call_bytes"], "call_gbs": gbs, "call_frac": gbs / HBM_PEAK_GBS,
+            "dominant": {k: v for k, v in dom.items() if k != "call_bytes"}}
 
 
 TUNED_KEYS = ("reg_cols", "stage_rows", "aux_rows", "bucket_high", "bucket_mid", "hub_degree", "batch_tile")
@@ -196,7 +207,8 @@ def s22_variant(sync, with_grad=True):
     the 10-vertex tree (autotuned), each beside its plain one-coloring pass,
     then an uneven mixed batch of the three templates in one call; then the
     12-vertex tree forward and forward+backward at unit and U(0.5,1.5)
-    weights, 16 colorings.  FP32 tables."""
+    weights, 16 colorings per call and one (the plain method), autotuned,
+    with the %HBM of whole calls.  FP32 tables."""
     import subgraph_amd as sg
     t0 = time.perf_counter()
     n = 1 << 22
@@ -240,7 +252,7 @@ def s22_variant(sync, with_grad=True):
     log(f"s22 mixed {counts}: {sum(counts) / t_m:.1f} colorings/s")
     if with_grad:
         import torch
-        ctx.set_option("autotune", 0)
+        ctx.set_option("autotune", 1)  # forward and gradient calls: settings tuned per (plan, tile)
         plan = ctx.plan(12, TREE12)
         rec = {}
         for wname in ("unit", "uniform_0.5_1.5"):
@@ -250,23 +262,45 @@ def s22_variant(sync, with_grad=True):
                 ctx.set_weights(hashed_weights(n, colptr, rowidx))
             t_f = _timed(lambda r=0: ctx.count(plan, 16, seed=400 + r), 1, sync)
             fwd_tile = ctx.get_option("last_tile")
+            tuned = {k: ctx.get_option(k) for k in TUNED_KEYS}
             gbuf = torch.empty(len(rowidx), dtype=torch.float32, device="cuda")  # the autograd path's output
             t_fb = _timed(lambda r=0: ctx.count_grad_device(plan, 16, gbuf.data_ptr(), seed=500 + r), 1, sync)
             rec[wname] = {"fwd_colorings_per_s": 16 / t_f, "fwd_bwd_colorings_per_s": 16 / t_fb,
                           "fwd_bwd_vs_fwd": t_f / t_fb, "fwd_colorings_per_pass": fwd_tile,
                           "fwd_bwd_colorings_per_pass": ctx.get_option("last_tile"),
-                          "adjoint_tables_recomputed": bool(ctx.get_option("last_recompute"))}
+                          "adjoint_tables_recomputed": bool(ctx.get_option("last_recompute")),
+                          "tuned_fwd": tuned}
             # the same with every table rebuilt from its subtree in the reverse sweep
             ctx.set_option("adjoint_recompute", 1)
             t_rc = _timed(lambda r=0: ctx.count_grad_device(plan, 16, gbuf.data_ptr(), seed=600 + r), 1, sync)
             rec[wname]["fwd_bwd_recompute_colorings_per_s"] = 16 / t_rc
             rec[wname]["fwd_bwd_recompute_colorings_per_pass"] = ctx.get_option("last_tile")
             ctx.set_option("adjoint_recompute", -1)
-            log(f"s22 tree12 {wname}: fwd {16 / t_f:.2f}/s ({fwd_tile}/pass), fwd+bwd {16 / t_fb:.2f}/s "
-                f"({rec[wname]['fwd_bwd_colorings_per_pass']}/pass, recompute "
-                f"{rec[wname]['adjoint_tables_recomputed']}); tables rebuilt: {16 / t_rc:.2f}/s "
-                f"({rec[wname]['fwd_bwd_recompute_colorings_per_pass']}/pass)")
-        out["tree12_fwd_bwd"] = {"colorings_per_call": 16, **rec}
+            # the plain method: one coloring per call, forward and forward+backward
+            t_pf = _timed(lambda r=0: ctx.count(plan, 1, seed=700 + r), 2, sync)
+            t_pfb = _timed(lambda r=0: ctx.count_grad_device(plan, 1, gbuf.data_ptr(), seed=800 + r), 2, sync)
+            rec[wname].update({"plain_fwd_colorings_per_s": 1 / t_pf, "plain_fwd_bwd_colorings_per_s": 1 / t_pfb,
+                               "fwd_vs_plain_fwd": t_pf * 16 / t_f, "fwd_bwd_vs_plain_fwd_bwd": t_pfb * 16 / t_fb})
+            log(f"s22 tree12 {wname}: fwd {16 / t_f:.2f}/s ({fwd_tile}/pass; plain {1 / t_pf:.2f}/s), "
+                f"fwd+bwd {16 / t_fb:.2f}/s ({rec[wname]['fwd_bwd_colorings_per_pass']}/pass, recompute "
+                f"{rec[wname]['adjoint_tables_recomputed']}; plain {1 / t_pfb:.2f}/s); tables rebuilt: "
+                f"{16 / t_rc:.2f}/s ({rec[wname]['fwd_bwd_recompute_colorings_per_pass']}/pass)")
+            if wname == "unit":
+                # %HBM of whole calls, batched and plain
+                roof = {"fwd": call_roofline(ctx, lambda: ctx.count(plan, 16, seed=401), t_f),
+                        "fwd_bwd": call_roofline(
+                            ctx, lambda: ctx.count_grad_device(plan, 16, gbuf.data_ptr(), seed=501), t_fb),
+                        "plain_fwd": call_roofline(ctx, lambda: ctx.count(plan, 1, seed=701), t_pf),
+                        "plain_fwd_bwd": call_roofline(
+                            ctx, lambda: ctx.count_grad_device(plan, 1, gbuf.data_ptr(), seed=801), t_pfb)}
+                rec[wname]["roofline"] = roof
+                log("s22 tree12 unit %HBM of whole calls: " + ", ".join(
+                    f"{k} {v['call_gbs']:.0f} GB/s ({100 * v['call_frac']:.2f}%)" for k, v in roof.items()))
+        out["tree12_fwd_bwd"] = {"colorings_per_call": 16, **rec,
+                                 "note": "roofline: algorithmic bytes of one call (forward: each table written "
+                                         "and read once, the CSC once per gathering launch; backward: the same "
+                                         "units plus the FP64 per-entry gradient read and written by each launch "
+                                         "that adds to it) over the call's graph-replayed time"}
     return out
 
 

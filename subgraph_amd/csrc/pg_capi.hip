@@ -888,7 +888,7 @@ int time_pass(pg_ctx* c, const pg_plan* p, int B, double* ms)
 }
 
 // Coordinate descent over the knobs at the largest tile B that fits, then
-// the column tile itself: B/2 and B/4 colorings per pass are timed with the
+// the column tile itself: 3B/4, B/2 and B/4 colorings per pass are timed with the
 // chosen knobs and kept when a coloring costs 2% less (wide templates can
 // run better with fewer, narrower passes).  A tile fixed by option
 // batch_tile is kept.  *tile receives the tile to run.
@@ -930,7 +930,10 @@ int autotune(pg_ctx* c, const pg_plan* p, int B, int* tile)
     if (rc == PG_OK && c->batch_tile <= 0 && best_ms < INFINITY) {
         apply_settings(c, best);
         double best_per = best_ms / B;
-        for (int t = B / 2; t >= 1 && t >= B / 4; t /= 2) {
+        int tried = B;
+        for (const int t : {B * 3 / 4, B / 2, B / 4}) {
+            if (t < 1 || t >= tried) continue;  // distinct, decreasing candidates
+            tried = t;
             double ms = INFINITY;
             rc = time_pass<T>(c, p, t, &ms);
             if (rc) break;
@@ -1345,9 +1348,16 @@ int enqueue_sub_backward(pg_ctx* c, const pg_plan* p, const DevSub& d, int B, co
     na.by_main = d.by_main.as<uint32_t>();
     na.scale = (d.scaled && fp32) ? (double)1.0e-12f : 1.0;
     const std::string node_name = std::string("k_bwd_node<") + (fp32 ? "float>" : "double>");
+    // Algorithmic bytes, in the forward's units (full-width FP32 columns of
+    // n_act x B values; Y and dY are as wide as the aux child's table; the
+    // root's G, its cotangents, counts nothing): every buffer a launch reads
+    // or writes, once, the CSC once per gathering launch, and the FP64
+    // per-entry gradient read and written by each launch that adds to it.
+    const double col = 4.0 * (double)c->n_act * B, Z = (double)c->nnz;
+    const double Cs = d.root ? 0.0 : d.Cs, Cm = d.mm > 1 ? d.Cm : 0.0, Ca = d.Ca;
     if (d.aa > 1) {
         na.part = 1;
-        record(c, node_name, 0.0, true);
+        record(c, node_name, col * (Cs + Cm + Ca), true);
         HIP_TRY(c, launch_bwd_node<T>(na, c->num_cus, st));
         record(c, "", 0, false);
         SubArgs<T> a = base_args;
@@ -1365,11 +1375,13 @@ int enqueue_sub_backward(pg_ctx* c, const pg_plan* p, const DevSub& d, int B, co
         a.dy = dYbuf;
         a.ldd = B * d.Wy;
         a.grad = c->d_grad.as<double>();
-        int rc = launch_gathering<T>(c, AUX_GATHER_G, MAIN_LEAF, a, pb.part, B * d.Wy, 0.0);
+        // reads T_aux and dY, writes Y
+        int rc = launch_gathering<T>(c, AUX_GATHER_G, MAIN_LEAF, a, pb.part, B * d.Wy,
+                                     csc_bytes(c) + col * 3.0 * Ca + 16.0 * Z);
         if (rc) return rc;
         if (na.Tm) {
             na.part = 2;
-            record(c, node_name, 0.0, true);
+            record(c, node_name, col * (Cs + Ca + Cm), true);
             HIP_TRY(c, launch_bwd_node<T>(na, c->num_cus, st));
             record(c, "", 0, false);
         }
@@ -1386,12 +1398,14 @@ int enqueue_sub_backward(pg_ctx* c, const pg_plan* p, const DevSub& d, int B, co
         t.Wy4 = d.Wa4;
         t.out = Ga;
         t.ldo = B * d.Wa;
-        rc = launch_gathering<T>(c, AUX_GATHER_T, MAIN_LEAF, t, pb.part, B * d.Wa, 0.0);
+        // reads the neighbours' dY, writes G_aux
+        rc = launch_gathering<T>(c, AUX_GATHER_T, MAIN_LEAF, t, pb.part, B * d.Wa, csc_bytes(c) + col * 2.0 * Ca);
         if (rc) return rc;
     } else {
         // aux child a leaf: Y is the histogram H; the entry's derivative is
-        // dY at the neighbour's color
-        record(c, node_name, 0.0, true);
+        // dY at the neighbour's color.  The node launch reads G and the main
+        // row, writes dY, and with a main table also reads H and writes G_main.
+        record(c, node_name, col * (Cs + 2.0 * Cm + (d.mm > 1 ? Ca : 0.0) + Ca), true);
         HIP_TRY(c, launch_bwd_node<T>(na, c->num_cus, st));
         record(c, "", 0, false);
         EdgeGradArgs<T> eg{};
@@ -1410,7 +1424,10 @@ int enqueue_sub_backward(pg_ctx* c, const pg_plan* p, const DevSub& d, int B, co
         eg.ymask = d.ymask.as<uint16_t>();
         eg.rank_tab = p->rank_tab.as<int16_t>();
         eg.grad = c->d_grad.as<double>();
-        record(c, std::string("k_edge_grad<") + (fp32 ? "float>" : "double>"), 0.0, true);
+        // entry rows and columns, the neighbour colors (a byte per entry and
+        // coloring), the vertex colors, dY, and the gradient
+        record(c, std::string("k_edge_grad<") + (fp32 ? "float>" : "double>"),
+               8.0 * Z + Z * B + (double)c->n_act * B + col * Ca + 16.0 * Z, true);
         HIP_TRY(c, launch_edge_grad<T>(eg, st));
         record(c, "", 0, false);
     }
@@ -1620,11 +1637,32 @@ int count_grad_impl(pg_ctx* c, const pg_plan* p, int ncol, const uint8_t* colors
             break;
         }
     const bool use_rec = tile_rec > 0;
-    const int tile = use_rec ? tile_rec : tile_keep;
-    const GradPlan gp = use_rec ? g_rec : g_keep;
+    int tile = use_rec ? tile_rec : tile_keep;
+    GradPlan gp = use_rec ? g_rec : g_keep;
     if (tile == 0) return set_err(c, PG_ERR_NOMEM, "adjoint tables of one coloring do not fit the device");
     c->last_tile = tile;
     c->last_recompute = use_rec ? 1 : 0;
+    if (c->autotune) {
+        // the kernel variants, degree buckets and column tile tuned on
+        // forward passes of this tile (the adjoint's gathers have the same
+        // shapes and share the buckets): a smaller tuned tile is taken over,
+        // as wide templates gather faster per coloring at a few colorings per
+        // pass in both sweeps (measured for the 12-vertex tree at R-MAT scale
+        // 22: 2.26 against 2.04 colorings/s at 3 and 5 per pass).  The plan
+        // is made again with the tuned buckets, which size the hub partial
+        // rows.
+        const size_t fwd_total = plan_memory(c, p, tile, tsz).total;
+        if (fwd_total > c->d_arena.bytes) {
+            drop_graphs(c);
+            HIP_TRY(c, c->d_arena.alloc(fwd_total));
+        }
+        int fwd_tile = tile;
+        const int rt = autotune<T>(c, p, tile, &fwd_tile);
+        if (rt) return rt;
+        if (fwd_tile >= 1 && fwd_tile < tile) tile = fwd_tile;
+        c->last_tile = tile;
+        gp = use_rec ? plan_memory_recompute<T>(c, p, tile, budget) : plan_memory_grad(c, p, tile, tsz);
+    }
     if (gp.total > c->d_arena.bytes) {
         drop_graphs(c);
         HIP_TRY(c, c->d_arena.alloc(gp.total));

@@ -170,6 +170,74 @@ def test_footprint_matches_pass_plan():
 
 
 @pytest.mark.gpu
+def test_adjoint_byte_model():
+    """The algorithmic bytes the library records per launch (option
+    "timing"), summed over a call, against the byte model worked out by hand
+    for the 7-vertex tree: the forward sweep; with kept tables the forward
+    plus the reverse sweep; with tables rebuilt, also every rebuilt
+    subtree's forward steps.  Units: full-width FP32 columns of n_act x B
+    values, the CSC once per gathering launch, the FP64 per-entry gradient
+    read and written by each launch that adds to it."""
+    from math import comb
+    s, d = sg.rmat_edges(10, 8, seed=1)
+    n, s, d = sg.remove_holes(1 << 10, s, d)
+    colptr, rowidx = sg.csc_from_edges(n, s, d)
+    ctx = sg.Context(0)
+    ctx.upload_graph(colptr, rowidx)
+    plan = ctx.plan(7, [(0, 1), (1, 2), (2, 3), (1, 4), (4, 5), (4, 6)])
+    k, B = 7, 3
+    na = ctx.stats()["n_active"]
+    Z = float(len(rowidx))
+    csc = 4.0 * (na + 1) + 4.0 * Z
+    col = 4.0 * na * B
+    Ck = [comb(k, j) for j in range(k + 1)]
+    # subtemplates (size; main, aux): R (7;1,6) root, A (6;4,2), B4 (4;1,3),
+    # C3 (3;2,1), D2 (2;1,1) = {2,3}, E2 (2;1,1) = {4,6}
+    fwd = {"R": csc + col * Ck[6], "A": csc + col * (Ck[2] + Ck[4] + Ck[6]), "B4": csc + col * (Ck[3] + Ck[4]),
+           "C3": col * (Ck[2] + Ck[3]), "D2": col * Ck[2], "E2": col * Ck[2]}
+    hist = csc + na * B
+    grad_rw = 16.0 * Z
+
+    def gathered(Ca, Cs, Cm, with_main_table):
+        """aux child a table: dY, the gradient gather (T_aux, dY in; Y out),
+        G_main when the main child is a table, the transposed gather (dY in,
+        G_aux out)"""
+        b = col * (Cs + Cm + Ca) + (csc + 3 * col * Ca + grad_rw) + (csc + 2 * col * Ca)
+        return b + (col * (Cs + Ca + Cm) if with_main_table else 0.0)
+
+    def leaf_aux(Cs, Cm):
+        """aux child a leaf: one node launch (G, main row, H in; dY, G_main
+        out), then the per-entry launch (entry rows and columns, neighbour
+        colors, vertex colors, dY, gradient)"""
+        node = col * (Cs + 2 * Cm + (k if Cm else 0) + k)
+        return node + 8 * Z + Z * B + na * B + col * k + grad_rw
+
+    bwd = {"R": gathered(Ck[6], 0, 0, False), "A": gathered(Ck[2], Ck[6], Ck[4], True),
+           "B4": gathered(Ck[3], Ck[4], 0, False), "C3": leaf_aux(Ck[3], Ck[2]),
+           "D2": leaf_aux(Ck[2], 0), "E2": leaf_aux(Ck[2], 0)}
+    subtree = {"A": ["A", "B4", "C3", "D2", "E2"], "B4": ["B4", "C3", "E2"], "C3": ["C3", "E2"],
+               "D2": ["D2"], "E2": ["E2"]}
+    rebuilt = {"R": ["A"], "A": ["B4", "D2"], "B4": ["C3"], "C3": ["E2"]}  # tables each backward step rebuilds
+    want_fwd = sum(fwd.values()) + hist
+    want_kept = want_fwd + sum(bwd.values())
+    want_rec = want_kept + sum(fwd[t] for kids in rebuilt.values() for c in kids for t in subtree[c])
+
+    def recorded():
+        return sum(v[1] for v in ctx.last_timing_kernels().values())
+
+    ctx.set_option("timing", 1)
+    ctx.count(plan, B, seed=5)
+    assert recorded() == pytest.approx(want_fwd, rel=1e-12)
+    for mode, want in ((0, want_kept), (1, want_rec)):
+        ctx.set_option("adjoint_recompute", mode)
+        ctx.count_grad(plan, B, seed=5, precision="fp32")
+        assert (ctx.get_option("last_tile"), ctx.get_option("last_recompute")) == (B, mode)
+        assert recorded() == pytest.approx(want, rel=1e-12), mode
+    ctx.set_option("adjoint_recompute", -1)
+    ctx.set_option("timing", 0)
+
+
+@pytest.mark.gpu
 def test_cli_verbose_report(tmp_path):
     """PG_VERBOSE=1: the estimates, then the count with its per-kernel device
     times and the model's bandwidth over the measured device time."""

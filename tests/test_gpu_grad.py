@@ -282,3 +282,48 @@ def test_grad_graph_replays_match_eager(gpu_ctx):
     finally:
         for k, v in zip(keys, saved):
             gpu_ctx.set_option(k, v)
+
+
+def test_autotuned_gradient(gpu_ctx):
+    """Option "autotune" = 1 on a gradient call: the kernel variants, degree
+    buckets and (when smaller) column tile tuned on forward passes of the
+    adjoint's tile are applied, and the adjoint's memory plan made again with
+    them.  Counts and gradients equal the untuned call's: exactly for unit
+    weights in FP64 (integer sums below 2^50), to FP64 rounding with
+    weights."""
+    n = 1 << 12
+    s, d = sg.rmat_edges(12, 16, seed=41)
+    colptr, rowidx = sg.csc_from_edges(n, s, d)
+    assert np.diff(colptr).max() > 1024  # hubs under every hub_degree candidate
+    rows = np.repeat(np.arange(n), np.diff(colptr))
+    lo, hi = np.minimum(rows, rowidx), np.maximum(rows, rowidx)
+    w = (0.5 + ((lo * 7919 + hi * 104729) % 1000) / 1000.0).astype(np.float32)
+    keys = ("autotune", "adjoint_recompute", "hub_degree")
+    saved = [gpu_ctx.get_option(k) for k in keys]
+    try:
+        for weighted in (False, True):
+            gpu_ctx.upload_graph(colptr, rowidx, vals=w if weighted else None)
+            for tv, te in ((7, TREE7), (10, TREE10)):
+                plan = gpu_ctx.plan(tv, te)
+                for mode in (0, 1):
+                    gpu_ctx.set_option("adjoint_recompute", mode)
+                    gpu_ctx.set_option("autotune", 0)
+                    gpu_ctx.set_option("hub_degree", 1024)  # few hub items before tuning
+                    r0, g0 = gpu_ctx.count_grad(plan, 4, seed=tv, precision="fp64")
+                    tile0 = gpu_ctx.get_option("last_tile")
+                    gpu_ctx.set_option("autotune", 1)
+                    print(f"[tuned-grad] k={tv} weighted={weighted} mode={mode}", flush=True)
+                    for rep in range(2):  # tuning, then the tuned settings recalled
+                        r1, g1 = gpu_ctx.count_grad(plan, 4, seed=tv, precision="fp64")
+                        assert 1 <= gpu_ctx.get_option("last_tile") <= tile0
+                        if not weighted and np.abs(g0).max() < 2.0 ** 50:
+                            assert np.array_equal(r1, r0) and np.array_equal(g1, g0)
+                        else:
+                            assert np.allclose(r1, r0, rtol=1e-12, atol=0.0)
+                            assert np.allclose(g1, g0, rtol=1e-12, atol=1e-9 * np.abs(g0).max())
+                    # a forward call of the same tile reuses the tuned settings
+                    c1 = gpu_ctx.count(plan, tile0, seed=tv, precision="fp64")
+                    assert np.allclose(c1, r0[:tile0], rtol=1e-12, atol=0.0)
+    finally:
+        for k, v in zip(keys, saved):
+            gpu_ctx.set_option(k, v)

@@ -184,7 +184,7 @@ def test_autotune_keeps_results(gpu_ctx):
         g64 = gpu_ctx.count(plan, 4, colors=cols, precision="fp64")
         chosen64 = {k: gpu_ctx.get_option(k) for k in keys[1:]}
         tile64 = gpu_ctx.get_option("last_tile")
-        assert tile64 in (1, 2, 4)  # the column tile is tuned too
+        assert tile64 in (1, 2, 3, 4)  # the column tile is tuned too: 3/4, 1/2, 1/4 of 4
         g32 = gpu_ctx.count(plan, 4, colors=cols, precision="fp32")  # its own key: tunes again
         chosen32 = {k: gpu_ctx.get_option(k) for k in keys[1:]}
         print(f"[autotune] chosen fp64 {chosen64} fp32 {chosen32}", flush=True)
