@@ -14,6 +14,15 @@ Printed JSON adds:
   batched       the same graph with 64 colorings per call (extra RHS columns)
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-batched]
+                    [--plain-settings KEY=VAL,...] [--autotune 0|1]
+
+The plain line runs with fixed context options (--plain-settings; by default
+the ones the library's autotuner settles on for this shape: staged neighbour
+rows, 64-lane teams from degree 64, hubs from degree 1024), so that its
+kernel mix, and the rocprofv3 summaries taken of it, do not change with the
+tuner's timing noise.  --autotune 1 tunes the plain shape instead, in one call
+before the warmup steps.  The batched line is autotuned.  config.settings
+records what ran.
 """
 from __future__ import annotations
 
@@ -113,7 +122,7 @@ def fwd_bwd_variant(sync):
     """Differentiation workload: R-MAT s18/ef16 with symmetric weights
     U(0.5,1.5), 5-star and 5-path, 16 colorings per call, forward + adjoint
     (per-entry gradient), beside the plain one-coloring forward on the same
-    graph.  FP32 tables."""
+    graph, autotuned.  FP32 tables."""
     import subgraph_amd as sg
     rng = np.random.default_rng(18)
     n = 1 << 18
@@ -127,6 +136,7 @@ def fwd_bwd_variant(sync):
     w = w_und[order[np.searchsorted(ukey[order], key)]]
     ctx = sg.Context(0)
     ctx.upload_graph(colptr, rowidx, vals=w)
+    ctx.set_option("autotune", 1)  # tuned per (plan, tile) in each measurement's untimed first call
     out = {"graph": "R-MAT scale 18 edge factor 16 (seed 1), weights U(0.5,1.5) symmetric",
            "nnz": int(len(rowidx)), "colorings_per_call": 16, "dtype": "fp32", "templates": {}}
     import torch
@@ -313,6 +323,10 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--cpu-colorings", type=int, default=12)
     ap.add_argument("--variants", default="grad,s22", help="comma list of extra measurements: grad, s22, none")
+    ap.add_argument("--plain-settings", default="stage_rows=1,bucket_high=64,hub_degree=1024,hub_chunk=1024",
+                    metavar="KEY=VAL,...", help="context options of the plain line ('' = library defaults)")
+    ap.add_argument("--autotune", type=int, default=0, choices=(0, 1),
+                    help="1: the autotuner picks the plain line's settings, in a call before the warmup steps")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run one elementwise kernel that reads and writes MIB MiB (the known-size "
                          "kernel rocprofv3 FETCH_SIZE / WRITE_SIZE readings are scaled by)")
@@ -342,6 +356,9 @@ def main():
     log(f"graph: n={n} undirected={len(src)} nnz={len(rowidx)} ({time.perf_counter() - t0:.1f}s)")
 
     ctx = sg.Context(0)
+    for kv in filter(None, args.plain_settings.split(",")):
+        key, val = kv.split("=")
+        ctx.set_option(key.strip(), int(val))
     ctx.upload_graph(colptr, rowidx)
     st = ctx.stats()
     plan = ctx.plan(7, TREE7)
@@ -353,6 +370,15 @@ def main():
     except Exception:  # torch is plumbing only
         sync = lambda: None  # noqa: E731
 
+    tuning_s = 0.0
+    if args.autotune:
+        ctx.set_option("autotune", 1)
+        t0 = time.perf_counter()
+        ctx.count(plan, 1, seed=999)  # tunes this shape once, before the warmup and the timed region
+        sync()
+        tuning_s = time.perf_counter() - t0
+    settings = {k: ctx.get_option(k) for k in TUNED_KEYS + ("hub_chunk",)}
+    log(f"plain settings ({'autotuned in %.1fs' % tuning_s if args.autotune else 'as set'}): {settings}")
     for w in range(args.warmup):
         ctx.count(plan, 1, seed=1000 + w)
     # timed region: the product path (each pass replays its captured hipGraph)
@@ -395,7 +421,8 @@ def main():
         "config": {"workload": "plain_rmat_s20_ef8_tree7",
                    "graph": "R-MAT scale 20 edge factor 8 (a=.57 b=.19 c=.19 seed 1), dedup, symmetrized",
                    "template": "7-vertex tree 0-1,1-2,2-3,1-4,4-5,4-6",
-                   "colorings_per_step": 1, "n": int(n), "n_active": st["n_active"], "nnz": int(len(rowidx))},
+                   "colorings_per_step": 1, "n": int(n), "n_active": st["n_active"], "nnz": int(len(rowidx)),
+                   "settings": settings, "autotuned": bool(args.autotune), "tuning_seconds": tuning_s},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(kern), "kernel": kern,
                      "launches": klaunch, "avg_launch_ms": kms / klaunch,
@@ -424,8 +451,13 @@ def main():
 
     if not args.no_batched:
         B = 64
-        ctx.count(plan, B, seed=7)
+        ctx.set_option("autotune", 1)
+        t0 = time.perf_counter()
+        ctx.count(plan, B, seed=7)  # tunes this shape (untimed)
         sync()
+        btune_s = time.perf_counter() - t0
+        bsettings = {k: ctx.get_option(k) for k in TUNED_KEYS}
+        bsettings["colorings_per_pass"] = ctx.get_option("last_tile")
         t0 = time.perf_counter()
         reps = 3
         for r in range(reps):
@@ -434,7 +466,8 @@ def main():
         bel = time.perf_counter() - t0
         bval = reps * B / bel
         result["batched"] = {"colorings_per_call": B, "value": bval, "unit": "colorings/s",
-                             "vs_plain": bval / value, "workload": "same graph and template as the plain line"}
+                             "vs_plain": bval / value, "workload": "same graph and template as the plain line",
+                             "settings": bsettings, "tuning_seconds": btune_s}
         log(f"batched: {bval:.1f} colorings/s ({bval / value:.2f}x plain)")
 
 

@@ -60,8 +60,9 @@ const char* pg_last_error(const pg_ctx* ctx);
  *   "autotune"     1 = before the first pass of each (plan, batch tile, precision), time
  *                  one pass per candidate of reg_cols, stage_rows, aux_rows, bucket_high,
  *                  bucket_mid and hub_degree (hub_chunk follows hub_degree) and keep
- *                  the fastest, then the column tile: 3/4, 1/2 and 1/4 of the colorings per
- *                  pass, kept when a coloring costs less (unless batch_tile fixes it);
+ *                  the fastest (a second sweep if the first changed one), then the
+ *                  column tile: 3/4, 1/2 and 1/4 of the colorings per pass, kept when
+ *                  a coloring costs less (unless batch_tile fixes it);
  *                  pg_get_option then reports the choice, "last_tile" the tile.  A
  *                  gradient call applies what forward passes of its own tile tuned,
  *                  including a smaller tile (default 0)

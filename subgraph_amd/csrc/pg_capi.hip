@@ -813,8 +813,9 @@ int run_pass(pg_ctx* c, const pg_plan* p, int B, int64_t first, const uint8_t* d
 //
 // Option "autotune" = 1: before the first pass of a (plan, batch tile,
 // precision) the library times one pass per candidate setting of the kernel
-// variant and degree-bucket knobs (coordinate descent, one sweep over the
-// knobs below, each candidate kept only if at least 2% faster), applies the
+// variant and degree-bucket knobs (coordinate descent over the knobs below,
+// a second sweep when the first changed one, each candidate kept only if at
+// least 2% faster), applies the
 // fastest and remembers it for that key.  Tuning passes draw their colorings
 // on the device from a fixed seed and discard the counts.
 
@@ -911,20 +912,29 @@ int autotune(pg_ctx* c, const pg_plan* p, int B, int* tile)
     double best_ms = INFINITY;
     int rc = time_pass<T>(c, p, B, &best_ms);
     const auto& knobs = tune_knobs();
-    for (size_t i = 0; i < knobs.size() && rc == PG_OK; ++i) {
-        for (int64_t v : knobs[i].values) {
-            if (v == best[i]) continue;
-            std::vector<int64_t> cand = best;
-            cand[i] = v;
-            apply_settings(c, cand);
-            double ms = INFINITY;
-            rc = time_pass<T>(c, p, B, &ms);
-            if (rc) break;
-            if (ms < 0.98 * best_ms) {
-                best = cand;
-                best_ms = ms;
+    // a second sweep when the first changed a knob: the variant knobs come
+    // first, and their best value can depend on the bucket knobs after them
+    // (R-MAT scale 20, 7-vertex tree, one coloring per pass: staged rows pay
+    // only with 64-lane teams from degree 64 and hubs from degree 1024)
+    for (int sweep = 0; sweep < 2 && rc == PG_OK; ++sweep) {
+        bool changed = false;
+        for (size_t i = 0; i < knobs.size() && rc == PG_OK; ++i) {
+            for (int64_t v : knobs[i].values) {
+                if (v == best[i]) continue;
+                std::vector<int64_t> cand = best;
+                cand[i] = v;
+                apply_settings(c, cand);
+                double ms = INFINITY;
+                rc = time_pass<T>(c, p, B, &ms);
+                if (rc) break;
+                if (ms < 0.98 * best_ms) {
+                    best = cand;
+                    best_ms = ms;
+                    changed = true;
+                }
             }
         }
+        if (!changed) break;
     }
     int best_tile = B;
     if (rc == PG_OK && c->batch_tile <= 0 && best_ms < INFINITY) {
