@@ -314,6 +314,50 @@ def s22_variant(sync, with_grad=True):
     return out
 
 
+def copy_calibration(mib):
+    """One elementwise kernel of known traffic: reads and writes `mib` MiB."""
+    import torch
+    src_t = torch.ones(mib << 18, dtype=torch.float32, device="cuda")
+    dst_t = torch.empty_like(src_t)
+    torch.add(src_t, 1.0, out=dst_t)
+    torch.cuda.synchronize()
+    log(f"copy calibration: {mib} MiB read + {mib} MiB written")
+
+
+# the settings the autotuner picks for the 12-vertex tree in the s22 measurement
+TREE12_SETTINGS = {"stage_rows": 1, "bucket_high": 32, "hub_degree": 1024, "hub_chunk": 1024}
+
+
+def tree12_variant(sync):
+    """The 12-vertex tree on R-MAT s22/ef16 alone, unit weights, with fixed
+    settings (TREE12_SETTINGS; 4 colorings per forward pass, 3 per adjoint
+    pass, as tuned): 16 colorings forward, 16 forward+backward, and one of
+    each (the plain method).  A short run with no tuning passes, for
+    profilers.  FP32 tables."""
+    import torch
+    import subgraph_amd as sg
+    n = 1 << 22
+    src, dst = sg.rmat_edges(22, 16, seed=1)
+    colptr, rowidx = sg.csc_from_edges(n, src, dst)
+    del src, dst
+    ctx = sg.Context(0, **TREE12_SETTINGS)
+    ctx.upload_graph(colptr, rowidx)
+    plan = ctx.plan(12, TREE12)
+    gbuf = torch.empty(len(rowidx), dtype=torch.float32, device="cuda")
+    out = {"graph": "R-MAT scale 22 edge factor 16 (a=.57 b=.19 c=.19 seed 1), dedup, symmetrized",
+           "template": "12-vertex tree", "settings": dict(TREE12_SETTINGS)}
+    for name, tile, N, fn in (
+            ("fwd", 4, 16, lambda r=0: ctx.count(plan, 16, seed=400 + r)),
+            ("fwd_bwd", 3, 16, lambda r=0: ctx.count_grad_device(plan, 16, gbuf.data_ptr(), seed=500 + r)),
+            ("plain_fwd", 1, 1, lambda r=0: ctx.count(plan, 1, seed=700 + r)),
+            ("plain_fwd_bwd", 1, 1, lambda r=0: ctx.count_grad_device(plan, 1, gbuf.data_ptr(), seed=800 + r))):
+        ctx.set_option("batch_tile", tile)
+        t = _timed(fn, 1 if N > 1 else 2, sync)
+        out[name] = {"colorings_per_s": N / t, "colorings_per_pass": ctx.get_option("last_tile")}
+        log(f"tree12 {name}: {N / t:.2f} colorings/s ({ctx.get_option('last_tile')}/pass)")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -322,9 +366,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--cpu-colorings", type=int, default=12)
-    ap.add_argument("--variants", default="grad,s22", help="comma list of extra measurements: grad, s22, none")
+    ap.add_argument("--variants", default="grad,s22",
+                    help="comma list of extra measurements: grad, s22, tree12 (the s22 12-tree part alone), none")
     ap.add_argument("--plain-settings", default="stage_rows=1,bucket_high=64,hub_degree=1024,hub_chunk=1024",
                     metavar="KEY=VAL,...", help="context options of the plain line ('' = library defaults)")
+    ap.add_argument("--variants-only", action="store_true",
+                    help="run only the --variants measurements (a short run for profilers; no primary line)")
     ap.add_argument("--autotune", type=int, default=0, choices=(0, 1),
                     help="1: the autotuner picks the plain line's settings, in a call before the warmup steps")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -338,14 +385,18 @@ def main():
 
     import subgraph_amd as sg
 
-    if args.copy_calibration > 0:
+    if args.variants_only:
         import torch
-        src_t = torch.ones(args.copy_calibration << 18, dtype=torch.float32, device="cuda")
-        dst_t = torch.empty_like(src_t)
-        torch.add(src_t, 1.0, out=dst_t)  # reads and writes copy_calibration MiB
-        torch.cuda.synchronize()
-        log(f"copy calibration: {args.copy_calibration} MiB read + {args.copy_calibration} MiB written")
-        del src_t, dst_t
+        result = {"metric": "colorings/sec + %HBM roofline; batched and fwd+bwd vs plain DP, same GPU",
+                  "note": "--variants-only: extra measurements only, no primary line"}
+        if args.copy_calibration > 0:
+            copy_calibration(args.copy_calibration)
+        run_variants(result, args.variants.split(","), torch.cuda.synchronize)
+        print(json.dumps(result), flush=True)
+        return
+
+    if args.copy_calibration > 0:
+        copy_calibration(args.copy_calibration)
 
     scale, ef = 20, 8
     t0 = time.perf_counter()
@@ -487,10 +538,17 @@ def main():
             result["cpu_baseline"] = {"value": None, "unit": "colorings/s", "cores": physical_cores(),
                                       "kind": "reference", "sample": f"failed: {e}"}
     variants = args.variants.split(",")
-    if "grad" in variants or "s22" in variants:
+    if "grad" in variants or "s22" in variants or "tree12" in variants:
         del ctx, plan
-    # the extra measurements never cost the primary line: a failure is recorded
-    for name, key, fn in (("grad", "fwd_bwd", fwd_bwd_variant), ("s22", "s22", s22_variant)):
+    run_variants(result, variants, sync)
+    print(json.dumps(result), flush=True)
+
+
+def run_variants(result, variants, sync):
+    """The extra measurements, each under its key of `result`; they never
+    cost the primary line: a failure is recorded."""
+    for name, key, fn in (("grad", "fwd_bwd", fwd_bwd_variant), ("s22", "s22", s22_variant),
+                          ("tree12", "s22_tree12", tree12_variant)):
         if name not in variants:
             continue
         try:
@@ -498,7 +556,6 @@ def main():
         except Exception as e:  # noqa: BLE001
             log(f"variant {name} failed: {e!r}")
             result[key] = {"error": repr(e)}
-    print(json.dumps(result), flush=True)
 
 
 if __name__ == "__main__":
