@@ -103,6 +103,8 @@ struct DevSub {
     int Pa = 0, Pm = 0;
     DevBuf by_aux;   // [Wy][Pa] uint32 (o << 16 | main rank)
     DevBuf by_main;  // [Wm][Pm] uint32 (o << 16 | y rank)
+    DevBuf by_auxT;  // the same lists split-major: [Pa][Wy], [Pm][Wm]
+    DevBuf by_mainT;
     DevBuf amask;    // [Wa] uint16
     // gather index table of AUX_GATHER (SubArgs::gidx), when it fits kGidxMaxBytes
     DevBuf gidx;     // [(k+1)*k][Wy4] int16
@@ -1356,6 +1358,9 @@ int enqueue_sub_backward(pg_ctx* c, const pg_plan* p, const DevSub& d, int B, co
     na.Pm = d.Pm;
     na.by_aux = d.by_aux.as<uint32_t>();
     na.by_main = d.by_main.as<uint32_t>();
+    na.by_auxT = d.by_auxT.as<uint32_t>();
+    na.by_mainT = d.by_mainT.as<uint32_t>();
+    na.P = d.P;
     na.scale = (d.scaled && fp32) ? (double)1.0e-12f : 1.0;
     const std::string node_name = std::string("k_bwd_node<") + (fp32 ? "float>" : "double>");
     // Algorithmic bytes, in the forward's units (full-width FP32 columns of
@@ -2347,7 +2352,9 @@ extern "C" int pg_plan_create(pg_ctx* c, int k, int nsub, const int32_t* sub_siz
             HIP_TRY(c, d->split.alloc(tab.size() * 4));
             HIP_TRY(c, hipMemcpy(d->split.p, tab.data(), tab.size() * 4, hipMemcpyHostToDevice));
             // adjoint lists: every (output, split) pair filed under its aux
-            // set and under its main set, in output order (equal lengths)
+            // set and under its main set, in output order (equal lengths);
+            // set-major for the flat adjoint kernel, and split-major (threads
+            // on consecutive sets read consecutive words) for the staged one
             std::vector<std::vector<uint32_t>> by_a(d->Wy), by_m(d->Wm);
             for (int o = 0; o < d->Wo; ++o)
                 for (int j = 0; j < d->P; ++j) {
@@ -2357,19 +2364,25 @@ extern "C" int pg_plan_create(pg_ctx* c, int k, int nsub, const int32_t* sub_siz
                 }
             d->Pa = (int)by_a[0].size();
             d->Pm = (int)by_m[0].size();
-            std::vector<uint32_t> fa, fm;
-            for (auto& l : by_a) {
+            for (auto& l : by_a)
                 if ((int)l.size() != d->Pa) return set_err(c, PG_ERR_INVALID, "uneven adjoint split lists");
-                fa.insert(fa.end(), l.begin(), l.end());
-            }
-            for (auto& l : by_m) {
+            for (auto& l : by_m)
                 if ((int)l.size() != d->Pm) return set_err(c, PG_ERR_INVALID, "uneven adjoint split lists");
-                fm.insert(fm.end(), l.begin(), l.end());
+            std::vector<uint32_t> fa, fm, fat((size_t)d->Pa * d->Wy), fmt((size_t)d->Pm * d->Wm);
+            for (int x = 0; x < d->Wy; ++x) {
+                fa.insert(fa.end(), by_a[x].begin(), by_a[x].end());
+                for (int j = 0; j < d->Pa; ++j) fat[(size_t)j * d->Wy + x] = by_a[x][j];
             }
-            HIP_TRY(c, d->by_aux.alloc(fa.size() * 4));
-            HIP_TRY(c, hipMemcpy(d->by_aux.p, fa.data(), fa.size() * 4, hipMemcpyHostToDevice));
-            HIP_TRY(c, d->by_main.alloc(fm.size() * 4));
-            HIP_TRY(c, hipMemcpy(d->by_main.p, fm.data(), fm.size() * 4, hipMemcpyHostToDevice));
+            for (int x = 0; x < d->Wm; ++x) {
+                fm.insert(fm.end(), by_m[x].begin(), by_m[x].end());
+                for (int j = 0; j < d->Pm; ++j) fmt[(size_t)j * d->Wm + x] = by_m[x][j];
+            }
+            const std::pair<DevBuf*, const std::vector<uint32_t>*> lists[] = {
+                {&d->by_aux, &fa}, {&d->by_main, &fm}, {&d->by_auxT, &fat}, {&d->by_mainT, &fmt}};
+            for (const auto& l : lists) {
+                HIP_TRY(c, l.first->alloc(l.second->size() * 4));
+                HIP_TRY(c, hipMemcpy(l.first->p, l.second->data(), l.second->size() * 4, hipMemcpyHostToDevice));
+            }
         }
         if (d->aa > 1) {
             const std::vector<uint16_t> ym = masks_of_size(K, d->aa);
@@ -2466,8 +2479,8 @@ extern "C" int pg_plan_footprint(pg_ctx* c, const pg_plan* p, int B, int precisi
         for (const DevBuf* g : graph_bufs)
             b += g->bytes;
         for (auto& d : p->subs)
-            for (const DevBuf* t : {&d->split, &d->ymask, &d->by_aux, &d->by_main, &d->amask, &d->gidx, &d->xidx,
-                                    &d->gidxT})
+            for (const DevBuf* t : {&d->split, &d->ymask, &d->by_aux, &d->by_main, &d->by_auxT, &d->by_mainT,
+                                    &d->amask, &d->gidx, &d->xidx, &d->gidxT})
                 b += t->bytes;
         *device_bytes = (int64_t)b;
     }

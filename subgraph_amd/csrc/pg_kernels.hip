@@ -23,6 +23,7 @@
 // DataTableColMajor.cpp:367 and :396 do.
 #include "pg_kernels.hpp"
 
+#include <algorithm>
 #include <map>
 
 namespace pgbsc {
@@ -882,14 +883,105 @@ __global__ void k_bwd_node(const NodeArgs<T> a)
     FLAT_LOOP(i, hi - lo) bwd_node_item(a, lo + i, ny);
 }
 
+// The same values with the rows staged in LDS: one workgroup per group of
+// `vpb` consecutive (vertex, coloring) rows copies their G row, main row (read
+// by the dY values) and Y row (read by the G_main values) into LDS with
+// coalesced loads, then every thread forms values from LDS.  The split lists
+// stay in global memory, where every workgroup reads the same few KB through
+// the cache.  Per value the same FP64 sum in the same order as bwd_node_item,
+// so the results are identical.
 template <typename T>
-hipError_t launch_bwd_node(const NodeArgs<T>& a, int, hipStream_t st)
+__global__ __launch_bounds__(256) void k_bwd_node_lds(const NodeArgs<T> a, int vpb)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const bool do_dy = a.part != 2;
+    const bool do_gm = a.part != 1 && a.Tm != nullptr;
+    const int wg = a.G ? a.Wo : 0;
+    const int wm = (do_dy && a.Tm) ? a.Wm : 0;
+    const int yw = a.y_from_h ? a.k : a.Wy;  // Y row length in global memory
+    const int wy = do_gm ? yw : 0;
+    T* sG = reinterpret_cast<T*>(smem_raw);  // [vpb][wg]
+    T* sM = sG + (size_t)vpb * wg;           // [vpb][wm]
+    T* sY = sM + (size_t)vpb * wm;           // [vpb][wy]
+    const int ny = do_dy ? a.Wy : 0;
+    const int per = ny + (do_gm ? a.Wm : 0);
+    const int64_t nvb = (int64_t)a.n_act * a.B;
+    const int64_t ngroups = (nvb + vpb - 1) / vpb;
+    for (int64_t g = blockIdx.x; g < ngroups; g += gridDim.x) {
+        const int64_t vb0 = g * vpb;
+        const int nv = (int)(nvb - vb0 < vpb ? nvb - vb0 : vpb);
+        __syncthreads();  // the previous group's reads are done
+        for (int i = threadIdx.x; i < nv * wg; i += blockDim.x) sG[i] = a.G[vb0 * wg + i];
+        for (int i = threadIdx.x; i < nv * wm; i += blockDim.x) sM[i] = a.Tm[vb0 * wm + i];
+        for (int i = threadIdx.x; i < nv * wy; i += blockDim.x) sY[i] = a.Y[vb0 * yw + i];
+        __syncthreads();
+        for (int it = threadIdx.x; it < nv * per; it += blockDim.x) {
+            const int lv = it / per;
+            const int x0 = it - lv * per;
+            const int64_t vb = vb0 + lv;
+            const int b = (int)(vb % a.B);
+            const T* G = wg ? sG + lv * wg : nullptr;
+            const double g0 = a.root_cot[b];
+            double acc = 0.0;
+            if (x0 < ny) {
+                const int x = x0;
+                if (!a.Tm) {
+                    acc = G ? (double)G[x] : g0;
+                } else {
+                    const T* m = sM + lv * wm;
+                    for (int p = 0; p < a.Pa; ++p) {
+                        const uint32_t e = a.by_auxT[(size_t)p * a.Wy + x];
+                        acc += (G ? (double)G[e >> 16] : g0) * (double)m[e & 0xffffu];
+                    }
+                }
+                a.dY[vb * a.Wy + x] = (T)(acc * a.scale);
+            } else {
+                const int x = x0 - ny;
+                const int cv = a.colors[vb];
+                const T* y = sY + lv * wy;
+                for (int p = 0; p < a.Pm; ++p) {
+                    const uint32_t e = a.by_mainT[(size_t)p * a.Wm + x];
+                    const int ra = (int)(e & 0xffffu);
+                    const double yv = a.y_from_h ? (double)y[ra + (ra >= cv ? 1 : 0)] : (double)y[ra];
+                    acc += (G ? (double)G[e >> 16] : g0) * yv;
+                }
+                a.Gm[vb * a.Wm + x] = (T)(acc * a.scale);
+            }
+        }
+    }
+}
+
+// LDS per workgroup of the staged kernel, the largest row set it takes, and
+// the split reads per staged value from which it pays.  Measured in FP32,
+// forward+backward with every main-table launch staged: the 12-vertex tree
+// at R-MAT scale 22 (7.4-21 reads per staged value) ran 8% faster, the
+// 10-vertex tree (2.6-5.4) 4% slower, the 5-star at scale 18 (0.9-1.6) 16%
+// slower.
+constexpr size_t kBwdLdsBudget = 32 * 1024;
+constexpr size_t kBwdLdsMax = 48 * 1024;
+constexpr int kBwdStageReuse = 6;
+
+template <typename T>
+hipError_t launch_bwd_node(const NodeArgs<T>& a, int num_cus, hipStream_t st)
 {
     const int64_t ny = (int64_t)a.n_act * a.B * a.Wy;
     const int64_t nm = a.Tm ? (int64_t)a.n_act * a.B * a.Wm : 0;
     const int64_t total = a.part == 1 ? ny : a.part == 2 ? nm : ny + nm;
     if (total == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_bwd_node<T>, flat_grid(total), dim3(256), 0, st, a);
+    const bool do_dy = a.part != 2, do_gm = a.part != 1 && a.Tm != nullptr;
+    const size_t row = sizeof(T) * ((a.G ? a.Wo : 0) + ((do_dy && a.Tm) ? a.Wm : 0) +
+                                    (do_gm ? (a.y_from_h ? a.k : a.Wy) : 0));
+    // split reads per vertex-coloring: Wo*P each for dY and for G_main (only
+    // main tables have split lists)
+    const int64_t reads = a.Tm ? (int64_t)a.Wo * a.P * ((do_dy ? 1 : 0) + (do_gm ? 1 : 0)) : 0;
+    if (row > kBwdLdsMax || reads < (int64_t)kBwdStageReuse * (int64_t)(row / sizeof(T))) {
+        hipLaunchKernelGGL(k_bwd_node<T>, flat_grid(total), dim3(256), 0, st, a);
+        return hipGetLastError();
+    }
+    const int vpb = (int)std::max<size_t>(1, std::min<size_t>(64, kBwdLdsBudget / std::max<size_t>(row, 1)));
+    const int64_t ngroups = ((int64_t)a.n_act * a.B + vpb - 1) / vpb;
+    const unsigned grid = (unsigned)std::min<int64_t>(ngroups, (int64_t)std::max(num_cus, 1) * 8);
+    hipLaunchKernelGGL(k_bwd_node_lds<T>, dim3(grid), dim3(256), (size_t)vpb * row, st, a, vpb);
     return hipGetLastError();
 }
 

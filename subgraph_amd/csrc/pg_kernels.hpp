@@ -173,8 +173,11 @@ struct NodeArgs {
     T* dY;                   // [n_act][B*Wy]
     T* Gm;                   // [n_act][B*Wm] (main not a leaf)
     int32_t Wo, Wy, Wm, Pa, Pm;
+    int32_t P;                // splits per output value
     const uint32_t* by_aux;  // [Wy][Pa] (o << 16 | main rank)
     const uint32_t* by_main; // [Wm][Pm] (o << 16 | y rank)
+    const uint32_t* by_auxT; // the same, split-major: [Pa][Wy]
+    const uint32_t* by_mainT;// [Pm][Wm]
     double scale;            // 1, or float(1e-12) for the scaled subtemplate
     int32_t part;            // 0: dY and G_main, 1: dY only, 2: G_main only
 };

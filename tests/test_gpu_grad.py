@@ -327,3 +327,36 @@ def test_autotuned_gradient(gpu_ctx):
     finally:
         for k, v in zip(keys, saved):
             gpu_ctx.set_option(k, v)
+
+
+@pytest.mark.parametrize("tv,te", [(10, TREE10), (12, TREE12)])
+def test_wide_template_grad_fd(gpu_ctx, tv, te):
+    """The wide templates' adjoint (main-table steps whose splits read each
+    row value many times take the LDS-staged node kernel) against FP64
+    central differences of the oracle's weighted count (affine in each entry
+    weight), and the forward against the oracle, on a small dense graph."""
+    rng = np.random.default_rng(tv + 100)
+    n = 48
+    s, d = _simple_graph(rng, n, 420)
+    colptr, rowidx = sg.csc_from_edges(n, s, d)
+    rows = np.repeat(np.arange(n), np.diff(colptr))
+    lo, hi = np.minimum(rows, rowidx), np.maximum(rows, rowidx)
+    w = (0.5 + ((lo * 7919 + hi * 104729) % 1000) / 1000.0).astype(np.float32).astype(np.float64)  # symmetric
+    gpu_ctx.upload_graph(colptr, rowidx, vals=w.astype(np.float32))
+    plan = gpu_ctx.plan(tv, te)
+    op = oracle.Plan(tv, te)
+    cols = rng.integers(0, tv, size=(1, n)).astype(np.uint8)
+    print(f"[wide-fd] k={tv} n={n} nnz={len(rowidx)}", flush=True)
+    raw, g = gpu_ctx.count_grad(plan, 1, colors=cols, precision="fp64")
+    want = oracle.count_f64(op, colptr, rowidx, cols[0], vals=w)
+    assert want > 0 and abs(raw[0] - want) <= 1e-12 * want
+    assert np.count_nonzero(g) > 0
+    h = 1e-3
+    picks = np.concatenate([rng.choice(np.flatnonzero(g), 6, replace=False), rng.choice(len(rowidx), 4, replace=False)])
+    for e in picks:
+        wp, wm = w.copy(), w.copy()
+        wp[e] += h
+        wm[e] -= h
+        fd = (oracle.count_f64(op, colptr, rowidx, cols[0], vals=wp)
+              - oracle.count_f64(op, colptr, rowidx, cols[0], vals=wm)) / (2 * h)
+        assert abs(fd - g[e]) <= 1e-7 * max(1.0, abs(g).max()), (e, fd, g[e])
