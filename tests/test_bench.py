@@ -1,0 +1,82 @@
+"""bench.py's contract, on the CPU: the one-GPU guard, the hashed weights,
+and the committed JSON line (profiles/bench_default.json) against the keys
+the line must carry and the rocprofv3 summaries it cites."""
+import json
+import os
+import re
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+import bench  # noqa: E402
+
+LINE = os.path.join(ROOT, "profiles", "bench_default.json")
+TRACE_MD = os.path.join(ROOT, "profiles", "k_sub_plain_rmat_s20_ef8_tree7.md")
+
+
+def test_more_than_one_gpu_is_refused():
+    out = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                         text=True, timeout=120)
+    assert out.returncode == 2
+    assert "one GPU" in json.loads(out.stdout.strip().splitlines()[-1])["error"]
+
+
+def test_hashed_weights_symmetric_and_in_range():
+    import subgraph_amd as sg
+    n = 1 << 10
+    s, d = sg.rmat_edges(10, 8, seed=3)
+    colptr, rowidx = sg.csc_from_edges(n, s, d)
+    w = bench.hashed_weights(n, colptr, rowidx)
+    assert w.dtype == np.float32 and len(w) == len(rowidx)
+    assert w.min() >= 0.5 and w.max() < 1.5
+    rows = np.repeat(np.arange(n), np.diff(colptr))
+    key = dict(((int(r), int(c)), float(x)) for r, c, x in zip(rows, rowidx, w))
+    assert all(key[(c, r)] == x for (r, c), x in key.items())  # both entries of an edge agree
+    assert len(np.unique(w)) > len(w) // 4  # not degenerate
+
+
+@pytest.fixture(scope="module")
+def line():
+    with open(LINE) as f:
+        return json.load(f)
+
+
+def test_committed_line_keeps_the_contract(line):
+    for k in ("metric", "value", "unit", "n_gpus", "steps", "warmup", "ms_per_step", "higher_is_better", "scaling",
+              "vs_baseline", "dtype", "data", "config", "roofline", "cpu_baseline"):
+        assert k in line, k
+    assert line["n_gpus"] == 1 and line["higher_is_better"] is True and line["vs_baseline"] is None
+    assert line["unit"] == "colorings/s" and line["dtype"] == "fp32" and line["config"]["workload"]
+    assert line["value"] == pytest.approx(1e3 / line["ms_per_step"], rel=1e-9)
+    r = line["roofline"]
+    assert r["bound"] == "hbm" and r["unit"] == "GB/s" and r["peak"] == 8000.0
+    assert r["frac"] == pytest.approx(r["achieved"] / r["peak"], rel=1e-12)
+    assert r["achieved"] == pytest.approx(r["algorithmic_bytes_per_launch"] / (r["avg_launch_ms"] * 1e-3) / 1e9,
+                                          rel=1e-9)
+    # traffic comes from the committed calibrated counter runs of that kernel
+    assert r["traffic"] == bench.measured_traffic(r["kernel"]) and r["traffic"] > 0
+    cb = line["cpu_baseline"]
+    assert cb["kind"] in ("reference", "port") and cb["cores"] >= 1 and cb["value"] > 0 and cb["sample"]
+    assert cb["parity_max_rel_err"] <= 1e-5
+
+
+def test_trace_summary_agrees_with_the_line(line):
+    """The committed rocprofv3 kernel-trace summary of the plain workload and
+    the line's live HIP-event figure name the same dominant kernel, and its
+    average duration agrees within 5%."""
+    kern = line["roofline"]["kernel"]
+    text = open(TRACE_MD).read()
+    rows = [ln for ln in text.splitlines() if ln.startswith(f"| `{kern}` |")]
+    assert len(rows) == 1, kern
+    avg_us = float(rows[0].split("|")[3])
+    assert avg_us / 1e3 == pytest.approx(line["roofline"]["avg_launch_ms"], rel=0.05)
+    # the dominant kernel of the trace table (first row by total time) is the line's
+    first = next(ln for ln in text.splitlines() if re.match(r"\| `k_", ln))
+    assert first.startswith(f"| `{kern}` |")
+    # settings the summary names are the line's plain settings
+    s = line["config"]["settings"]
+    assert f"stage_rows={s['stage_rows']}, bucket_high={s['bucket_high']}, hub_degree={s['hub_degree']}" in text
