@@ -473,7 +473,12 @@ __global__ __launch_bounds__(kBlock) void k_sub(const SubArgs<T> a)
                             // The 4 neighbours' dots reduce together (fixed order):
                             // xor 1 and xor 2 halve the values a lane carries, so lane
                             // bits (0,1) end up naming the neighbour 2*b0 + b1, then
-                            // the higher xor steps sum across the group.
+                            // the higher xor steps sum across the group.  The FP64 add
+                            // to the entry's gradient is a hardware atomic that returns
+                            // nothing, so no wait for it holds up the rows already in
+                            // flight; an entry has one owning lane, whose adds (one per
+                            // column chunk) land in program order, as a plain
+                            // read-modify-write would.
                             T dot[4];
 #pragma unroll
                             for (int q = 0; q < 4; ++q) {
@@ -488,13 +493,13 @@ __global__ __launch_bounds__(kBlock) void k_sub(const SubArgs<T> a)
                                 T sum = (h2 ? pb : pa) + __shfl_xor(h2 ? pa : pb, 2, TEAM);
                                 for (int sh = 4; sh < cl; sh <<= 1) sum = sum + __shfl_xor(sum, sh, TEAM);
                                 const int q = 2 * (int)h1 + (int)h2;
-                                if (cln < 4 && ok[q]) a.grad[e + q * step] += (double)sum;
+                                if (cln < 4 && ok[q]) (void)unsafeAtomicAdd(&a.grad[e + q * step], (double)sum);
                             } else {
 #pragma unroll
                                 for (int q = 0; q < 4; ++q) {
                                     T sum = dot[q];
                                     for (int sh = 1; sh < cl; sh <<= 1) sum = sum + __shfl_xor(sum, sh, TEAM);
-                                    if (cln == 0 && ok[q]) a.grad[e + q * step] += (double)sum;
+                                    if (cln == 0 && ok[q]) (void)unsafeAtomicAdd(&a.grad[e + q * step], (double)sum);
                                 }
                             }
                         }
