@@ -334,7 +334,8 @@ def test_wide_template_grad_fd(gpu_ctx, tv, te):
     """The wide templates' adjoint (main-table steps whose splits read each
     row value many times take the LDS-staged node kernel) against FP64
     central differences of the oracle's weighted count (affine in each entry
-    weight), and the forward against the oracle, on a small dense graph."""
+    weight), the forward against the oracle, and FP32 against FP64, on a
+    small dense graph."""
     rng = np.random.default_rng(tv + 100)
     n = 48
     s, d = _simple_graph(rng, n, 420)
@@ -360,3 +361,7 @@ def test_wide_template_grad_fd(gpu_ctx, tv, te):
         fd = (oracle.count_f64(op, colptr, rowidx, cols[0], vals=wp)
               - oracle.count_f64(op, colptr, rowidx, cols[0], vals=wm)) / (2 * h)
         assert abs(fd - g[e]) <= 1e-7 * max(1.0, abs(g).max()), (e, fd, g[e])
+    # FP32 tables (the staged node kernel in float): within 1e-5 of FP64, norm-wise
+    r32, g32 = gpu_ctx.count_grad(plan, 1, colors=cols, precision="fp32")
+    assert abs(r32[0] - raw[0]) <= 1e-5 * raw[0]
+    assert np.linalg.norm(g32 - g) <= 1e-5 * np.linalg.norm(g)
