@@ -380,7 +380,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print(json.dumps({"error": "this build measures one GPU: bench.py supports --gpus 1 only; "
-                                   "colorings shard across GPUs by design (see DESIGN.md)"}))
+                                   "colorings would shard across GPUs, each rank counting its share of the coloring stream"}))
         sys.exit(2)
 
     import subgraph_amd as sg
