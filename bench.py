@@ -12,6 +12,7 @@ Printed JSON adds:
   cpu_baseline  the compiled reference (oracle/_ref/pg_ref) on this host's
                 cores, same graph and colorings, with per-coloring parity
   batched       the same graph with 64 colorings per call (extra RHS columns)
+  fp64          the plain line with FP64 tables (--no-batched skips both)
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-batched]
                     [--plain-settings KEY=VAL,...] [--autotune 0|1]
@@ -501,6 +502,18 @@ def main():
         f"{result['estimates']['algorithmic_gbs']:.0f} GB/s over the whole pass")
 
     if not args.no_batched:
+        # the FP64 tables (exact counts below 2^53), same plain settings
+        ctx.count(plan, 1, seed=900, precision="fp64")
+        sync()
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            ctx.count(plan, 1, seed=2000 + k, precision="fp64")
+        sync()
+        f64 = args.steps / (time.perf_counter() - t0)
+        result["fp64"] = {"value": f64, "unit": "colorings/s", "vs_fp32": f64 / value,
+                          "workload": "the plain line with FP64 tables"}
+        log(f"fp64 plain: {f64:.1f} colorings/s ({f64 / value:.2f}x the FP32 line)")
+
         B = 64
         ctx.set_option("autotune", 1)
         t0 = time.perf_counter()
