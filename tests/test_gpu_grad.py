@@ -365,3 +365,41 @@ def test_wide_template_grad_fd(gpu_ctx, tv, te):
     r32, g32 = gpu_ctx.count_grad(plan, 1, colors=cols, precision="fp32")
     assert abs(r32[0] - raw[0]) <= 1e-5 * raw[0]
     assert np.linalg.norm(g32 - g) <= 1e-5 * np.linalg.norm(g)
+
+
+def _random_tree(rng, k):
+    return [(int(rng.integers(0, j)), j) for j in range(1, k)]
+
+
+def test_random_tree_grads_fd(gpu_ctx):
+    """Adjoint of random trees (k = 6..12: split shapes on both sides of the
+    staged node kernel's threshold, main and aux children of every size)
+    against FP64 central differences of the oracle's weighted count."""
+    rng = np.random.default_rng(2026)
+    n = 40
+    s, d = _simple_graph(rng, n, 360)
+    colptr, rowidx = sg.csc_from_edges(n, s, d)
+    rows = np.repeat(np.arange(n), np.diff(colptr))
+    lo, hi = np.minimum(rows, rowidx), np.maximum(rows, rowidx)
+    w = (0.5 + ((lo * 7919 + hi * 104729) % 1000) / 1000.0).astype(np.float32).astype(np.float64)
+    gpu_ctx.upload_graph(colptr, rowidx, vals=w.astype(np.float32))
+    h = 1e-3
+    for trial in range(8):
+        k = int(rng.integers(6, 13))
+        te = _random_tree(rng, k)
+        plan = gpu_ctx.plan(k, te)
+        op = oracle.Plan(k, te)
+        cols = rng.integers(0, k, size=(1, n)).astype(np.uint8)
+        print(f"[random-grad] trial={trial} k={k} edges={te}", flush=True)
+        raw, g = gpu_ctx.count_grad(plan, 1, colors=cols, precision="fp64")
+        want = oracle.count_f64(op, colptr, rowidx, cols[0], vals=w)
+        assert abs(raw[0] - want) <= 1e-12 * max(abs(want), 1.0), (trial, raw[0], want)
+        nz = np.flatnonzero(g)
+        picks = rng.choice(nz, min(4, len(nz)), replace=False) if len(nz) else rng.choice(len(rowidx), 2)
+        for e in picks:
+            wp, wm = w.copy(), w.copy()
+            wp[e] += h
+            wm[e] -= h
+            fd = (oracle.count_f64(op, colptr, rowidx, cols[0], vals=wp)
+                  - oracle.count_f64(op, colptr, rowidx, cols[0], vals=wm)) / (2 * h)
+            assert abs(fd - g[e]) <= 1e-7 * max(1.0, abs(g).max()), (trial, k, e, fd, g[e])
