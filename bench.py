@@ -19,7 +19,8 @@ Printed JSON adds:
 
 The plain line runs with fixed context options (--plain-settings; by default
 the ones the library's autotuner settles on for this shape: staged neighbour
-rows, 64-lane teams from degree 64, hubs from degree 1024), so that its
+rows, 64-lane teams from degree 64, hubs from degree 512 in 1024-entry
+chunks), so that its
 kernel mix, and the rocprofv3 summaries taken of it, do not change with the
 tuner's timing noise.  --autotune 1 tunes the plain shape instead, in one call
 before the warmup steps.  The batched line is autotuned.  config.settings
@@ -209,7 +210,8 @@ def call_roofline(ctx, run, seconds_per_call):
             "dominant": {k: v for k, v in dom.items() if k != "call_bytes"}}
 
 
-TUNED_KEYS = ("reg_cols", "stage_rows", "aux_rows", "bucket_high", "bucket_mid", "hub_degree", "batch_tile")
+TUNED_KEYS = ("reg_cols", "stage_rows", "aux_rows", "bucket_high", "bucket_mid", "hub_degree", "hub_chunk",
+              "batch_tile")
 
 
 def s22_variant(sync, with_grad=True):
@@ -369,7 +371,7 @@ def main():
     ap.add_argument("--cpu-colorings", type=int, default=12)
     ap.add_argument("--variants", default="grad,s22",
                     help="comma list of extra measurements: grad, s22, tree12 (the s22 12-tree part alone), none")
-    ap.add_argument("--plain-settings", default="stage_rows=1,bucket_high=64,hub_degree=1024,hub_chunk=1024",
+    ap.add_argument("--plain-settings", default="stage_rows=1,bucket_high=64,hub_degree=512,hub_chunk=1024",
                     metavar="KEY=VAL,...", help="context options of the plain line ('' = library defaults)")
     ap.add_argument("--variants-only", action="store_true",
                     help="run only the --variants measurements (a short run for profilers; no primary line)")
@@ -429,7 +431,7 @@ def main():
         ctx.count(plan, 1, seed=999)  # tunes this shape once, before the warmup and the timed region
         sync()
         tuning_s = time.perf_counter() - t0
-    settings = {k: ctx.get_option(k) for k in TUNED_KEYS + ("hub_chunk",)}
+    settings = {k: ctx.get_option(k) for k in TUNED_KEYS}
     log(f"plain settings ({'autotuned in %.1fs' % tuning_s if args.autotune else 'as set'}): {settings}")
     for w in range(args.warmup):
         ctx.count(plan, 1, seed=1000 + w)

@@ -59,7 +59,7 @@ const char* pg_last_error(const pg_ctx* ctx);
  *   "mem_budget_mb" device bytes the tables may use, 0 = 90% of free memory
  *   "autotune"     1 = before the first pass of each (plan, batch tile, precision), time
  *                  one pass per candidate of reg_cols, stage_rows, aux_rows, bucket_high,
- *                  bucket_mid and hub_degree (hub_chunk follows hub_degree) and keep
+ *                  bucket_mid, hub_degree and hub_chunk and keep
  *                  the fastest (a second sweep if the first changed one), then the
  *                  column tile: 3/4, 1/2 and 1/4 of the colorings per pass, kept when
  *                  a coloring costs less (unless batch_tile fixes it);

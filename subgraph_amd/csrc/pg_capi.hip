@@ -836,6 +836,7 @@ const std::vector<Knob>& tune_knobs()
         {"bucket_high", &pg_ctx::bucket_high, {16, 32, 64}},
         {"bucket_mid", &pg_ctx::bucket_mid, {4, 8}},
         {"hub_degree", &pg_ctx::hub_degree, {128, 256, 1024}},
+        {"hub_chunk", &pg_ctx::hub_chunk, {256, 1024, 4096}},
     };
     return knobs;
 }
@@ -854,7 +855,6 @@ void apply_settings(pg_ctx* c, const std::vector<int64_t>& v)
         if (c->*knobs[i].field != v[i]) drop_graphs(c);
         c->*knobs[i].field = v[i];
     }
-    c->hub_chunk = c->hub_degree;  // hub work items as long as the hub threshold
 }
 
 // Device time of one eager pass of B device-drawn colorings with the current

@@ -191,7 +191,7 @@ def test_autotune_keeps_results(gpu_ctx):
         for chosen in (chosen64, chosen32):
             assert chosen["stage_rows"] in (0, 1) and chosen["aux_rows"] in (0, -1) and chosen["reg_cols"] in (0, 1)
             assert chosen["bucket_high"] in (16, 32, 64) and chosen["bucket_mid"] in (4, 8)
-            assert chosen["hub_degree"] in (128, 256, 1024) and chosen["hub_chunk"] == chosen["hub_degree"]
+            assert chosen["hub_degree"] in (128, 256, 1024) and chosen["hub_chunk"] in (256, 1024, 4096)
         for i in range(4):
             assert _same64(g64[i], oracle.count_f64(op, colptr, rowidx, cols[i]))
             assert _rel(g32[i], oracle.count_f32(op, colptr, rowidx, cols[i])) <= TOL32
