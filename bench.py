@@ -19,8 +19,8 @@ Printed JSON adds:
 
 The plain line runs with fixed context options (--plain-settings; by default
 the ones the library's autotuner settles on for this shape: staged neighbour
-rows, 64-lane teams from degree 64, hubs from degree 512 in 1024-entry
-chunks), so that its
+rows, hubs above degree 512 in 1024-entry chunks on 64-lane teams, 16-lane
+teams from degree 64, 8-lane teams below), so that its
 kernel mix, and the rocprofv3 summaries taken of it, do not change with the
 tuner's timing noise.  --autotune 1 tunes the plain shape instead, in one call
 before the warmup steps.  The batched line is autotuned.  config.settings
@@ -328,7 +328,7 @@ def copy_calibration(mib):
 
 
 # the settings the autotuner picks for the 12-vertex tree in the s22 measurement
-TREE12_SETTINGS = {"stage_rows": 1, "bucket_high": 32, "hub_degree": 1024, "hub_chunk": 1024}
+TREE12_SETTINGS = {"stage_rows": 1, "bucket_high": 32, "bucket_mid": 8, "hub_degree": 512, "hub_chunk": 1024}
 
 
 def tree12_variant(sync):
@@ -371,7 +371,8 @@ def main():
     ap.add_argument("--cpu-colorings", type=int, default=12)
     ap.add_argument("--variants", default="grad,s22",
                     help="comma list of extra measurements: grad, s22, tree12 (the s22 12-tree part alone), none")
-    ap.add_argument("--plain-settings", default="stage_rows=1,bucket_high=64,hub_degree=512,hub_chunk=1024",
+    ap.add_argument("--plain-settings",
+                    default="stage_rows=1,bucket_high=512,bucket_mid=64,hub_degree=512,hub_chunk=1024",
                     metavar="KEY=VAL,...", help="context options of the plain line ('' = library defaults)")
     ap.add_argument("--variants-only", action="store_true",
                     help="run only the --variants measurements (a short run for profilers; no primary line)")

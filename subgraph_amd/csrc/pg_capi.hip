@@ -833,9 +833,9 @@ const std::vector<Knob>& tune_knobs()
         {"reg_cols", &pg_ctx::reg_cols, {0, 1}},
         {"stage_rows", &pg_ctx::stage_rows, {0, 1}},
         {"aux_rows", &pg_ctx::aux_rows, {0, -1}},
-        {"bucket_high", &pg_ctx::bucket_high, {16, 32, 64}},
-        {"bucket_mid", &pg_ctx::bucket_mid, {4, 8}},
-        {"hub_degree", &pg_ctx::hub_degree, {128, 256, 1024}},
+        {"bucket_high", &pg_ctx::bucket_high, {16, 32, 64, 128, 256, 512}},
+        {"bucket_mid", &pg_ctx::bucket_mid, {4, 8, 16, 32, 64, 128}},
+        {"hub_degree", &pg_ctx::hub_degree, {128, 256, 512, 1024}},
         {"hub_chunk", &pg_ctx::hub_chunk, {256, 1024, 4096}},
     };
     return knobs;
@@ -2097,7 +2097,8 @@ extern "C" int pg_graph_upload(pg_ctx* c, int32_t n, int64_t nnz, const int32_t*
                 }
             }
         }
-        static const int64_t bounds[] = {1025, 257, 129, 64, 32, 16, 8, 4};  // degree >= bound
+        // degree >= bound: hubs above 128, 256, 512, 1024; team buckets from 4 .. 512
+        static const int64_t bounds[] = {1025, 513, 512, 257, 256, 129, 128, 64, 32, 16, 8, 4};
         auto cls = [&](int32_t v) {
             int k = 0;
             for (int64_t b : bounds) {

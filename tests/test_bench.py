@@ -79,4 +79,5 @@ def test_trace_summary_agrees_with_the_line(line):
     assert first.startswith(f"| `{kern}` |")
     # settings the summary names are the line's plain settings
     s = line["config"]["settings"]
-    assert f"stage_rows={s['stage_rows']}, bucket_high={s['bucket_high']}, hub_degree={s['hub_degree']}" in text
+    for key in ("stage_rows", "bucket_high", "bucket_mid", "hub_degree", "hub_chunk"):
+        assert f"{key}={s[key]}" in text, key

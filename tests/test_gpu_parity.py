@@ -190,8 +190,9 @@ def test_autotune_keeps_results(gpu_ctx):
         print(f"[autotune] chosen fp64 {chosen64} fp32 {chosen32}", flush=True)
         for chosen in (chosen64, chosen32):
             assert chosen["stage_rows"] in (0, 1) and chosen["aux_rows"] in (0, -1) and chosen["reg_cols"] in (0, 1)
-            assert chosen["bucket_high"] in (16, 32, 64) and chosen["bucket_mid"] in (4, 8)
-            assert chosen["hub_degree"] in (128, 256, 1024) and chosen["hub_chunk"] in (256, 1024, 4096)
+            assert chosen["bucket_high"] in (16, 32, 64, 128, 256, 512)
+            assert chosen["bucket_mid"] in (4, 8, 16, 32, 64, 128)
+            assert chosen["hub_degree"] in (128, 256, 512, 1024) and chosen["hub_chunk"] in (256, 1024, 4096)
         for i in range(4):
             assert _same64(g64[i], oracle.count_f64(op, colptr, rowidx, cols[i]))
             assert _rel(g32[i], oracle.count_f32(op, colptr, rowidx, cols[i])) <= TOL32
@@ -479,3 +480,49 @@ def test_vertex_order_option(gpu_ctx):
         for k, v in zip(keys, saved):
             gpu_ctx.set_option(k, v)
         gpu_ctx.upload_graph(colptr, rowidx)
+
+
+BUCKET_SHAPES = [
+    # hub_degree, bucket_high, bucket_mid, hub_chunk
+    (512, 512, 64, 1024),   # no 64-lane bucket below the hubs; 16-lane teams from degree 64
+    (512, 512, 128, 1024),
+    (128, 512, 256, 256),   # bucket_high above hub_degree: its range is empty
+    (1024, 16, 128, 4096),  # bucket_mid above bucket_high: every non-high vertex on 8-lane teams
+    (256, 256, 256, 1024),
+    (128, 16, 4, 256),      # the smallest candidates
+]
+
+
+@pytest.mark.parametrize("hub_degree,bucket_high,bucket_mid,hub_chunk", BUCKET_SHAPES)
+def test_bucket_shapes_match_oracle(gpu_ctx, hub_degree, bucket_high, bucket_mid, hub_chunk):
+    """The degree buckets the autotuner may choose, including orderings that
+    leave a bucket empty, on a power-law graph with hubs: FP64 exact, FP32
+    within the tolerance, one and several colorings per pass."""
+    keys = ("hub_degree", "bucket_high", "bucket_mid", "hub_chunk")
+    saved = [gpu_ctx.get_option(k) for k in keys]
+    s, d = sg.rmat_edges(12, 16, seed=7)
+    n = 1 << 12
+    colptr, rowidx = sg.csc_from_edges(n, s, d)
+    assert np.diff(colptr).max() > 1024
+    try:
+        for key, v in zip(keys, (hub_degree, bucket_high, bucket_mid, hub_chunk)):
+            gpu_ctx.set_option(key, v)
+        gpu_ctx.upload_graph(colptr, rowidx)
+        rng = np.random.default_rng(hub_degree + bucket_high + bucket_mid)
+        for k, te in ((7, TREE7), (10, [(0, 1), (1, 2), (2, 3), (3, 4), (1, 5), (5, 6), (5, 7), (7, 8), (7, 9)])):
+            plan = gpu_ctx.plan(k, te)
+            op = oracle.Plan(k, te)
+            for B in (1, 4):
+                print(f"[buckets] hd={hub_degree} bh={bucket_high} bm={bucket_mid} hc={hub_chunk} k={k} B={B}",
+                      flush=True)
+                cols = rng.integers(0, k, size=(B, n)).astype(np.uint8)
+                g64 = gpu_ctx.count(plan, B, colors=cols, precision="fp64")
+                g32 = gpu_ctx.count(plan, B, colors=cols, precision="fp32")
+                for i in range(min(B, 2)):
+                    o64 = oracle.count_f64(op, colptr, rowidx, cols[i])
+                    assert _same64(g64[i], o64), (k, B, i, g64[i], o64)
+                    o32 = oracle.count_f32(op, colptr, rowidx, cols[i])
+                    assert _rel(g32[i], o32) <= TOL32, (k, B, i, g32[i], o32)
+    finally:
+        for key, v in zip(keys, saved):
+            gpu_ctx.set_option(key, v)
