@@ -51,10 +51,10 @@ void pg_destroy(pg_ctx* ctx);
 const char* pg_last_error(const pg_ctx* ctx);
 /* Tuning knobs (no reference counterpart; the reference hard-codes
  * _bufMatCols = 16, CountMat.cpp:62):
- *   "hub_degree"   vertices above this degree are split across teams (default 256)
- *   "hub_chunk"    neighbours per hub work item (default 256)
- *   "bucket_high"  vertices of at least this degree get 64-lane teams (default 32)
- *   "bucket_mid"   ... at least this degree 16-lane teams, the rest 8 (default 8)
+ *   "hub_degree"   vertices above this degree are split across teams (default 512)
+ *   "hub_chunk"    neighbours per hub work item (default 1024)
+ *   "bucket_high"  vertices of at least this degree get 64-lane teams (default 512)
+ *   "bucket_mid"   ... at least this degree 16-lane teams, the rest 8 (default 64)
  *   "batch_tile"   colorings per pass, 0 = automatic (default 0)
  *   "mem_budget_mb" device bytes the tables may use, 0 = 90% of free memory
  *   "autotune"     1 = before the first pass of each (plan, batch tile, precision), time

@@ -134,8 +134,11 @@ struct pg_ctx {
     hipStream_t stream = nullptr;
     std::string err;
     // options
-    int64_t hub_degree = 256, hub_chunk = 256, batch_tile = 0, mem_budget_mb = 0, timing = 0, autotune = 0;
-    int64_t bucket_high = 32, bucket_mid = 8;
+    // bucket defaults: what the autotuner settles on for one coloring per pass
+    // on power-law graphs (small teams for low degrees); wide batched rows run
+    // 64-lane teams whatever the thresholds
+    int64_t hub_degree = 512, hub_chunk = 1024, batch_tile = 0, mem_budget_mb = 0, timing = 0, autotune = 0;
+    int64_t bucket_high = 512, bucket_mid = 64;
     int64_t blocks_per_cu = 0;  // 0: resident workgroups from the occupancy query
     int64_t aux_rows = 0;       // neighbour sums row-wise: 1 always, 0 never, -1 per subtemplate
     int64_t stage_rows = 0;     // gathers read neighbour rows staged in LDS (1) or directly (0)

@@ -83,7 +83,9 @@ def test_random_graphs_and_trees(gpu_ctx, hub):
     """Random simple graphs, random trees k=2..12; hub=16 forces the split
     (partial-row) path for every vertex of degree > 16."""
     rng = np.random.default_rng(1234 if hub is None else 99)
-    saved = gpu_ctx.get_option("hub_degree"), gpu_ctx.get_option("hub_chunk")
+    saved = [gpu_ctx.get_option(k) for k in ("hub_degree", "hub_chunk", "bucket_high", "bucket_mid")]
+    gpu_ctx.set_option("bucket_high", 32)  # all three team shapes on these small graphs
+    gpu_ctx.set_option("bucket_mid", 8)
     if hub is not None:
         gpu_ctx.set_option("hub_degree", hub)
         gpu_ctx.set_option("hub_chunk", 16)
@@ -107,8 +109,8 @@ def test_random_graphs_and_trees(gpu_ctx, hub):
                 assert _rel(g32[i], o32) <= TOL32, (trial, i, g32[i], o32)
                 assert _same64(g64[i], o64), (trial, i, g64[i], o64)
     finally:
-        gpu_ctx.set_option("hub_degree", saved[0])
-        gpu_ctx.set_option("hub_chunk", saved[1])
+        for key, v in zip(("hub_degree", "hub_chunk", "bucket_high", "bucket_mid"), saved):
+            gpu_ctx.set_option(key, v)
 
 
 VARIANTS = [  # (aux_rows, stage_rows, reg_cols): every neighbour-sum kernel the library can pick
@@ -128,8 +130,10 @@ def test_kernel_variants_match_oracle(gpu_ctx, aux_rows, stage_rows, reg_cols, B
     one or several colorings per chunk), against the oracle: FP64 exact,
     FP32 within the tolerance; the hub split path included."""
     rng = np.random.default_rng(500 + 10 * B + 3 * aux_rows + stage_rows + reg_cols)
-    keys = ("aux_rows", "stage_rows", "reg_cols", "hub_degree", "hub_chunk")
+    keys = ("aux_rows", "stage_rows", "reg_cols", "hub_degree", "hub_chunk", "bucket_high", "bucket_mid")
     saved = [gpu_ctx.get_option(k) for k in keys]
+    gpu_ctx.set_option("bucket_high", 32)  # all three team shapes on these small graphs
+    gpu_ctx.set_option("bucket_mid", 8)
     gpu_ctx.set_option("aux_rows", aux_rows)
     gpu_ctx.set_option("stage_rows", stage_rows)
     gpu_ctx.set_option("reg_cols", reg_cols)
