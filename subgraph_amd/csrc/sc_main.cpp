@@ -15,6 +15,10 @@
 //   PG_VERBOSE=1   print the cost-model estimates before counting and the
 //                  per-kernel device times and model bandwidth after it
 //   PG_ESTIMATE=1  stop after the estimates (compute(t, isEstimate = true))
+// and one switch has no reference counterpart:
+//   PG_AUTOTUNE=1  let the library's autotuner pick kernel variants, degree
+//                  buckets and colorings per pass (one-time, before the first
+//                  pass of the template)
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -119,6 +123,11 @@ int main(int argc, char** argv)
     rc = executor.initialization(&g, comp_thds, iterations, isPruned, useSPMM, vtuneStart, false, 0, precision, seed);
     if (rc) {
         fprintf(stderr, "error: %s\n", executor.error().c_str());
+        return 1;
+    }
+    const char* autotune = getenv("PG_AUTOTUNE");
+    if (autotune && atoi(autotune) != 0 && pg_set_option(executor.context(), "autotune", 1) != PG_OK) {
+        fprintf(stderr, "error: %s\n", pg_last_error(executor.context()));
         return 1;
     }
     const char* verbose = getenv("PG_VERBOSE");

@@ -260,3 +260,18 @@ def test_cli_verbose_report(tmp_path):
     assert plain.returncode == 0, plain.stderr
     raw = [ln for ln in text.splitlines() if ln.startswith("Final raw count is ")]
     assert raw == [ln for ln in plain.stdout.splitlines() if ln.startswith("Final raw count is ")]
+
+
+@pytest.mark.gpu
+def test_cli_autotune_switch(tmp_path):
+    """sc-bin with PG_AUTOTUNE=1 tunes before counting; FP64 counts (exact
+    below 2^53) equal the untuned run's for the same seed."""
+    tp = tmp_path / "t.fascia"
+    tp.write_text("7\n6\n0 1\n1 2\n2 3\n1 4\n4 5\n4 6\n")
+    args = [SC_BIN, "rmat:12:8:1", str(tp), "4", "1", "0", "0", "1", "1", "-1", "1", "9", "fp64"]
+    plain = subprocess.run(args, cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    tuned = subprocess.run(args, cwd=tmp_path, env=dict(os.environ, PG_AUTOTUNE="1"), capture_output=True,
+                           text=True, timeout=120)
+    assert plain.returncode == 0 and tuned.returncode == 0, (plain.stderr, tuned.stderr)
+    raw = [ln for ln in plain.stdout.splitlines() if ln.startswith("Final raw count is ")]
+    assert raw and raw == [ln for ln in tuned.stdout.splitlines() if ln.startswith("Final raw count is ")]
