@@ -478,14 +478,23 @@ std::string kname(int team, int aux, int main_mode, int rc = kRegCols)
     return buf;
 }
 
+// A step's algorithmic bytes by what they scale with: the CSC (rowptr per
+// vertex, one column index and weight per entry), bytes gathered per entry
+// (the aux child's rows; the adjoint's per-entry gradient), and rows read or
+// written per vertex (main child, output, Y, dY).
+struct StepBytes {
+    double csc = 0, per_entry = 0, per_vertex = 0;
+};
+
 // The neighbour-summing launches of one subtemplate (or of H): explicit hub
 // chunks first (partial rows), then the degree buckets of the remaining
 // vertices with team shapes fitted to their degree, then the hub rows from
 // their partials.  `a` carries everything but the item range and team shape.
-// The step's algorithmic bytes are shared among its launches by the work
-// each covers (entries + vertices), so per-kernel sums stay exact.
+// The step's algorithmic bytes are shared among its launches by the entries
+// and vertices each covers: the CSC by entries + vertices, per-entry bytes by
+// entries, per-vertex bytes by vertices, so per-kernel sums stay exact.
 template <typename T>
-int launch_gathering(pg_ctx* c, int aux, int main_mode, SubArgs<T> a, T* part, int y_cols, double bytes)
+int launch_gathering(pg_ctx* c, int aux, int main_mode, SubArgs<T> a, T* part, int y_cols, StepBytes bytes)
 {
     const int rc = rc_for(c, aux, a.B, a.Wy);
     const int cl = cl_of(aux, a.B, a.k, a.Wy, a.Wa, rc);
@@ -493,8 +502,14 @@ int launch_gathering(pg_ctx* c, int aux, int main_mode, SubArgs<T> a, T* part, i
     if (a.segpad > 0) aux = AUX_STAGED;
     const hipStream_t st = c->stream;
     const int cus = c->num_cus;
-    const double units = (double)c->nnz + (double)c->n_act;
-    auto share = [&](double ents, double verts) { return units > 0 ? bytes * (ents + verts) / units : 0.0; };
+    const double Z = (double)c->nnz, N = (double)c->n_act;
+    auto share = [&](double ents, double verts) {
+        double b = 0;
+        if (Z + N > 0) b += bytes.csc * (ents + verts) / (Z + N);
+        if (Z > 0) b += bytes.per_entry * ents / Z;
+        if (N > 0) b += bytes.per_vertex * verts / N;
+        return b;
+    };
     if (c->n_hub > 0) {
         SubArgs<T> h = a;
         h.out = part;
@@ -617,6 +632,10 @@ int enqueue_step(pg_ctx* c, const pg_plan* p, const DevSub& d, int B, const Pass
         a.out_mode = (d.scaled && sizeof(T) == 4) ? OUT_EMA_SCALED : OUT_EMA;
     }
     const double tab_bytes = 4.0 * na * B * ((d.aa > 1 ? d.Ca : 0) + (d.mm > 1 ? d.Cm : 0) + (d.root ? 0 : d.Cs));
+    StepBytes sb;  // aux rows gathered by entries; main and output rows per vertex
+    sb.csc = csc_bytes(c);
+    sb.per_entry = 4.0 * na * B * (d.aa > 1 ? d.Ca : 0);
+    sb.per_vertex = 4.0 * na * B * ((d.mm > 1 ? d.Cm : 0) + (d.root ? 0 : d.Cs));
     if (d.aa == 1) {
         const int team = team_for(std::max(B * d.Wy, B * d.Wo));
         a.aux = pb.H;
@@ -636,7 +655,7 @@ int enqueue_step(pg_ctx* c, const pg_plan* p, const DevSub& d, int B, const Pass
         a.Wy4 = d.Wy4;
         a.xidx = d.xidx.as<int16_t>();
         a.Wa4 = d.Wa4;
-        int rc = launch_gathering<T>(c, aux_mode_for(c, p, d), main_mode, a, pb.part, B * d.Wy, csc_bytes(c) + tab_bytes);
+        int rc = launch_gathering<T>(c, aux_mode_for(c, p, d), main_mode, a, pb.part, B * d.Wy, sb);
         if (rc) return rc;
     }
     if (d.root) HIP_TRY(c, launch_root_reduce(pb.rootv, c->n_act, B, pb.tmp, pb.rawd, st));
@@ -670,7 +689,7 @@ int enqueue_pass(pg_ctx* c, const pg_plan* p, int B, const uint8_t* d_colors_in,
         a.ldo = B * p->k;
         a.out_mode = OUT_Y;
         int rc = launch_gathering<T>(c, AUX_COLORS, MAIN_LEAF, a, pb.part, B * p->k,
-                                     csc_bytes(c) + (double)c->n_act * B);
+                                     StepBytes{csc_bytes(c), 0.0, (double)c->n_act * B});
         if (rc) return rc;
     }
 
@@ -1395,7 +1414,7 @@ int enqueue_sub_backward(pg_ctx* c, const pg_plan* p, const DevSub& d, int B, co
         a.grad = c->d_grad.as<double>();
         // reads T_aux and dY, writes Y
         int rc = launch_gathering<T>(c, AUX_GATHER_G, MAIN_LEAF, a, pb.part, B * d.Wy,
-                                     csc_bytes(c) + col * 3.0 * Ca + 16.0 * Z);
+                                     StepBytes{csc_bytes(c), col * Ca + 16.0 * Z, col * 2.0 * Ca});
         if (rc) return rc;
         if (na.Tm) {
             na.part = 2;
@@ -1417,7 +1436,8 @@ int enqueue_sub_backward(pg_ctx* c, const pg_plan* p, const DevSub& d, int B, co
         t.out = Ga;
         t.ldo = B * d.Wa;
         // reads the neighbours' dY, writes G_aux
-        rc = launch_gathering<T>(c, AUX_GATHER_T, MAIN_LEAF, t, pb.part, B * d.Wa, csc_bytes(c) + col * 2.0 * Ca);
+        rc = launch_gathering<T>(c, AUX_GATHER_T, MAIN_LEAF, t, pb.part, B * d.Wa,
+                                 StepBytes{csc_bytes(c), col * Ca, col * Ca});
         if (rc) return rc;
     } else {
         // aux child a leaf: Y is the histogram H; the entry's derivative is
