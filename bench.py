@@ -1,30 +1,36 @@
 """Benchmark: colorings/s of the PGBSC color-coding DP on one MI355X.
 
-Workload (one "step" = one coloring through the whole DP, the reference's
-plain method): R-MAT scale 20, edge factor 8 (a=.57 b=.19 c=.19, seed 1,
-self loops and duplicates removed, symmetrized into CSC), 7-vertex tree
-0-1,1-2,2-3,1-4,4-5,4-6, FP32 tables with the reference's numerics.
-Inputs are resident in HBM before the timed region; each step draws a fresh
-coloring on the device.
+Primary line (the north-star workload): R-MAT scale 22, edge factor 16
+(a=.57 b=.19 c=.19, seed 1, self loops and duplicates removed, symmetrized
+into CSC with the reference text loader's hole rule: 4,194,304 vertices,
+128M entries), the 12-vertex tree 0-1,1-2,2-3,3-4,4-5,2-6,6-7,7-8,6-9,9-10,
+10-11, unit weights, FP32 tables with the reference's numerics.  One step is
+one batched forward+backward call of 16 colorings drawn on the device: the
+counts and, through the hand-written adjoint, the gradient of their sum
+with respect to every entry weight (left on the device as float32, as the
+torch.autograd path takes it).  Inputs are resident in HBM before the timed
+region.  value = colorings/s over the K timed steps.
 
 Printed JSON adds:
-  roofline      dominant kernel's algorithmic bytes / its HIP-event time
+  roofline      the step's dominant kernel: its algorithmic bytes per launch
+                over its average HIP-event launch time (eager run of one step)
   cpu_baseline  the compiled reference (oracle/_ref/pg_ref) on this host's
-                cores, same graph and colorings, with per-coloring parity
-  batched       the same graph with 64 colorings per call (extra RHS columns)
-  fp64          the plain line with FP64 tables (--no-batched skips both)
+                physical cores, one coloring of the same graph and color
+                stream, with parity against the device count
+  comparison    the same graph and template: batched forward, plain (one
+                coloring per call) forward and forward+backward, the whole
+                calls' %HBM, U(0.5,1.5) weights, and parity against the
+                reference's golden counts
+  plain_s20     the reference's plain method on R-MAT s20/ef8 with the
+                7-vertex tree, one coloring per step, with its own roofline
+                and reference CPU baseline; fp64 and 64-per-pass beside it
+  s22_batching  u5, the 7- and 10-vertex trees at 64 colorings per call
+                (autotuned) beside their plain passes, and the mixed batch
+  fwd_bwd       s18/ef16 with U(0.5,1.5) weights: 5-star and 5-path
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-batched]
-                    [--plain-settings KEY=VAL,...] [--autotune 0|1]
-
-The plain line runs with fixed context options (--plain-settings; by default
-the ones the library's autotuner settles on for this shape: staged neighbour
-rows, hubs above degree 512 in 1024-entry chunks on 64-lane teams, 16-lane
-teams from degree 64, 8-lane teams below), so that its
-kernel mix, and the rocprofv3 summaries taken of it, do not change with the
-tuner's timing noise.  --autotune 1 tunes the plain shape instead, in one call
-before the warmup steps.  The batched line is autotuned.  config.settings
-records what ran.
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
+                    [--variants plain,s22,grad] [--autotune 0|1]
+                    [--settings KEY=VAL,...]
 """
 from __future__ import annotations
 
@@ -43,66 +49,121 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 TREE7 = [(0, 1), (1, 2), (2, 3), (1, 4), (4, 5), (4, 6)]
+GOLDEN_S22 = os.path.join(ROOT, "tests", "golden", "golden_s22.json")
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def physical_cores() -> int:
+def host_cpu():
+    """(physical cores among the CPUs this process may run on, logical
+    CPUs in that set, CPU model name)."""
     try:
-        n = len(os.sched_getaffinity(0))
+        allowed = sorted(os.sched_getaffinity(0))
     except AttributeError:
-        n = os.cpu_count() or 1
-    return max(1, min(16, n))
+        allowed = list(range(os.cpu_count() or 1))
+    model, phys = "unknown", set()
+    try:
+        cpu, pid, core = None, None, None
+        with open("/proc/cpuinfo") as f:
+            for line in f.read().splitlines() + [""]:
+                if line.startswith("processor"):
+                    cpu = int(line.split(":")[1])
+                elif line.startswith("physical id"):
+                    pid = line.split(":")[1].strip()
+                elif line.startswith("core id"):
+                    core = line.split(":")[1].strip()
+                elif line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                elif not line.strip() and cpu is not None:
+                    if cpu in allowed:
+                        phys.add((pid, core))
+                    cpu, pid, core = None, None, None
+    except OSError:
+        pass
+    return max(1, len(phys) or len(allowed)), len(allowed), model
 
 
-def cpu_baseline(n_decl, src, dst, seed, ncol, gpu_raw_fn):
-    """The compiled reference timed on host cores, or the C oracle if absent."""
+def physical_cores() -> int:
+    """Threads of the CPU baseline: the physical cores this process may use,
+    at most 16 (one GPU's share of the host)."""
+    return max(1, min(16, host_cpu()[0]))
+
+
+def write_el(path, n_decl, src, dst):
+    with open(path, "w") as f:
+        f.write(f"{n_decl}\n{len(src)}\n")
+        step = 1 << 20
+        for i in range(0, len(src), step):
+            f.write("".join(f"{a} {b}\n" for a, b in zip(src[i:i + step].tolist(), dst[i:i + step].tolist())))
+
+
+def write_graph_data(path, m, colptr, rowidx):
+    """The reference's binary CSC (CSCGraph::serialize, CSCGraph.hpp:426-434):
+    edges, vertices, degrees, column pointers, row indices, 1.0f values."""
+    n = len(colptr) - 1
+    with open(path, "wb") as f:
+        np.array([m, n], np.int32).tofile(f)
+        np.diff(colptr).astype(np.int32).tofile(f)
+        np.ascontiguousarray(colptr, np.int32).tofile(f)
+        np.ascontiguousarray(rowidx, np.int32).tofile(f)
+        np.ones(len(rowidx), np.float32).tofile(f)
+
+
+def run_reference(graph_path, tv, tedges, seed, ncol, threads, timeout):
+    """pg_ref on ncol colorings of stream `seed`; its lines are echoed as
+    they come (loading, then one per coloring).  Returns ([raw], [s])."""
     import oracle
-    threads = physical_cores()
     with tempfile.TemporaryDirectory(dir="/tmp") as tmp:
-        gp, tp = os.path.join(tmp, "g.el"), os.path.join(tmp, "t.fascia")
-        with open(gp, "w") as f:
-            f.write(f"{n_decl}\n{len(src)}\n")
-            step = 1 << 20
-            for i in range(0, len(src), step):
-                f.write("".join(f"{a} {b}\n" for a, b in zip(src[i:i + step].tolist(), dst[i:i + step].tolist())))
+        tp = os.path.join(tmp, "t.fascia")
         with open(tp, "w") as f:
-            f.write(f"7\n6\n" + "".join(f"{a} {b}\n" for a, b in TREE7))
-        if oracle.have_ref():
-            raws, secs = oracle.run_ref(gp, tp, "-", ncol, threads=threads, seed=seed, omp_threads=threads,
-                                        timeout=600)
-            kind = "reference"
-        else:
-            import subgraph_amd as sg
-            n, s2, d2 = sg.remove_holes(n_decl, src, dst)
-            colptr, rowidx = oracle.csc_build(n, s2, d2)
-            plan = oracle.Plan(7, TREE7)
-            cols = sg.colors_fill(seed, 0, ncol, n, 7)
-            raws, secs = [], []
-            for i in range(ncol):
-                t0 = time.perf_counter()
-                raws.append(oracle.count_f32(plan, colptr, rowidx, cols[i]))
-                secs.append(time.perf_counter() - t0)
-            kind = "port"
+            f.write(f"{tv}\n{len(tedges)}\n" + "".join(f"{a} {b}\n" for a, b in tedges))
+        env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+        proc = subprocess.Popen([oracle.REF_BIN, graph_path, tp, "-", str(ncol), str(threads), str(seed)],
+                                stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, env=env)
+        raws, secs = [], []
+        t0 = time.perf_counter()
+        for line in proc.stdout:
+            log("  reference: " + line.rstrip())
+            if line.startswith("PGREF coloring"):
+                f = line.split()
+                raws.append(float(f[4]))
+                secs.append(float(f[6]))
+            if time.perf_counter() - t0 > timeout:
+                proc.kill()
+                raise subprocess.TimeoutExpired(oracle.REF_BIN, timeout)
+        if proc.wait() != 0 or len(raws) != ncol:
+            raise subprocess.CalledProcessError(proc.returncode, oracle.REF_BIN)
+    return raws, secs
+
+
+def cpu_baseline(graph_path, tv, tedges, seed, ncol, gpu_raw_fn, sample, timeout=900):
+    """The compiled reference timed on host cores (per-coloring time, the
+    scope of the reference's "Time for count per iter", CountMat.cpp:200-213;
+    graph load excluded), with per-coloring parity against the device."""
+    import oracle
+    if not oracle.have_ref():
+        return {"value": None, "unit": "colorings/s", "cores": physical_cores(), "kind": "reference",
+                "sample": "skipped: oracle/_ref/pg_ref is not built here"}
+    threads = physical_cores()
+    phys, logical, model = host_cpu()
+    raws, secs = run_reference(graph_path, tv, tedges, seed, ncol, threads, timeout)
     gpu = gpu_raw_fn(ncol)
     err = max(abs(g - r) / abs(r) for g, r in zip(gpu, raws))
-    return {"value": ncol / sum(secs), "unit": "colorings/s", "cores": threads, "kind": kind,
-            "sample": f"{ncol} colorings (stream seed {seed}) of the bench workload, per-coloring time as "
-                      f"CountMat::colorCounting; graph load excluded",
-            "seconds_per_coloring": sum(secs) / ncol, "parity_max_rel_err": err}
+    return {"value": ncol / sum(secs), "unit": "colorings/s", "cores": threads, "kind": "reference",
+            "cpu_model": model, "physical_cores_available": phys, "logical_cpus_available": logical,
+            "sample": sample, "seconds_per_coloring": sum(secs) / ncol, "parity_max_rel_err": err,
+            "reference_raw": raws, "device_raw": gpu}
 
 
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_plain_rmat_s20_ef8_tree7.json")
-
-
-def measured_traffic(kernel):
-    """HBM bytes per launch of `kernel` on this workload from the committed
-    rocprofv3 FETCH_SIZE / WRITE_SIZE runs (calibrated against a copy kernel
-    of known size), or None when that file does not cover the kernel."""
+def measured_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` in `workload` from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE runs (profiles/traffic_<workload>.json,
+    calibrated against a copy kernel of known size), or None when no such
+    record covers the kernel."""
     try:
-        with open(TRAFFIC_FILE) as f:
+        with open(os.path.join(ROOT, "profiles", f"traffic_{workload}.json")) as f:
             rec = json.load(f)
     except (OSError, ValueError):
         return None
@@ -120,55 +181,11 @@ def _timed(fn, reps, sync):
     return (time.perf_counter() - t0) / reps
 
 
-def fwd_bwd_variant(sync):
-    """Differentiation workload: R-MAT s18/ef16 with symmetric weights
-    U(0.5,1.5), 5-star and 5-path, 16 colorings per call, forward + adjoint
-    (per-entry gradient), beside the plain one-coloring forward on the same
-    graph, autotuned.  FP32 tables."""
-    import subgraph_amd as sg
-    rng = np.random.default_rng(18)
-    n = 1 << 18
-    src, dst = sg.rmat_edges(18, 16, seed=1)
-    w_und = rng.uniform(0.5, 1.5, len(src)).astype(np.float32)
-    colptr, rowidx = sg.csc_from_edges(n, src, dst)
-    rows = np.repeat(np.arange(n), np.diff(colptr))
-    key = np.minimum(rows, rowidx).astype(np.int64) * n + np.maximum(rows, rowidx)
-    ukey = src.astype(np.int64) * n + dst
-    order = np.argsort(ukey)
-    w = w_und[order[np.searchsorted(ukey[order], key)]]
-    ctx = sg.Context(0)
-    ctx.upload_graph(colptr, rowidx, vals=w)
-    ctx.set_option("autotune", 1)  # tuned per (plan, tile) in each measurement's untimed first call
-    out = {"graph": "R-MAT scale 18 edge factor 16 (seed 1), weights U(0.5,1.5) symmetric",
-           "nnz": int(len(rowidx)), "colorings_per_call": 16, "dtype": "fp32", "templates": {}}
-    import torch
-    grad = torch.empty(len(rowidx), dtype=torch.float32, device="cuda")  # the autograd path's output
-    for name, te in (("star5", [(0, 1), (0, 2), (0, 3), (0, 4)]), ("u5", [(0, 1), (1, 2), (2, 3), (3, 4)])):
-        plan = ctx.plan(5, te)
-        t_plain = _timed(lambda r=0: ctx.count(plan, 1, seed=50 + r), 10, sync)
-        t_fwd = _timed(lambda r=0: ctx.count(plan, 16, seed=60 + r), 5, sync)
-        t_fb1 = _timed(lambda r=0: ctx.count_grad_device(plan, 1, grad.data_ptr(), seed=80 + r), 10, sync)
-        t_fb = _timed(lambda r=0: ctx.count_grad_device(plan, 16, grad.data_ptr(), seed=70 + r), 5, sync)
-        t_fbh = _timed(lambda r=0: ctx.count_grad(plan, 16, seed=90 + r, precision="fp32"), 3, sync)
-        rec = {"plain_fwd_colorings_per_s": 1.0 / t_plain, "batched_fwd_colorings_per_s": 16 / t_fwd,
-               "plain_fwd_bwd_colorings_per_s": 1.0 / t_fb1, "fwd_bwd_colorings_per_s": 16 / t_fb,
-               "fwd_bwd_host_grad_colorings_per_s": 16 / t_fbh}
-        rec["fwd_bwd_vs_plain_fwd_bwd"] = rec["fwd_bwd_colorings_per_s"] / rec["plain_fwd_bwd_colorings_per_s"]
-        rec["fwd_bwd_vs_plain_fwd"] = rec["fwd_bwd_colorings_per_s"] / rec["plain_fwd_colorings_per_s"]
-        out["templates"][name] = rec
-        log(f"fwd+bwd s18 {name}: plain fwd {rec['plain_fwd_colorings_per_s']:.0f}/s, batched fwd "
-            f"{rec['batched_fwd_colorings_per_s']:.0f}/s, plain fwd+bwd {rec['plain_fwd_bwd_colorings_per_s']:.0f}/s, "
-            f"fwd+bwd {rec['fwd_bwd_colorings_per_s']:.0f}/s (gradient to host memory: "
-            f"{rec['fwd_bwd_host_grad_colorings_per_s']:.0f}/s)")
-    out["note"] = ("fwd+bwd: the gradient stays on the device as float32 (the torch.autograd path); "
-                   "host_grad: the same call returning FP64 gradients in host memory (a PCIe copy of 8 B per "
-                   "entry per call)")
-    return out
-
-
 TREE10 = [(0, 1), (1, 2), (2, 3), (3, 4), (1, 5), (5, 6), (5, 7), (7, 8), (7, 9)]
 TREE12 = [(0, 1), (1, 2), (2, 3), (3, 4), (4, 5), (2, 6), (6, 7), (7, 8), (6, 9), (9, 10), (10, 11)]
 U5 = [(0, 1), (1, 2), (2, 3), (3, 4)]
+TUNED_KEYS = ("reg_cols", "stage_rows", "aux_rows", "bucket_high", "bucket_mid", "hub_degree", "hub_chunk",
+              "batch_tile")
 
 
 def hashed_weights(n, colptr, rowidx):
@@ -185,19 +202,44 @@ def hashed_weights(n, colptr, rowidx):
     return (0.5 + (x >> np.uint64(40)).astype(np.float64) / float(1 << 24)).astype(np.float32)
 
 
+def kernel_times(ctx, run, reps=1):
+    """{kernel: [ms, algorithmic bytes, launches]} summed over `reps` eager
+    runs with a HIP event pair around every launch on the library's stream."""
+    agg = {}
+    ctx.set_option("timing", 1)
+    try:
+        for r in range(reps):
+            run()
+            for name, (ms, nbytes, nl) in ctx.last_timing_kernels().items():
+                a = agg.setdefault(name, [0.0, 0.0, 0])
+                a[0] += ms
+                a[1] += nbytes
+                a[2] += nl
+    finally:
+        ctx.set_option("timing", 0)
+    if not agg or max(v[0] for v in agg.values()) <= 0:
+        raise RuntimeError("per-launch HIP event timing returned no durations")
+    return agg
+
+
+def roofline_of(agg, workload):
+    """The dominant kernel (largest summed event time) of kernel_times()."""
+    kern, (kms, kbytes, klaunch) = max(agg.items(), key=lambda kv: kv[1][0])
+    achieved = kbytes / (kms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(kern, workload), "kernel": kern,
+            "launches": klaunch, "avg_launch_ms": kms / klaunch, "algorithmic_bytes_per_launch": kbytes / klaunch,
+            "share_of_kernel_time": kms / sum(v[0] for v in agg.values())}
+
+
 def dominant_kernel(ctx, run):
     """Run once with per-launch HIP events; the dominant kernel's name,
     launches, ms per launch and algorithmic GB/s."""
-    ctx.set_option("timing", 1)
-    try:
-        run()
-        kern = ctx.last_timing_kernels()
-    finally:
-        ctx.set_option("timing", 0)
-    name, (ms, nbytes, nl) = max(kern.items(), key=lambda kv: kv[1][0])
+    agg = kernel_times(ctx, run)
+    name, (ms, nbytes, nl) = max(agg.items(), key=lambda kv: kv[1][0])
     return {"kernel": name, "launches": nl, "avg_launch_ms": ms / nl,
             "achieved_gbs": nbytes / (ms * 1e-3) / 1e9, "frac": nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
-            "call_bytes": sum(v[1] for v in kern.values())}
+            "call_bytes": sum(v[1] for v in agg.values())}
 
 
 def call_roofline(ctx, run, seconds_per_call):
@@ -210,30 +252,171 @@ def call_roofline(ctx, run, seconds_per_call):
             "dominant": {k: v for k, v in dom.items() if k != "call_bytes"}}
 
 
-TUNED_KEYS = ("reg_cols", "stage_rows", "aux_rows", "bucket_high", "bucket_mid", "hub_degree", "hub_chunk",
-              "batch_tile")
+def golden_s22():
+    """The compiled reference's per-coloring raw counts at R-MAT s22/ef16
+    (color stream seed 101), by template."""
+    try:
+        with open(GOLDEN_S22) as f:
+            return {c["template"]: c for c in json.load(f)["counts"]}
+    except (OSError, ValueError):
+        return {}
 
 
-def s22_variant(sync, with_grad=True):
-    """Batching on R-MAT scale 22, edge factor 16 (seed 1, dedup,
-    symmetrized): 64 colorings per pass of the 5-path, the 7-vertex tree and
-    the 10-vertex tree (autotuned), each beside its plain one-coloring pass,
-    then an uneven mixed batch of the three templates in one call; then the
-    12-vertex tree forward and forward+backward at unit and U(0.5,1.5)
-    weights, 16 colorings per call and one (the plain method), autotuned,
-    with the %HBM of whole calls.  FP32 tables."""
+def parity_vs_golden(ctx, plan, rec):
+    """Worst relative error of the device's FP32 counts of the golden
+    colorings against the reference's."""
+    want = np.array([float.fromhex(r) for r in rec["raw"]])
+    got = ctx.count(plan, len(want), seed=rec["colors"]["seed"])
+    return float(np.max(np.abs(got - want) / np.abs(want))), len(want)
+
+
+def s22_graph():
+    """R-MAT s22/ef16, seed 1, with the reference text loader's hole rule."""
     import subgraph_amd as sg
     t0 = time.perf_counter()
-    n = 1 << 22
+    n_decl = 1 << 22
     src, dst = sg.rmat_edges(22, 16, seed=1)
-    colptr, rowidx = sg.csc_from_edges(n, src, dst)
-    del src, dst
+    n, s2, d2 = sg.remove_holes(n_decl, src, dst)
+    colptr, rowidx = sg.csc_from_edges(n, s2, d2)
+    log(f"s22 graph: n={n} undirected={len(src)} nnz={len(rowidx)} ({time.perf_counter() - t0:.1f}s)")
+    return n, int(len(src)), colptr, rowidx
+
+
+# the 12-vertex tree's settings when the autotuner is off (--autotune 0): what
+# it settles on for this template at s22 (every gather of this template runs
+# 64-lane teams, so the bucket thresholds matter little)
+TREE12_SETTINGS = {"stage_rows": 1, "bucket_high": 32, "bucket_mid": 8, "hub_degree": 512, "hub_chunk": 1024}
+HEADLINE = "tree12_fwd_bwd_rmat_s22_ef16"
+
+
+def headline(args, sync):
+    """The primary line: batched forward+backward of the 12-vertex tree on
+    R-MAT s22/ef16, 16 colorings per step, and what it is compared with."""
+    import torch
+    import subgraph_amd as sg
+    n, m, colptr, rowidx = s22_graph()
     ctx = sg.Context(0)
+    if args.autotune:
+        ctx.set_option("autotune", 1)
+    else:
+        for k, v in TREE12_SETTINGS.items():
+            ctx.set_option(k, v)
+    for kv in filter(None, args.settings.split(",")):
+        key, val = kv.split("=")
+        ctx.set_option(key.strip(), int(val))
     ctx.upload_graph(colptr, rowidx)
     st = ctx.stats()
-    log(f"s22: n={n} nnz={len(rowidx)} n_active={st['n_active']} ({time.perf_counter() - t0:.1f}s)")
-    out = {"graph": "R-MAT scale 22 edge factor 16 (a=.57 b=.19 c=.19 seed 1), dedup, symmetrized",
-           "n": n, "nnz": int(len(rowidx)), "n_active": st["n_active"], "dtype": "fp32", "templates": {}}
+    plan = ctx.plan(12, TREE12)
+    B = args.colorings
+    gbuf = torch.empty(len(rowidx), dtype=torch.float32, device="cuda")  # the autograd path's output
+
+    def step(k):
+        ctx.count_grad_device(plan, B, gbuf.data_ptr(), seed=5000 + k)
+
+    t0 = time.perf_counter()
+    ctx.count(plan, B, seed=4999)        # the forward tile's tuning (autotune 1), untimed
+    step(-1)                             # the gradient call's own tile and captured graphs, untimed
+    sync()
+    tune_s = time.perf_counter() - t0
+    settings = {k: ctx.get_option(k) for k in TUNED_KEYS}
+    for w in range(args.warmup):
+        step(-2 - w)
+    sync()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        step(k)
+    sync()
+    elapsed = time.perf_counter() - t0
+    value = args.steps * B / elapsed
+    tile = ctx.get_option("last_tile")
+    log(f"headline: 12-tree fwd+bwd {value:.2f} colorings/s ({1e3 * elapsed / args.steps:.1f} ms per step of {B}, "
+        f"{tile} per pass; settings {settings}; first calls {tune_s:.1f}s)")
+    agg = kernel_times(ctx, lambda: step(0))
+    roof = roofline_of(agg, HEADLINE)
+    log(f"headline dominant {roof['kernel']}: {roof['avg_launch_ms']:.3f} ms/launch, {roof['achieved']:.0f} GB/s "
+        f"algorithmic ({100 * roof['frac']:.2f}%), {100 * roof['share_of_kernel_time']:.1f}% of kernel time")
+    result = {
+        "metric": "colorings/sec + %HBM roofline; batched and fwd+bwd vs plain DP, same GPU",
+        "value": value, "unit": "colorings/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": HEADLINE,
+                   "graph": "R-MAT scale 22 edge factor 16 (a=.57 b=.19 c=.19 seed 1), dedup, symmetrized, "
+                            "the reference loader's hole rule",
+                   "template": "12-vertex tree 0-1,1-2,2-3,3-4,4-5,2-6,6-7,7-8,6-9,9-10,10-11",
+                   "step": f"one forward+backward call of {B} colorings (counts and the per-entry gradient, "
+                           "unit weights, gradient kept on the device)",
+                   "colorings_per_step": B, "colorings_per_pass": tile,
+                   "adjoint_tables_recomputed": bool(ctx.get_option("last_recompute")),
+                   "n": int(n), "n_active": st["n_active"], "nnz": int(len(rowidx)),
+                   "settings": settings, "autotuned": bool(args.autotune), "first_calls_seconds": tune_s},
+        "roofline": roof,
+    }
+    # ---- what the line is compared with, same graph and template
+    cmp_ = {}
+    t_fb = elapsed / args.steps
+    t_f = _timed(lambda r=0: ctx.count(plan, B, seed=6000 + r), 1, sync)
+    cmp_["fwd_colorings_per_s"] = B / t_f
+    cmp_["fwd_colorings_per_pass"] = ctx.get_option("last_tile")
+    t_pf = _timed(lambda r=0: ctx.count(plan, 1, seed=7000 + r), 2, sync)
+    t_pfb = _timed(lambda r=0: ctx.count_grad_device(plan, 1, gbuf.data_ptr(), seed=8000 + r), 2, sync)
+    cmp_.update({"fwd_bwd_colorings_per_s": value, "plain_fwd_colorings_per_s": 1 / t_pf,
+                 "plain_fwd_bwd_colorings_per_s": 1 / t_pfb, "fwd_vs_plain_fwd": t_pf * B / t_f,
+                 "fwd_bwd_vs_plain_fwd_bwd": t_pfb * B / t_fb, "fwd_bwd_vs_fwd": t_f / t_fb})
+    log(f"12-tree: fwd {B / t_f:.2f}/s (plain {1 / t_pf:.2f}/s, {cmp_['fwd_vs_plain_fwd']:.2f}x); fwd+bwd "
+        f"{value:.2f}/s (plain {1 / t_pfb:.2f}/s, {cmp_['fwd_bwd_vs_plain_fwd_bwd']:.2f}x)")
+    roofs = {"fwd": call_roofline(ctx, lambda: ctx.count(plan, B, seed=6001), t_f),
+             "fwd_bwd": call_roofline(ctx, lambda: step(1), t_fb),
+             "plain_fwd": call_roofline(ctx, lambda: ctx.count(plan, 1, seed=7001), t_pf),
+             "plain_fwd_bwd": call_roofline(ctx, lambda: ctx.count_grad_device(plan, 1, gbuf.data_ptr(), seed=8001),
+                                            t_pfb)}
+    cmp_["whole_call_roofline"] = roofs
+    log("12-tree %HBM of whole calls: " + ", ".join(
+        f"{k} {v['call_gbs']:.0f} GB/s ({100 * v['call_frac']:.2f}%)" for k, v in roofs.items()))
+    gold = golden_s22().get("tree12")
+    if gold:
+        err, nc = parity_vs_golden(ctx, plan, gold)
+        cmp_["parity_vs_reference"] = {"max_rel_err": err, "colorings": nc,
+                                       "source": "tests/golden/golden_s22.json (compiled reference, stream seed 101)"}
+        log(f"12-tree parity vs the reference's s22 counts: max rel err {err:.2e} over {nc} colorings")
+    ctx.set_weights(hashed_weights(n, colptr, rowidx))
+    t_wf = _timed(lambda r=0: ctx.count(plan, B, seed=9000 + r), 1, sync)
+    t_wfb = _timed(lambda r=0: ctx.count_grad_device(plan, B, gbuf.data_ptr(), seed=9100 + r), 1, sync)
+    cmp_["weighted_uniform_0.5_1.5"] = {"fwd_colorings_per_s": B / t_wf, "fwd_bwd_colorings_per_s": B / t_wfb}
+    log(f"12-tree U(0.5,1.5) weights: fwd {B / t_wf:.2f}/s, fwd+bwd {B / t_wfb:.2f}/s")
+    ctx.set_weights(None)
+    cmp_["note"] = ("whole_call_roofline: algorithmic bytes of one call (forward: each table written and read once, "
+                    "the CSC once per gathering launch; backward: the same units plus the FP64 per-entry gradient "
+                    "read and written by each launch that adds to it) over the call's graph-replayed time")
+    result["comparison"] = cmp_
+    if not args.no_cpu:
+        seed = 4242
+        log("cpu baseline: the compiled reference, one coloring of the 12-vertex tree at s22 ...")
+        try:
+            with tempfile.TemporaryDirectory(dir="/tmp") as tmp:
+                gp = os.path.join(tmp, "graph.data")
+                write_graph_data(gp, m, colptr, rowidx)
+                result["cpu_baseline"] = cpu_baseline(
+                    gp, 12, TREE12, seed, 1, lambda c: ctx.count(plan, c, seed=seed).tolist(),
+                    "one coloring (stream seed 4242, coloring 0) of the headline graph and template: the whole "
+                    "12-vertex-tree DP of CountMat::colorCounting (forward only; the reference has no adjoint)")
+            cb = result["cpu_baseline"]
+            log(f"cpu: {cb['value']:.4f} colorings/s on {cb['cores']} threads ({cb['cpu_model']}), parity max rel "
+                f"err {cb['parity_max_rel_err']:.2e}")
+        except (subprocess.SubprocessError, OSError) as e:
+            result["cpu_baseline"] = {"value": None, "unit": "colorings/s", "cores": physical_cores(),
+                                      "kind": "reference", "sample": f"failed: {e}"}
+    return result, ctx, (n, colptr, rowidx)
+
+
+def s22_batching(ctx, graph, sync):
+    """Batching on the headline graph: 64 colorings per call of the 5-path,
+    the 7-vertex tree and the 10-vertex tree (autotuned), each beside its
+    plain one-coloring pass and with parity against the reference's s22
+    counts, then an uneven mixed batch of the three in one call."""
+    n, colptr, rowidx = graph
+    gold = golden_s22()
+    out = {"graph": "R-MAT scale 22 edge factor 16 (the headline graph)", "dtype": "fp32", "templates": {}}
     ctx.set_option("autotune", 1)
     plans = {}
     for name, nv, te, reps in (("u5", 5, U5, 3), ("tree7", 7, TREE7, 2), ("tree10", 10, TREE10, 1)):
@@ -253,9 +436,14 @@ def s22_variant(sync, with_grad=True):
         rec = {"batched_colorings_per_call": B, "batched_colorings_per_s": B / t_b,
                "plain_colorings_per_s": 1.0 / t_p, "batched_vs_plain": (B / t_b) * t_p,
                "tuned": tuned, "tuning_seconds": tune_s, "roofline_dominant": dom}
+        if name in gold:
+            err, nc = parity_vs_golden(ctx, plan, gold[name])
+            rec["parity_vs_reference"] = {"max_rel_err": err, "colorings": nc}
         out["templates"][name] = rec
         log(f"s22 {name}: batched {B / t_b:.1f}/s, plain {1 / t_p:.1f}/s ({rec['batched_vs_plain']:.2f}x), "
-            f"dominant {dom['kernel']} {dom['achieved_gbs']:.0f} GB/s, tuned {tuned} in {tune_s:.1f}s")
+            f"dominant {dom['kernel']} {dom['achieved_gbs']:.0f} GB/s, parity "
+            f"{rec.get('parity_vs_reference', {}).get('max_rel_err', float('nan')):.2e}")
+    ctx.set_option("autotune", 0)
     counts = [64, 40, 16]
     pl = [plans["u5"], plans["tree7"], plans["tree10"]]
     ctx.count_multi(pl, counts, [21, 22, 23])
@@ -263,57 +451,117 @@ def s22_variant(sync, with_grad=True):
     out["mixed"] = {"templates": ["u5", "tree7", "tree10"], "colorings": counts,
                     "colorings_per_s": sum(counts) / t_m, "seconds": t_m}
     log(f"s22 mixed {counts}: {sum(counts) / t_m:.1f} colorings/s")
-    if with_grad:
-        import torch
-        ctx.set_option("autotune", 1)  # forward and gradient calls: settings tuned per (plan, tile)
-        plan = ctx.plan(12, TREE12)
-        rec = {}
-        for wname in ("unit", "uniform_0.5_1.5"):
-            if wname == "unit":
-                ctx.set_weights(None)
-            else:
-                ctx.set_weights(hashed_weights(n, colptr, rowidx))
-            t_f = _timed(lambda r=0: ctx.count(plan, 16, seed=400 + r), 1, sync)
-            fwd_tile = ctx.get_option("last_tile")
-            tuned = {k: ctx.get_option(k) for k in TUNED_KEYS}
-            gbuf = torch.empty(len(rowidx), dtype=torch.float32, device="cuda")  # the autograd path's output
-            t_fb = _timed(lambda r=0: ctx.count_grad_device(plan, 16, gbuf.data_ptr(), seed=500 + r), 1, sync)
-            rec[wname] = {"fwd_colorings_per_s": 16 / t_f, "fwd_bwd_colorings_per_s": 16 / t_fb,
-                          "fwd_bwd_vs_fwd": t_f / t_fb, "fwd_colorings_per_pass": fwd_tile,
-                          "fwd_bwd_colorings_per_pass": ctx.get_option("last_tile"),
-                          "adjoint_tables_recomputed": bool(ctx.get_option("last_recompute")),
-                          "tuned_fwd": tuned}
-            # the same with every table rebuilt from its subtree in the reverse sweep
-            ctx.set_option("adjoint_recompute", 1)
-            t_rc = _timed(lambda r=0: ctx.count_grad_device(plan, 16, gbuf.data_ptr(), seed=600 + r), 1, sync)
-            rec[wname]["fwd_bwd_recompute_colorings_per_s"] = 16 / t_rc
-            rec[wname]["fwd_bwd_recompute_colorings_per_pass"] = ctx.get_option("last_tile")
-            ctx.set_option("adjoint_recompute", -1)
-            # the plain method: one coloring per call, forward and forward+backward
-            t_pf = _timed(lambda r=0: ctx.count(plan, 1, seed=700 + r), 2, sync)
-            t_pfb = _timed(lambda r=0: ctx.count_grad_device(plan, 1, gbuf.data_ptr(), seed=800 + r), 2, sync)
-            rec[wname].update({"plain_fwd_colorings_per_s": 1 / t_pf, "plain_fwd_bwd_colorings_per_s": 1 / t_pfb,
-                               "fwd_vs_plain_fwd": t_pf * 16 / t_f, "fwd_bwd_vs_plain_fwd_bwd": t_pfb * 16 / t_fb})
-            log(f"s22 tree12 {wname}: fwd {16 / t_f:.2f}/s ({fwd_tile}/pass; plain {1 / t_pf:.2f}/s), "
-                f"fwd+bwd {16 / t_fb:.2f}/s ({rec[wname]['fwd_bwd_colorings_per_pass']}/pass, recompute "
-                f"{rec[wname]['adjoint_tables_recomputed']}; plain {1 / t_pfb:.2f}/s); tables rebuilt: "
-                f"{16 / t_rc:.2f}/s ({rec[wname]['fwd_bwd_recompute_colorings_per_pass']}/pass)")
-            if wname == "unit":
-                # %HBM of whole calls, batched and plain
-                roof = {"fwd": call_roofline(ctx, lambda: ctx.count(plan, 16, seed=401), t_f),
-                        "fwd_bwd": call_roofline(
-                            ctx, lambda: ctx.count_grad_device(plan, 16, gbuf.data_ptr(), seed=501), t_fb),
-                        "plain_fwd": call_roofline(ctx, lambda: ctx.count(plan, 1, seed=701), t_pf),
-                        "plain_fwd_bwd": call_roofline(
-                            ctx, lambda: ctx.count_grad_device(plan, 1, gbuf.data_ptr(), seed=801), t_pfb)}
-                rec[wname]["roofline"] = roof
-                log("s22 tree12 unit %HBM of whole calls: " + ", ".join(
-                    f"{k} {v['call_gbs']:.0f} GB/s ({100 * v['call_frac']:.2f}%)" for k, v in roof.items()))
-        out["tree12_fwd_bwd"] = {"colorings_per_call": 16, **rec,
-                                 "note": "roofline: algorithmic bytes of one call (forward: each table written "
-                                         "and read once, the CSC once per gathering launch; backward: the same "
-                                         "units plus the FP64 per-entry gradient read and written by each launch "
-                                         "that adds to it) over the call's graph-replayed time"}
+    return out
+
+
+PLAIN_S20 = "plain_rmat_s20_ef8_tree7"
+
+
+def plain_s20(args, sync):
+    """The reference's plain method on the single-GPU drop-in configuration:
+    R-MAT s20/ef8 (seed 1), the 7-vertex tree, one coloring per step drawn
+    on the device, fixed settings (the autotuner's choice for this shape),
+    with its dominant kernel's roofline and the compiled reference on host
+    cores; FP64 tables and 64 colorings per pass beside it."""
+    import subgraph_amd as sg
+    t0 = time.perf_counter()
+    src, dst = sg.rmat_edges(20, 8, seed=1)
+    n_decl = 1 << 20
+    n, s2, d2 = sg.remove_holes(n_decl, src, dst)
+    colptr, rowidx = sg.csc_from_edges(n, s2, d2)
+    log(f"s20 graph: n={n} nnz={len(rowidx)} ({time.perf_counter() - t0:.1f}s)")
+    ctx = sg.Context(0)
+    for kv in filter(None, args.plain_settings.split(",")):
+        key, val = kv.split("=")
+        ctx.set_option(key.strip(), int(val))
+    ctx.upload_graph(colptr, rowidx)
+    st = ctx.stats()
+    plan = ctx.plan(7, TREE7)
+    settings = {k: ctx.get_option(k) for k in TUNED_KEYS}
+    steps = max(args.steps, 20)
+    for w in range(3):
+        ctx.count(plan, 1, seed=1000 + w)
+    sync()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        ctx.count(plan, 1, seed=2000 + k)
+    sync()
+    elapsed = time.perf_counter() - t0
+    value = steps / elapsed
+    agg = kernel_times(ctx, lambda: ctx.count(plan, 1, seed=2000), reps=steps)
+    roof = roofline_of(agg, PLAIN_S20)
+    out = {"value": value, "unit": "colorings/s", "steps": steps, "ms_per_step": 1e3 * elapsed / steps,
+           "workload": PLAIN_S20,
+           "graph": "R-MAT scale 20 edge factor 8 (a=.57 b=.19 c=.19 seed 1), dedup, symmetrized",
+           "template": "7-vertex tree 0-1,1-2,2-3,1-4,4-5,4-6", "colorings_per_step": 1,
+           "n": int(n), "n_active": st["n_active"], "nnz": int(len(rowidx)), "settings": settings,
+           "roofline": roof}
+    fp = ctx.footprint(plan, 1)
+    out["algorithmic_gbs_whole_pass"] = fp["algorithmic_bytes_per_coloring"] * value / 1e9
+    log(f"plain s20: {value:.1f} colorings/s; dominant {roof['kernel']} {roof['avg_launch_ms']:.4f} ms/launch, "
+        f"{roof['achieved']:.0f} GB/s algorithmic ({100 * roof['frac']:.2f}%)")
+    ctx.count(plan, 1, seed=900, precision="fp64")
+    f64 = _timed(lambda r=0: ctx.count(plan, 1, seed=2000 + r, precision="fp64"), steps, sync)
+    out["fp64"] = {"value": 1 / f64, "vs_fp32": (1 / f64) / value}
+    B = 64
+    ctx.set_option("autotune", 1)
+    ctx.count(plan, B, seed=7)  # tunes this shape (untimed)
+    sync()
+    bsettings = {k: ctx.get_option(k) for k in TUNED_KEYS}
+    bsettings["colorings_per_pass"] = ctx.get_option("last_tile")
+    t_b = _timed(lambda r=0: ctx.count(plan, B, seed=3000 + r), 3, sync)
+    out["batched"] = {"colorings_per_call": B, "value": B / t_b, "vs_plain": (B / t_b) / value,
+                      "settings": bsettings}
+    ctx.set_option("autotune", 0)
+    log(f"plain s20 fp64 {1 / f64:.1f}/s; 64 per pass {B / t_b:.1f}/s ({(B / t_b) / value:.2f}x)")
+    if not args.no_cpu:
+        seed = 4242
+        try:
+            with tempfile.TemporaryDirectory(dir="/tmp") as tmp:
+                gp = os.path.join(tmp, "g.el")
+                write_el(gp, n_decl, src, dst)
+                out["cpu_baseline"] = cpu_baseline(
+                    gp, 7, TREE7, seed, args.cpu_colorings, lambda c: ctx.count(plan, c, seed=seed).tolist(),
+                    f"{args.cpu_colorings} colorings (stream seed {seed}) of this workload")
+            cb = out["cpu_baseline"]
+            log(f"plain s20 cpu: {cb['value']:.3f} colorings/s on {cb['cores']} threads, parity "
+                f"{cb['parity_max_rel_err']:.2e}")
+        except (subprocess.SubprocessError, OSError) as e:
+            out["cpu_baseline"] = {"value": None, "sample": f"failed: {e}"}
+    return out
+
+
+def fwd_bwd_variant(sync):
+    """Differentiation workload: R-MAT s18/ef16 with symmetric weights
+    U(0.5,1.5), 5-star and 5-path, 16 colorings per call, forward + adjoint
+    (per-entry gradient), beside the plain one-coloring forward on the same
+    graph, autotuned.  FP32 tables."""
+    import torch
+    import subgraph_amd as sg
+    n = 1 << 18
+    src, dst = sg.rmat_edges(18, 16, seed=1)
+    colptr, rowidx = sg.csc_from_edges(n, src, dst)
+    ctx = sg.Context(0)
+    ctx.upload_graph(colptr, rowidx, vals=hashed_weights(n, colptr, rowidx))
+    ctx.set_option("autotune", 1)  # tuned per (plan, tile) in each measurement's untimed first call
+    out = {"graph": "R-MAT scale 18 edge factor 16 (seed 1), weights U(0.5,1.5) symmetric",
+           "nnz": int(len(rowidx)), "colorings_per_call": 16, "dtype": "fp32", "templates": {}}
+    grad = torch.empty(len(rowidx), dtype=torch.float32, device="cuda")  # the autograd path's output
+    for name, te in (("star5", [(0, 1), (0, 2), (0, 3), (0, 4)]), ("u5", U5)):
+        plan = ctx.plan(5, te)
+        t_plain = _timed(lambda r=0: ctx.count(plan, 1, seed=50 + r), 10, sync)
+        t_fwd = _timed(lambda r=0: ctx.count(plan, 16, seed=60 + r), 5, sync)
+        t_fb1 = _timed(lambda r=0: ctx.count_grad_device(plan, 1, grad.data_ptr(), seed=80 + r), 10, sync)
+        t_fb = _timed(lambda r=0: ctx.count_grad_device(plan, 16, grad.data_ptr(), seed=70 + r), 5, sync)
+        rec = {"plain_fwd_colorings_per_s": 1.0 / t_plain, "batched_fwd_colorings_per_s": 16 / t_fwd,
+               "plain_fwd_bwd_colorings_per_s": 1.0 / t_fb1, "fwd_bwd_colorings_per_s": 16 / t_fb}
+        rec["fwd_bwd_vs_plain_fwd_bwd"] = rec["fwd_bwd_colorings_per_s"] / rec["plain_fwd_bwd_colorings_per_s"]
+        rec["fwd_bwd_vs_plain_fwd"] = rec["fwd_bwd_colorings_per_s"] / rec["plain_fwd_colorings_per_s"]
+        out["templates"][name] = rec
+        log(f"fwd+bwd s18 {name}: plain fwd {rec['plain_fwd_colorings_per_s']:.0f}/s, batched fwd "
+            f"{rec['batched_fwd_colorings_per_s']:.0f}/s, plain fwd+bwd {rec['plain_fwd_bwd_colorings_per_s']:.0f}/s, "
+            f"fwd+bwd {rec['fwd_bwd_colorings_per_s']:.0f}/s")
+    out["note"] = "the gradient stays on the device as float32 (the torch.autograd path)"
     return out
 
 
@@ -327,253 +575,89 @@ def copy_calibration(mib):
     log(f"copy calibration: {mib} MiB read + {mib} MiB written")
 
 
-# settings the autotuner picked for the 12-vertex tree in an s22 measurement; between
-# runs it also picks direct loads or hubs from 1024, within about 1% (every gather of
-# this template runs 64-lane teams, so the bucket thresholds do not matter)
-TREE12_SETTINGS = {"stage_rows": 1, "bucket_high": 32, "bucket_mid": 8, "hub_degree": 512, "hub_chunk": 1024}
-
-
-def tree12_variant(sync):
-    """The 12-vertex tree on R-MAT s22/ef16 alone, unit weights, with fixed
-    settings (TREE12_SETTINGS; 4 colorings per forward pass, 3 per adjoint
-    pass, as tuned): 16 colorings forward, 16 forward+backward, and one of
-    each (the plain method).  A short run with no tuning passes, for
-    profilers.  FP32 tables."""
-    import torch
-    import subgraph_amd as sg
-    n = 1 << 22
-    src, dst = sg.rmat_edges(22, 16, seed=1)
-    colptr, rowidx = sg.csc_from_edges(n, src, dst)
-    del src, dst
-    ctx = sg.Context(0, **TREE12_SETTINGS)
-    ctx.upload_graph(colptr, rowidx)
-    plan = ctx.plan(12, TREE12)
-    gbuf = torch.empty(len(rowidx), dtype=torch.float32, device="cuda")
-    out = {"graph": "R-MAT scale 22 edge factor 16 (a=.57 b=.19 c=.19 seed 1), dedup, symmetrized",
-           "template": "12-vertex tree", "settings": dict(TREE12_SETTINGS)}
-    for name, tile, N, fn in (
-            ("fwd", 4, 16, lambda r=0: ctx.count(plan, 16, seed=400 + r)),
-            ("fwd_bwd", 3, 16, lambda r=0: ctx.count_grad_device(plan, 16, gbuf.data_ptr(), seed=500 + r)),
-            ("plain_fwd", 1, 1, lambda r=0: ctx.count(plan, 1, seed=700 + r)),
-            ("plain_fwd_bwd", 1, 1, lambda r=0: ctx.count_grad_device(plan, 1, gbuf.data_ptr(), seed=800 + r))):
-        ctx.set_option("batch_tile", tile)
-        t = _timed(fn, 1 if N > 1 else 2, sync)
-        out[name] = {"colorings_per_s": N / t, "colorings_per_pass": ctx.get_option("last_tile")}
-        log(f"tree12 {name}: {N / t:.2f} colorings/s ({ctx.get_option('last_tile')}/pass)")
-    return out
-
-
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--colorings", type=int, default=16, help="colorings per step of the primary line")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-batched", action="store_true")
-    ap.add_argument("--cpu-colorings", type=int, default=12)
-    ap.add_argument("--variants", default="grad,s22",
-                    help="comma list of extra measurements: grad, s22, tree12 (the s22 12-tree part alone), none")
+    ap.add_argument("--cpu-colorings", type=int, default=12, help="reference colorings of the plain_s20 baseline")
+    ap.add_argument("--variants", default="plain,s22,grad",
+                    help="comma list of extra measurements: plain (s20 plain method), s22 (batching), "
+                         "grad (s18 differentiation), none")
+    ap.add_argument("--autotune", type=int, default=1, choices=(0, 1),
+                    help="primary line: 1 = the autotuner picks settings and colorings per pass in the untimed "
+                         "first calls; 0 = fixed settings (TREE12_SETTINGS, plus --settings)")
+    ap.add_argument("--settings", default="", metavar="KEY=VAL,...", help="primary line context options")
     ap.add_argument("--plain-settings",
                     default="stage_rows=1,bucket_high=512,bucket_mid=64,hub_degree=512,hub_chunk=1024",
-                    metavar="KEY=VAL,...", help="context options of the plain line ('' = library defaults)")
-    ap.add_argument("--variants-only", action="store_true",
-                    help="run only the --variants measurements (a short run for profilers; no primary line)")
-    ap.add_argument("--autotune", type=int, default=0, choices=(0, 1),
-                    help="1: the autotuner picks the plain line's settings, in a call before the warmup steps")
+                    metavar="KEY=VAL,...", help="context options of the plain_s20 line")
+    ap.add_argument("--only", default="", choices=("", "headline", "plain"),
+                    help="run just that measurement (a short run for profilers); headline skips its comparisons")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run one elementwise kernel that reads and writes MIB MiB (the known-size "
                          "kernel rocprofv3 FETCH_SIZE / WRITE_SIZE readings are scaled by)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print(json.dumps({"error": "this build measures one GPU: bench.py supports --gpus 1 only; "
-                                   "colorings would shard across GPUs, each rank counting its share of the coloring stream"}))
+                                   "colorings would shard across GPUs, each rank counting its share of the "
+                                   "coloring stream"}))
         sys.exit(2)
-
-    import subgraph_amd as sg
-
-    if args.variants_only:
-        import torch
-        result = {"metric": "colorings/sec + %HBM roofline; batched and fwd+bwd vs plain DP, same GPU",
-                  "note": "--variants-only: extra measurements only, no primary line"}
-        if args.copy_calibration > 0:
-            copy_calibration(args.copy_calibration)
-        run_variants(result, args.variants.split(","), torch.cuda.synchronize)
-        print(json.dumps(result), flush=True)
-        return
-
+    import torch
+    sync = torch.cuda.synchronize
     if args.copy_calibration > 0:
         copy_calibration(args.copy_calibration)
-
-    scale, ef = 20, 8
-    t0 = time.perf_counter()
-    src, dst = sg.rmat_edges(scale, ef, seed=1)
-    n_decl = 1 << scale
-    n, s2, d2 = sg.remove_holes(n_decl, src, dst)  # the reference text loader's rule
-    colptr, rowidx = sg.csc_from_edges(n, s2, d2)
-    log(f"graph: n={n} undirected={len(src)} nnz={len(rowidx)} ({time.perf_counter() - t0:.1f}s)")
-
-    ctx = sg.Context(0)
-    for kv in filter(None, args.plain_settings.split(",")):
-        key, val = kv.split("=")
-        ctx.set_option(key.strip(), int(val))
-    ctx.upload_graph(colptr, rowidx)
-    st = ctx.stats()
-    plan = ctx.plan(7, TREE7)
-    log(f"uploaded: n_active={st['n_active']} max_degree={st['max_degree']}")
-
-    try:
-        import torch
-        sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
-    except Exception:  # torch is plumbing only
-        sync = lambda: None  # noqa: E731
-
-    tuning_s = 0.0
-    if args.autotune:
-        ctx.set_option("autotune", 1)
-        t0 = time.perf_counter()
-        ctx.count(plan, 1, seed=999)  # tunes this shape once, before the warmup and the timed region
-        sync()
-        tuning_s = time.perf_counter() - t0
-    settings = {k: ctx.get_option(k) for k in TUNED_KEYS}
-    log(f"plain settings ({'autotuned in %.1fs' % tuning_s if args.autotune else 'as set'}): {settings}")
-    for w in range(args.warmup):
-        ctx.count(plan, 1, seed=1000 + w)
-    # timed region: the product path (each pass replays its captured hipGraph)
-    sync()
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        ctx.count(plan, 1, seed=2000 + k)
-    sync()
-    elapsed = time.perf_counter() - t0
-    value = args.steps / elapsed
-    # roofline region: the same K steps launched eagerly with a HIP event pair
-    # around every kernel on the library's stream (graphs carry no events)
-    ctx.set_option("timing", 1)
-    agg = {}
-    sync()
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        ctx.count(plan, 1, seed=2000 + k)
-        for name, (ms, nbytes, launches) in ctx.last_timing_kernels().items():
-            a = agg.setdefault(name, [0.0, 0.0, 0])
-            a[0] += ms
-            a[1] += nbytes
-            a[2] += launches
-    sync()
-    elapsed_ev = time.perf_counter() - t0
-    ctx.set_option("timing", 0)
-    if not agg or max(v[0] for v in agg.values()) <= 0:
-        raise RuntimeError("per-launch HIP event timing returned no durations")
-    kern, (kms, kbytes, klaunch) = max(agg.items(), key=lambda kv: kv[1][0])
-    achieved = kbytes / (kms * 1e-3) / 1e9
-    log(f"plain: {value:.1f} colorings/s ({1e3 * elapsed / args.steps:.3f} ms/step; eager with events "
-        f"{1e3 * elapsed_ev / args.steps:.3f} ms/step); dominant {kern} {kms / klaunch:.4f} ms/launch, "
-        f"{achieved:.0f} GB/s algorithmic")
-
-    result = {
-        "metric": "colorings/sec + %HBM roofline; batched and fwd+bwd vs plain DP, same GPU",
-        "value": value, "unit": "colorings/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": 1e3 * elapsed / args.steps, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
-        "config": {"workload": "plain_rmat_s20_ef8_tree7",
-                   "graph": "R-MAT scale 20 edge factor 8 (a=.57 b=.19 c=.19 seed 1), dedup, symmetrized",
-                   "template": "7-vertex tree 0-1,1-2,2-3,1-4,4-5,4-6",
-                   "colorings_per_step": 1, "n": int(n), "n_active": st["n_active"], "nnz": int(len(rowidx)),
-                   "settings": settings, "autotuned": bool(args.autotune), "tuning_seconds": tuning_s},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(kern), "kernel": kern,
-                     "launches": klaunch, "avg_launch_ms": kms / klaunch,
-                     "algorithmic_bytes_per_launch": kbytes / klaunch},
-    }
-    # the reference's cost models (CountMat::estimate*, GB/Gflop = 2^30) over
-    # the measured time per coloring, beside this library's own byte model
-    est = sg.estimate_model(n, len(rowidx), 7, TREE7)
-    fp = ctx.footprint(plan, 1)
-    sec = elapsed / args.steps
-    gib = float(1 << 30)
-    result["estimates"] = {
-        "reference_model": {k: est[k] for k in ("peak_mem_gb", "pgbsc_spmv_gbytes", "pgbsc_fma_gbytes",
-                                                "pgbsc_spmv_gflop", "pgbsc_fma_gflop")},
-        "reference_model_gbs": (est["pgbsc_spmv_gbytes"] + est["pgbsc_fma_gbytes"]) * gib / sec / 1e9,
-        "reference_model_gflops": (est["pgbsc_spmv_gflop"] + est["pgbsc_fma_gflop"]) * gib / sec / 1e9,
-        "algorithmic_bytes_per_coloring": fp["algorithmic_bytes_per_coloring"],
-        "algorithmic_gbs": fp["algorithmic_bytes_per_coloring"] / sec / 1e9,
-        "device_bytes_one_coloring_pass": fp["device_bytes"],
-        "note": "reference_model_*: the reference's PGBSC SpMV + eMA byte and flop models per coloring over the "
-                "measured ms_per_step (decimal GB/s and Gflop/s); algorithmic_*: each table written and read "
-                "once, CSC once per gathering launch"}
-    log(f"models: reference PGBSC {result['estimates']['reference_model_gbs']:.0f} GB/s, "
-        f"{result['estimates']['reference_model_gflops']:.0f} Gflop/s; algorithmic "
-        f"{result['estimates']['algorithmic_gbs']:.0f} GB/s over the whole pass")
-
-    if not args.no_batched:
-        # the FP64 tables (exact counts below 2^53), same plain settings
-        ctx.count(plan, 1, seed=900, precision="fp64")
-        sync()
-        t0 = time.perf_counter()
-        for k in range(args.steps):
-            ctx.count(plan, 1, seed=2000 + k, precision="fp64")
-        sync()
-        f64 = args.steps / (time.perf_counter() - t0)
-        result["fp64"] = {"value": f64, "unit": "colorings/s", "vs_fp32": f64 / value,
-                          "workload": "the plain line with FP64 tables"}
-        log(f"fp64 plain: {f64:.1f} colorings/s ({f64 / value:.2f}x the FP32 line)")
-
-        B = 64
-        ctx.set_option("autotune", 1)
-        t0 = time.perf_counter()
-        ctx.count(plan, B, seed=7)  # tunes this shape (untimed)
-        sync()
-        btune_s = time.perf_counter() - t0
-        bsettings = {k: ctx.get_option(k) for k in TUNED_KEYS}
-        bsettings["colorings_per_pass"] = ctx.get_option("last_tile")
-        t0 = time.perf_counter()
-        reps = 3
-        for r in range(reps):
-            ctx.count(plan, B, seed=3000 + r)
-        sync()
-        bel = time.perf_counter() - t0
-        bval = reps * B / bel
-        result["batched"] = {"colorings_per_call": B, "value": bval, "unit": "colorings/s",
-                             "vs_plain": bval / value, "workload": "same graph and template as the plain line",
-                             "settings": bsettings, "tuning_seconds": btune_s}
-        log(f"batched: {bval:.1f} colorings/s ({bval / value:.2f}x plain)")
-
-
-    if not args.no_cpu:
-        seed = 4242
-        log("cpu baseline: running the compiled reference on host cores ...")
-
-        def gpu_raw(m):
-            return ctx.count(plan, m, seed=seed).tolist()
-
-        try:
-            result["cpu_baseline"] = cpu_baseline(n_decl, src, dst, seed, args.cpu_colorings, gpu_raw)
-            cb = result["cpu_baseline"]
-            log(f"cpu: {cb['value']:.3f} colorings/s on {cb['cores']} threads ({cb['kind']}), "
-                f"parity max rel err {cb['parity_max_rel_err']:.2e}")
-        except (subprocess.SubprocessError, OSError) as e:
-            result["cpu_baseline"] = {"value": None, "unit": "colorings/s", "cores": physical_cores(),
-                                      "kind": "reference", "sample": f"failed: {e}"}
+    if args.only == "plain":
+        args.no_cpu = True
+        print(json.dumps({"plain_s20": plain_s20(args, sync)}), flush=True)
+        return
+    if args.only == "headline":
+        args.no_cpu = True
+        result = headline_only(args, sync)
+        print(json.dumps(result), flush=True)
+        return
+    result, ctx, graph = headline(args, sync)
     variants = args.variants.split(",")
-    if "grad" in variants or "s22" in variants or "tree12" in variants:
-        del ctx, plan
-    run_variants(result, variants, sync)
-    print(json.dumps(result), flush=True)
-
-
-def run_variants(result, variants, sync):
-    """The extra measurements, each under its key of `result`; they never
-    cost the primary line: a failure is recorded."""
-    for name, key, fn in (("grad", "fwd_bwd", fwd_bwd_variant), ("s22", "s22", s22_variant),
-                          ("tree12", "s22_tree12", tree12_variant)):
+    for name, key, fn in (("s22", "s22_batching", lambda: s22_batching(ctx, graph, sync)),
+                          ("plain", "plain_s20", lambda: plain_s20(args, sync)),
+                          ("grad", "fwd_bwd", lambda: fwd_bwd_variant(sync))):
         if name not in variants:
             continue
         try:
-            result[key] = fn(sync)
-        except Exception as e:  # noqa: BLE001
+            result[key] = fn()
+        except Exception as e:  # noqa: BLE001  (an extra never costs the primary line)
             log(f"variant {name} failed: {e!r}")
             result[key] = {"error": repr(e)}
+        if name == "s22":
+            del ctx
+            graph = None
+    print(json.dumps(result), flush=True)
+
+
+def headline_only(args, sync):
+    """The primary line's timed steps alone (no comparisons, no CPU
+    baseline): a short run whose kernel trace and counters hold only the
+    headline's work plus its untimed first calls."""
+    import torch
+    import subgraph_amd as sg
+    n, m, colptr, rowidx = s22_graph()
+    ctx = sg.Context(0)
+    for k, v in TREE12_SETTINGS.items():
+        ctx.set_option(k, v)
+    for kv in filter(None, args.settings.split(",")):
+        key, val = kv.split("=")
+        ctx.set_option(key.strip(), int(val))
+    ctx.upload_graph(colptr, rowidx)
+    plan = ctx.plan(12, TREE12)
+    gbuf = torch.empty(len(rowidx), dtype=torch.float32, device="cuda")
+    B = args.colorings
+    for k in range(args.warmup + args.steps):
+        t0 = time.perf_counter()
+        ctx.count_grad_device(plan, B, gbuf.data_ptr(), seed=5000 + k)
+        sync()
+        log(f"step {k}: {B / (time.perf_counter() - t0):.2f} colorings/s ({ctx.get_option('last_tile')} per pass)")
+    return {"workload": HEADLINE, "settings": {k: ctx.get_option(k) for k in TUNED_KEYS}}
 
 
 if __name__ == "__main__":

@@ -41,6 +41,17 @@ extern "C" {
 /* flags */
 #define PG_FLAG_FIX_2VERTEX 1u /* do not multiply two-vertex templates by 1e12
                                   (the reference does, CountMat.cpp:811-824) */
+#define PG_FLAG_UNPRUNED 2u    /* pg_count only, PG_FP32 only: the reference's unpruned
+                                  color coding (isPruned = 0, countNonBottomeOriginal,
+                                  CountMat.cpp:851-977) with its numerics: unscaled
+                                  FP32 tables, one SpMV per (color set, split) pair
+                                  added onto an accumulator that is zeroed only at
+                                  pg_graph_upload and carried across pairs,
+                                  subtemplates, colorings and calls (the reference
+                                  clears _bufVec once, CountMat.cpp:52-59), a float
+                                  root buffer summed in FP64 in vertex order, no 1e12.
+                                  Colorings run one after another, slowly: it is the
+                                  reference's baseline mode, kept for CLI parity */
 
 typedef struct pg_ctx pg_ctx;
 typedef struct pg_plan pg_plan;
@@ -217,6 +228,15 @@ int pg_last_timing_kernel(const pg_ctx* ctx, char* buf, int len);
 int pg_last_timing_entry(const pg_ctx* ctx, int idx, char* name, int len, double* ms,
                          int64_t* launches, double* bytes);
 
+/* the same launches broken down by (subtemplate, kernel): the per-subtemplate
+ * "Sub %d, counting time ..." report of the reference's VERBOSE build
+ * (CountMat.cpp:833-840, 967-970).  *sub is the subtemplate index (forward
+ * and adjoint steps of it alike), -1 for launches outside any step (colors,
+ * the neighbour-color histogram, root reductions).  Entries are ordered by
+ * (sub, name).  Returns the number of entries, or PG_ERR_RANGE past the end. */
+int pg_last_timing_sub(const pg_ctx* ctx, int idx, int* sub, char* name, int len, double* ms,
+                       int64_t* launches, double* bytes);
+
 /* ---- host surface helpers used by the CLI and the Python layer ---- */
 /* R-MAT generator: n = 2^scale, edge_factor*n draws with quadrant
  * probabilities a,b,c (d = 1-a-b-c), self loops dropped, (min,max)
@@ -229,6 +249,13 @@ int pg_rmat_edges(int scale, int edge_factor, double a, double b, double c, uint
  * colptr[n+1], rowidx[2m] */
 int pg_csc_from_edges(int32_t n, int64_t m, const int32_t* src, const int32_t* dst,
                       int32_t* colptr, int32_t* rowidx);
+/* Replaces CountMat::automorphismNum (CountMat.cpp:1498-1572): the number of
+ * permutations s of the template's vertices that the reference's
+ * calcAutomorphismZero (CountMat.cpp:1510-1544) accepts, i.e. deg(i) =
+ * deg(s(i)) and N(i) a subset of s(N(s(i))) for every vertex i; for most
+ * trees this is the automorphism group's order.  Host only.  Returns the
+ * count, or PG_ERR_INVALID. */
+int64_t pg_template_automorphisms(int nv, int ne, const int32_t* src, const int32_t* dst);
 /* The text loader's "hole" rule (EdgeList.cpp:70-112) on an in-memory list:
  * if the largest id is not n-1, the ids that occur are renumbered 0.. in
  * ascending order.  Returns the resulting n (or a negative PG_ERR_*). */

@@ -20,11 +20,32 @@
 //    raw double with full precision and its wall time (the scope of the
 //    reference's "Time for count per iter", CountMat.cpp:200-213).
 //
-// Usage: pg_ref <graph.el> <template.fascia> <colors.u8|-> <ncolorings> <threads> [seed]
+// Usage: pg_ref <graph.el> <template.fascia> <colors.u8|-> <ncolorings> <threads> [seed] [isPruned] [useSPMM]
 //   colors.u8: ncolorings*n bytes, coloring-major; "-" draws colors from the
 //   library's color stream (seed, coloring, vertex) documented in pgbsc.h.
+//   isPruned / useSPMM (default 1 1) are CountMat::initialization's mode
+//   flags: 1 0 runs countNonBottomePruned (CountMat.cpp:325-554), 0 x the
+//   unpruned countNonBottomeOriginal (CountMat.cpp:851-977).
 // Output lines: "PGREF n <n> nnz <nnz> k <k>" then per coloring
 //   "PGREF coloring <i> raw <%.17g> seconds <t>".
+//
+//        pg_ref compute <graph.el> <template.fascia> <ncolorings> <threads> <seed> <isPruned> <useSPMM> <autom>
+//   runs the whole CountMat::compute(tmpl, false) (CountMat.cpp:79-266), as
+//   sc-main.cpp:1793-1795 does, with the color stream: each thread's n-th
+//   srand() call (one per colorInit) selects coloring n-1.  The reference
+//   prints its own "Final raw count is", "Prob is" and "Final count is"
+//   lines; the driver adds "PGREF final <%.17g>" with compute()'s return.
+//
+//        pg_ref serialize <graph.el> <out.data>
+//   writes the graph through the reference's CSCGraph::serialize
+//   (CSCGraph.hpp:426-434), the graph.data of sc-main's write_binary.
+//
+// Every <graph> argument may also be a binary graph.data (a path ending in
+// ".data"), read with the reference's CSCGraph::deserialize.
+//
+//        pg_ref autom <template.fascia>
+//   prints "PGREF autom <n>", CountMat::automorphismNum() (CountMat.cpp:
+//   1498-1572) of the template.
 //
 //        pg_ref dump <graph.el> <template.fascia>
 //   prints the reference's CSC ("PGREF colptr ...", "PGREF rowidx ...") and
@@ -56,6 +77,8 @@
 #include <algorithm>
 #include <omp.h>
 #include <sys/time.h>
+#include <fcntl.h>
+#include <unistd.h>
 
 #define private public
 #include "CountMat.hpp"
@@ -69,6 +92,9 @@ static int64_t g_coloring = 0;
 static bool g_hash = false;
 static uint64_t g_k = 1;
 static thread_local int64_t t_pos = 0;
+static bool g_auto = false;                 // compute mode: coloring = this thread's srand() calls - 1
+static thread_local int64_t t_calls = 0;
+static thread_local int64_t t_coloring = 0;
 
 static inline uint64_t splitmix64(uint64_t x)
 {
@@ -86,6 +112,7 @@ extern "C" void srand(unsigned) noexcept
     int64_t i = omp_get_thread_num();
     int64_t q = g_n / T, r = g_n % T;
     t_pos = (i < r) ? (q + 1) * i : q * i + r;
+    t_coloring = g_auto ? t_calls++ : g_coloring;
 }
 
 // In hash mode the color is the library's documented color stream
@@ -94,10 +121,28 @@ extern "C" int rand(void) noexcept
 {
     int64_t v = t_pos++;
     if (g_hash) {
-        uint64_t h = splitmix64(g_seed ^ splitmix64((uint64_t)g_coloring * 0x9E3779B97F4A7C15ull + (uint64_t)v));
+        uint64_t h = splitmix64(g_seed ^ splitmix64((uint64_t)t_coloring * 0x9E3779B97F4A7C15ull + (uint64_t)v));
         return (int)(((h >> 32) * (uint64_t)g_k) >> 32);
     }
     return g_colors[v];
+}
+
+// A text edge list (EdgeList -> createFromEdgeListFile, sc-main.cpp:1692,
+// 1762) or, for a path ending in ".data", the binary CSC through the
+// reference's own CSCGraph::deserialize (sc-main.cpp:1659-1675).
+static CSCGraph<int32_t, float>* load_graph(const std::string& path)
+{
+    CSCGraph<int32_t, float>* g = new CSCGraph<int32_t, float>();
+    if (path.size() > 5 && path.compare(path.size() - 5, 5, ".data") == 0) {
+        int fd = open(path.c_str(), O_RDONLY);
+        if (fd < 0) { perror(path.c_str()); exit(2); }
+        g->deserialize(fd);
+        close(fd);
+        return g;
+    }
+    EdgeList elist(path);
+    g->createFromEdgeListFile(elist.getNumVertices(), elist.getNumEdges(), elist.getSrcList(), elist.getDstList(), false);
+    return g;
 }
 
 static double now_s()
@@ -116,9 +161,7 @@ static void print_ints(const char* tag, int s, const int* a, long long len)
 
 static int dump(const std::string& graph_name, const std::string& tmpl_name)
 {
-    EdgeList elist(graph_name);
-    CSCGraph<int32_t, float>* g = new CSCGraph<int32_t, float>();
-    g->createFromEdgeListFile(elist.getNumVertices(), elist.getNumEdges(), elist.getSrcList(), elist.getDstList(), false);
+    CSCGraph<int32_t, float>* g = load_graph(graph_name);
     int n = g->getNumVertices();
     print_ints("colptr", n, g->getIndexCol(), (long long)n + 1);
     print_ints("rowidx", n, g->getIndexRow(), (long long)g->getNNZ());
@@ -155,9 +198,7 @@ static int dump(const std::string& graph_name, const std::string& tmpl_name)
 
 static int estimate(const std::string& graph_name, const std::string& tmpl_name)
 {
-    EdgeList elist(graph_name);
-    CSCGraph<int32_t, float>* g = new CSCGraph<int32_t, float>();
-    g->createFromEdgeListFile(elist.getNumVertices(), elist.getNumEdges(), elist.getSrcList(), elist.getDstList(), false);
+    CSCGraph<int32_t, float>* g = load_graph(graph_name);
     g->splitCSC(4);
     Graph tmpl;
     tmpl.read_enlist(tmpl_name);
@@ -191,9 +232,7 @@ static int estimate(const std::string& graph_name, const std::string& tmpl_name)
 
 static int estimate_template(const std::string& graph_name, const std::string& tmpl_name)
 {
-    EdgeList elist(graph_name);
-    CSCGraph<int32_t, float>* g = new CSCGraph<int32_t, float>();
-    g->createFromEdgeListFile(elist.getNumVertices(), elist.getNumEdges(), elist.getSrcList(), elist.getDstList(), false);
+    CSCGraph<int32_t, float>* g = load_graph(graph_name);
     g->splitCSC(4);
     Graph tmpl;
     tmpl.read_enlist(tmpl_name);
@@ -206,8 +245,67 @@ static int estimate_template(const std::string& graph_name, const std::string& t
     return 0;
 }
 
+// CountMat::compute with the color stream (see the usage above)
+static int compute_mode(int argc, char** argv)
+{
+    if (argc != 10) {
+        fprintf(stderr, "usage: %s compute graph.el template.fascia ncolorings threads seed isPruned useSPMM autom\n",
+                argv[0]);
+        return 2;
+    }
+    const int ncol = atoi(argv[4]), thds = atoi(argv[5]);
+    g_seed = strtoull(argv[6], nullptr, 10);
+    const int isPruned = atoi(argv[7]), useSPMM = atoi(argv[8]), autom = atoi(argv[9]);
+    CSCGraph<int32_t, float>* g = load_graph(argv[2]);
+    g->splitCSC(4 * thds);
+    g_n = g->getNumVertices();
+    Graph tmpl;
+    tmpl.read_enlist(argv[3]);
+    g_k = (uint64_t)tmpl.get_vert_num();
+    g_hash = true;
+    g_auto = true;
+    CountMat ex;
+    ex.initialization(nullptr, g, thds, ncol, isPruned, useSPMM, -1, autom != 0);
+    const double fin = ex.compute(tmpl, false);
+    printf("PGREF final %.17g\n", fin);
+    fflush(stdout);
+    delete g;
+    return 0;
+}
+
+static int autom_mode(const std::string& tmpl_name)
+{
+    Graph tmpl;
+    tmpl.read_enlist(tmpl_name);
+    // never destroyed: without initialization() the destructor would free
+    // buffers it never allocated
+    CountMat* ex = new CountMat();
+    ex->_templates = &tmpl;
+    printf("PGREF autom %d\n", ex->automorphismNum());
+    fflush(stdout);
+    return 0;
+}
+
+// the reference's own CSCGraph::serialize of a text edge list (CSCGraph.hpp:426-434)
+static int serialize_mode(const std::string& graph_name, const std::string& out_name)
+{
+    CSCGraph<int32_t, float>* g = load_graph(graph_name);
+    ofstream out(out_name.c_str(), ios::binary);
+    g->serialize(out);
+    out.close();
+    printf("PGREF serialized n %d nnz %lld\n", g->getNumVertices(), (long long)g->getNNZ());
+    delete g;
+    return 0;
+}
+
 int main(int argc, char** argv)
 {
+    if (argc == 4 && std::string(argv[1]) == "serialize")
+        return serialize_mode(argv[2], argv[3]);
+    if (argc >= 2 && std::string(argv[1]) == "compute")
+        return compute_mode(argc, argv);
+    if (argc == 3 && std::string(argv[1]) == "autom")
+        return autom_mode(argv[2]);
     if (argc == 4 && std::string(argv[1]) == "dump")
         return dump(argv[2], argv[3]);
     if (argc == 4 && std::string(argv[1]) == "estimate")
@@ -222,10 +320,10 @@ int main(int argc, char** argv)
     int ncol = atoi(argv[4]);
     int thds = atoi(argv[5]);
     g_seed = argc > 6 ? strtoull(argv[6], nullptr, 10) : 1;
+    const int isPruned = argc > 7 ? atoi(argv[7]) : 1;
+    const int useSPMM = argc > 8 ? atoi(argv[8]) : 1;
 
-    EdgeList elist(graph_name);
-    CSCGraph<int32_t, float>* g = new CSCGraph<int32_t, float>();
-    g->createFromEdgeListFile(elist.getNumVertices(), elist.getNumEdges(), elist.getSrcList(), elist.getDstList(), false);
+    CSCGraph<int32_t, float>* g = load_graph(graph_name);
     g->splitCSC(4 * thds);
     g_n = g->getNumVertices();
 
@@ -248,12 +346,16 @@ int main(int argc, char** argv)
     fflush(stdout);
 
     CountMat ex;
-    ex.initialization(nullptr, g, thds, ncol, 1, 1, -1, false);
+    ex.initialization(nullptr, g, thds, ncol, isPruned, useSPMM, -1, false);
     ex.compute(tmpl, true);
-    // CountMat.cpp:174-197 (the useSPMM == 1 branch)
+    // CountMat.cpp:174-197: one buffer per color (useSPMM == 0) or one block
     ex._bufVecLeaf = (float**)malloc(k * sizeof(float*));
-    ex._bufVecLeaf[0] = (float*)aligned_alloc(64, (size_t)g_n * k * sizeof(float));
-    for (int i = 1; i < k; ++i) ex._bufVecLeaf[i] = ex._bufVecLeaf[0] + (size_t)i * g_n;
+    if (useSPMM == 0) {
+        for (int i = 0; i < k; ++i) ex._bufVecLeaf[i] = (float*)aligned_alloc(64, (size_t)g_n * sizeof(float));
+    } else {
+        ex._bufVecLeaf[0] = (float*)aligned_alloc(64, (size_t)g_n * k * sizeof(float));
+        for (int i = 1; i < k; ++i) ex._bufVecLeaf[i] = ex._bufVecLeaf[0] + (size_t)i * g_n;
+    }
 
     for (int i = 0; i < ncol; ++i) {
         g_coloring = i;

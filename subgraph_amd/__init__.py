@@ -17,8 +17,8 @@ import math
 
 import numpy as np
 
-from ._lib import (ESTIMATE_FIELDS, PG_FLAG_FIX_2VERTEX, PG_FP32, PG_FP64, PGError, PgEstimate,
-                   exported_symbols, lib)
+from ._lib import (ESTIMATE_FIELDS, PG_FLAG_FIX_2VERTEX, PG_FLAG_UNPRUNED, PG_FP32, PG_FP64, PGError,
+                   PgEstimate, exported_symbols, lib)
 
 __all__ = ["Context", "Plan", "CountMat", "PGError", "rmat_edges", "csc_from_edges", "colors_fill",
            "remove_holes", "exported_symbols", "estimate_model", "PG_FP32", "PG_FP64"]
@@ -209,8 +209,13 @@ class Context:
         return Plan(self, p.value, k)
 
     def count(self, plan: Plan, ncolorings: int, colors=None, seed: int = 0, precision="fp32",
-              fix_2vertex: bool = False) -> np.ndarray:
-        """Raw colorful count of each coloring (CountMat::colorCounting)."""
+              fix_2vertex: bool = False, unpruned: bool = False) -> np.ndarray:
+        """Raw colorful count of each coloring (CountMat::colorCounting).
+
+        unpruned=True runs the reference's isPruned = 0 mode with its
+        numerics (countNonBottomeOriginal, CountMat.cpp:851-977), whose
+        accumulator carries over from one coloring and call to the next until
+        the graph is uploaded again."""
         out = np.zeros(ncolorings, np.float64)
         cp = None
         if colors is not None:
@@ -218,7 +223,7 @@ class Context:
             if colors.size != ncolorings * self.n:
                 raise ValueError("colors must be [ncolorings][n]")
             cp = colors.ctypes.data
-        flags = PG_FLAG_FIX_2VERTEX if fix_2vertex else 0
+        flags = (PG_FLAG_FIX_2VERTEX if fix_2vertex else 0) | (PG_FLAG_UNPRUNED if unpruned else 0)
         self._check(lib().pg_count(self.ptr, plan.ptr, ncolorings, cp, seed, _PREC[precision], flags, out),
                     "pg_count")
         return out
@@ -294,6 +299,27 @@ class Context:
         return out
 
 
+    def last_timing_subs(self) -> dict:
+        """{(subtemplate, kernel): (ms, algorithmic bytes, launches)} of the
+        last call with option timing on; subtemplate -1 = pass-level launches."""
+        out = {}
+        buf = C.create_string_buffer(128)
+        sub, ms, nl, nb = C.c_int(), C.c_double(), C.c_int64(), C.c_double()
+        idx = 0
+        while True:
+            rc = lib().pg_last_timing_sub(self.ptr, idx, C.byref(sub), buf, 128, C.byref(ms), C.byref(nl),
+                                          C.byref(nb))
+            if rc == -6:
+                break
+            if rc < 0:
+                self._check(rc, "pg_last_timing_sub")
+            out[(sub.value, buf.value.decode())] = (ms.value, nb.value, nl.value)
+            idx += 1
+            if idx >= rc:
+                break
+        return out
+
+
 class CountMat:
     """CountMat.hpp:19-191: initialization(graph, ...) then compute(template)."""
 
@@ -305,13 +331,17 @@ class CountMat:
         self.n, self.nnz = len(colptr) - 1, len(rowidx)
         self.itr_num, self.precision, self.seed = itr_num, precision, seed
         self.calculate_automorphisms = calculate_automorphisms
+        # useSPMM = 0 (countNonBottomePruned, CountMat.cpp:325-554) forms the
+        # same FP32 sums in the same order as the SpMM path: one mode here
+        self.isPruned, self.useSPMM = isPruned, useSPMM
 
     def compute(self, nv: int, edges, isEstimate: bool = False) -> float:
         self.template = (nv, edges)
         plan = self.ctx.plan(nv, edges)
         if isEstimate:
             return 0.0
-        self.raw = self.ctx.count(plan, self.itr_num, seed=self.seed, precision=self.precision)
+        self.raw = self.ctx.count(plan, self.itr_num, seed=self.seed, precision=self.precision,
+                                  unpruned=self.isPruned == 0)
         k = nv
         prob = math.factorial(k) / float(k) ** k  # CountMat.cpp:248-249
         aut = automorphisms(nv, edges) if self.calculate_automorphisms else 1
@@ -324,28 +354,11 @@ class CountMat:
 
 
 def automorphisms(nv: int, edges) -> int:
-    """Edge-preserving vertex permutations of a small template (CountMat.cpp:1498-1572)."""
-    adj = np.zeros((nv, nv), bool)
-    for u, v in np.asarray(edges).reshape(-1, 2):
-        adj[u, v] = adj[v, u] = True
-    deg = adj.sum(1)
-    count = 0
-    perm = [-1] * nv
-    used = [False] * nv
-
-    def rec(i):
-        nonlocal count
-        if i == nv:
-            count += 1
-            return
-        for cnd in range(nv):
-            if used[cnd] or deg[cnd] != deg[i]:
-                continue
-            if all(adj[i, j] == adj[cnd, perm[j]] for j in range(i)):
-                used[cnd] = True
-                perm[i] = cnd
-                rec(i + 1)
-                used[cnd] = False
-
-    rec(0)
-    return count
+    """CountMat::automorphismNum (CountMat.cpp:1498-1572): permutations s with
+    deg(i) = deg(s(i)) and N(i) a subset of s(N(s(i))) for every vertex i, the
+    reference's acceptance rule (pg_template_automorphisms)."""
+    e = np.asarray(edges, dtype=np.int32).reshape(-1, 2)
+    r = lib().pg_template_automorphisms(nv, len(e), np.ascontiguousarray(e[:, 0]), np.ascontiguousarray(e[:, 1]))
+    if r < 0:
+        raise PGError(int(r), "pg_template_automorphisms")
+    return int(r)

@@ -18,6 +18,7 @@ PG_OK = 0
 PG_FP32 = 0
 PG_FP64 = 1
 PG_FLAG_FIX_2VERTEX = 1
+PG_FLAG_UNPRUNED = 2
 ERRORS = {-1: "invalid argument", -2: "out of memory", -3: "HIP error", -4: "bad state",
           -5: "I/O error", -6: "out of range"}
 
@@ -97,10 +98,13 @@ def lib() -> C.CDLL:
         "pg_last_timing_kernel": (C.c_int, [_vp, C.c_char_p, C.c_int]),
         "pg_last_timing_entry": (C.c_int, [_vp, C.c_int, C.c_char_p, C.c_int, C.POINTER(C.c_double),
                                            C.POINTER(C.c_int64), C.POINTER(C.c_double)]),
+        "pg_last_timing_sub": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int), C.c_char_p, C.c_int,
+                                         C.POINTER(C.c_double), C.POINTER(C.c_int64), C.POINTER(C.c_double)]),
         "pg_rmat_edges": (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_uint64,
                                     C.c_int64, _vp, _vp, C.POINTER(C.c_int64)]),
         "pg_csc_from_edges": (C.c_int, [C.c_int32, C.c_int64, _i32p, _i32p, _i32p, _vp]),
         "pg_edges_remove_holes": (C.c_int64, [C.c_int64, C.c_int64, _i32p, _i32p]),
+        "pg_template_automorphisms": (C.c_int64, [C.c_int, C.c_int, _i32p, _i32p]),
         "pg_template_sizes": (C.c_int, [C.c_int, C.c_int, _i32p, _i32p, C.POINTER(C.c_int32),
                                         C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
         "pg_template_tables": (C.c_int, [C.c_int, C.c_int, _i32p, _i32p] + [_i32p] * 6),
@@ -123,5 +127,6 @@ def exported_symbols() -> list[str]:
             "pg_graph_upload", "pg_graph_set_weights", "pg_graph_stats", "pg_plan_create",
             "pg_plan_from_edges", "pg_plan_destroy", "pg_plan_info", "pg_colors_fill", "pg_count",
             "pg_count_multi", "pg_count_grad", "pg_last_timing", "pg_last_timing_kernel",
-            "pg_last_timing_entry", "pg_rmat_edges", "pg_csc_from_edges", "pg_edges_remove_holes",
-            "pg_template_sizes", "pg_template_tables", "pg_estimate_model", "pg_plan_footprint"]
+            "pg_last_timing_entry", "pg_last_timing_sub", "pg_rmat_edges", "pg_csc_from_edges", "pg_edges_remove_holes",
+            "pg_template_sizes", "pg_template_tables", "pg_estimate_model", "pg_plan_footprint",
+            "pg_template_automorphisms"]

@@ -7,7 +7,7 @@
 #include <cmath>
 #include <cstdio>
 #include <fstream>
-#include <functional>
+#include <map>
 
 namespace pgbsc {
 
@@ -23,7 +23,16 @@ int CountMat::initialization(const CSCGraph* g, int thd_num, int itr_num, int is
         err_ = "initialization needs a graph and itr_num > 0";
         return PG_ERR_INVALID;
     }
-    (void)isPruned;
+    // isPruned = 0 is the reference's unpruned mode, with its own numerics
+    // (PG_FLAG_UNPRUNED).  useSPMM = 0 (countNonBottomePruned, CountMat.cpp:
+    // 325-554) forms the same FP32 sums in the same order as the SpMM path
+    // (spmvNaiveSplit adds 1.0f * x, CSCGraph.hpp:220-239), so it runs the
+    // same kernels.
+    if (isPruned == 0 && precision != PG_FP32) {
+        err_ = "isPruned = 0 has the reference's FP32 numerics only";
+        return PG_ERR_INVALID;
+    }
+    unpruned_ = isPruned == 0;
     (void)useSPMM;
     graph_ = g;
     itr_ = itr_num;
@@ -43,7 +52,9 @@ int CountMat::initialization(const CSCGraph* g, int thd_num, int itr_num, int is
     return rc;
 }
 
-static int factorial(int n) { return n <= 1 ? 1 : n * factorial(n - 1); }
+// CountMat.cpp:993-999 computes k! in int, which overflows for k >= 13
+// (13! > 2^31); the product here is exact in double for every k <= 14.
+static double factorial(int n) { return n <= 1 ? 1.0 : n * factorial(n - 1); }
 
 double CountMat::compute(const Graph& t, bool isEstimate)
 {
@@ -89,15 +100,28 @@ double CountMat::compute(const Graph& t, bool isEstimate)
     raw_.assign(itr_, 0.0);
     if (verbose_) pg_set_option(ctx_, "timing", 1);
     auto t0 = std::chrono::steady_clock::now();
-    rc = pg_count(ctx_, plan, itr_, nullptr, seed_, precision_, 0u, raw_.data());
+    rc = pg_count(ctx_, plan, itr_, nullptr, seed_, precision_, unpruned_ ? PG_FLAG_UNPRUNED : 0u, raw_.data());
     auto t1 = std::chrono::steady_clock::now();
     if (verbose_) pg_set_option(ctx_, "timing", 0);
-    pg_plan_destroy(plan);
     if (rc) {
         err_ = pg_last_error(ctx_);
+        pg_plan_destroy(plan);
         return rc;
     }
     sec_per_col_ = std::chrono::duration<double>(t1 - t0).count() / itr_;
+    // CountMat.cpp:306-311, once per subtemplate of every coloring: the
+    // memory in use (here the device bytes of the call's pass plan) and the
+    // value colorCounting holds after the step, which is 0 until the root
+    // (subtemplate 0) returns the raw count
+    {
+        int64_t dev = 0, tile = 1;
+        pg_get_option(ctx_, "last_tile", &tile);
+        pg_plan_footprint(ctx_, plan, (int)std::max<int64_t>(tile, 1), precision_, &dev, nullptr);
+        for (int i = 0; i < itr_; ++i)
+            for (int s = fp_.nsub - 1; s >= 0; --s)
+                printf("Mem utilization compute step sub %d: %9.6lf GB, count val: %e\n", s, dev / 1073741824.0,
+                       s == 0 ? raw_[i] : 0.0);
+    }
     printf("\nTime for count per iter: %9.6lf seconds\n", sec_per_col_);
     if (verbose_) {
         // CountMat.cpp:216-242.  SpMM and eMA run fused in one kernel, so the
@@ -114,6 +138,37 @@ double CountMat::compute(const Graph& t, bool isEstimate)
             if (pg_last_timing_entry(ctx_, i, name, (int)sizeof name, &ms, &nl, &kb) >= 0)
                 printf("Kernel %s : %f ms in %lld launches, %f GB/s algorithmic\n", name, ms, (long long)nl,
                        ms > 0 ? kb / (ms * 1e-3) / 1e9 : 0.0);
+        // CountMat.cpp:833-840 per subtemplate, averaged over the colorings:
+        // its device time, then each kernel of it.  SpMM and eMA run fused,
+        // so the split the reference prints (Spmv / Mul) is by launch: the
+        // neighbour-summing launches against the rest.
+        const int ne = pg_last_timing_sub(ctx_, 0, nullptr, nullptr, 0, nullptr, nullptr, nullptr);
+        struct Row { std::string name; double ms, bytes; int64_t launches; };
+        std::map<int, std::vector<Row>> by_sub;
+        int sub = 0;
+        for (int i = 0; i < ne; ++i)
+            if (pg_last_timing_sub(ctx_, i, &sub, name, (int)sizeof name, &ms, &nl, &kb) >= 0)
+                by_sub[sub].push_back({name, ms, kb, nl});
+        for (auto it = by_sub.rbegin(); it != by_sub.rend(); ++it) {
+            double tot = 0, spmv = 0;
+            for (const Row& r : it->second) {
+                tot += r.ms;
+                // aux modes of k_sub<T,team,aux,main>: 2 reads the histogram, 3 sums hub partials
+                const size_t c1 = r.name.find(',');
+                const size_t c2 = c1 == std::string::npos ? c1 : r.name.find(',', c1 + 1);
+                const int aux = c2 == std::string::npos ? -1 : atoi(r.name.c_str() + c2 + 1);
+                if (r.name.rfind("k_sub<", 0) == 0 && aux != 2 && aux != 3) spmv += r.ms;
+            }
+            const double sec = tot * 1e-3 / itr_, ssec = spmv * 1e-3 / itr_;
+            if (it->first < 0)
+                printf("Pass-level launches (colors, neighbour-color histogram, root sums), counting time %f\n", sec);
+            else
+                printf("Sub %d, counting time %f, Spmv time %f: ratio: %f%%,  Mul time %f: ratio: %f%% \n", it->first,
+                       sec, ssec, sec > 0 ? 100 * ssec / sec : 0.0, sec - ssec, sec > 0 ? 100 * (sec - ssec) / sec : 0.0);
+            for (const Row& r : it->second)
+                printf("    %s : %f ms per iter in %lld launches, %f GB/s algorithmic\n", r.name.c_str(), r.ms / itr_,
+                       (long long)(r.launches / itr_), r.ms > 0 ? r.bytes / (r.ms * 1e-3) / 1e9 : 0.0);
+        }
         if (sec > 0) {
             printf("Total Memory bandwidth is : %f GBytes per second\n",
                    ((est.pgbsc_spmv_gbytes + est.pgbsc_fma_gbytes) * itr_) / sec);
@@ -133,38 +188,18 @@ double CountMat::compute(const Graph& t, bool isEstimate)
     const double fin = std::floor(mean / (prob * (double)aut) + 0.5);
     printf("Final count is %e\n", fin);
     fflush(stdout);
+    pg_plan_destroy(plan);
     return fin;
 }
 
-// Permutations of the template's vertices that map edges to edges, found by
-// backtracking with a degree filter (the reference enumerates all k!
-// permutations and then checks them, CountMat.cpp:1498-1572).
+// CountMat.cpp:1498-1572, the reference's acceptance rule
+// (pg_template_automorphisms)
 int CountMat::automorphismNum() const
 {
     if (!tmpl_) return 0;
-    const Graph& g = *tmpl_;
-    const int n = g.get_vert_num();
-    std::vector<std::vector<char>> adj(n, std::vector<char>(n, 0));
-    for (int v = 0; v < n; ++v)
-        for (int j = 0; j < g.get_out_deg(v); ++j) adj[v][g.get_adj_list(v)[j]] = 1;
-    std::vector<int> map(n, -1);
-    std::vector<char> used(n, 0);
-    int count = 0;
-    std::function<void(int)> rec = [&](int i) {
-        if (i == n) { ++count; return; }
-        for (int c = 0; c < n; ++c) {
-            if (used[c] || g.get_out_deg(c) != g.get_out_deg(i)) continue;
-            bool ok = true;
-            for (int j = 0; j < i && ok; ++j) ok = adj[i][j] == adj[c][map[j]];
-            if (!ok) continue;
-            used[c] = 1;
-            map[i] = c;
-            rec(i + 1);
-            used[c] = 0;
-        }
-    };
-    rec(0);
-    return count;
+    const int64_t n = pg_template_automorphisms(tmpl_->get_vert_num(), (int)tmpl_->src().size(),
+                                                tmpl_->src().data(), tmpl_->dst().data());
+    return n < 0 ? 0 : (int)n;
 }
 
 bool CountMat::model(pg_estimate* e)

@@ -15,9 +15,11 @@ namespace pgbsc {
 class CountMat {
 public:
     ~CountMat();
-    // CountMat.cpp:21-77.  thd_num sizes the host-side OpenMP work only;
-    // isPruned / useSPMM select reference modes that all map onto the same
-    // kernels (identical results).  Returns 0 or a PG_ERR_* code.
+    // CountMat.cpp:21-77.  thd_num sizes the host-side OpenMP work only.
+    // isPruned = 0 selects the reference's unpruned mode with its numerics
+    // (PG_FLAG_UNPRUNED, FP32 only); useSPMM = 0 forms the same sums in the
+    // same order as the SpMM path and runs its kernels.  Returns 0 or a
+    // PG_ERR_* code.
     int initialization(const CSCGraph* graphCSC, int thd_num, int itr_num, int isPruned, int useSPMM,
                        int vtuneStart = -1, bool calculate_automorphisms = false, int device = 0,
                        int precision = PG_FP32, uint64_t seed = 0);
@@ -67,6 +69,7 @@ private:
     bool verbose_ = false;
     int itr_ = 1, thd_ = 1, precision_ = PG_FP32;
     bool autom_ = false;
+    bool unpruned_ = false;
     uint64_t seed_ = 0;
     std::vector<double> raw_;
     double sec_per_col_ = 0;

@@ -58,6 +58,7 @@ struct DevBuf {
 struct LaunchRec {
     std::string name;
     double bytes;
+    int sub;  // subtemplate index, -1 for pass-level launches (colors, histogram, reductions)
     hipEvent_t beg, end;
 };
 
@@ -127,6 +128,13 @@ struct pg_plan {
     std::vector<int> sizes, mains, auxs;
     std::vector<std::unique_ptr<DevSub>> subs;  // processing order (descending s)
     DevBuf rank_tab;                            // [2^K] int16
+    // the unpruned mode's tables over absolute color sets, per subtemplate
+    // index (non-leaf only): combToCount [C(k,size)] and the split pairs
+    // [C(k,size)][C(size,main)] as (split_main << 16 | split_aux), uploaded
+    // on first use
+    std::vector<std::vector<int32_t>> h_c2c;
+    std::vector<std::vector<uint32_t>> h_fsplit;
+    std::vector<std::unique_ptr<DevBuf>> d_c2c, d_fsplit;
 };
 
 struct pg_ctx {
@@ -175,9 +183,16 @@ struct pg_ctx {
     std::vector<hipEvent_t> ev_join;
     hipEvent_t ev_fork = nullptr;
     std::map<std::tuple<const pg_plan*, int, int>, std::vector<int64_t>> tuned;  // (plan, B, tsize) -> settings, tile
+    // the unpruned mode (PG_FLAG_UNPRUNED): the reference's _bufVec, zeroed
+    // at graph upload and then only ever added to, and its tables
+    DevBuf d_fascia_acc, d_fascia;
+    bool fascia_acc_ready = false;
     // timing of the last call
     std::vector<LaunchRec>* cur_recs = nullptr;
     std::map<std::string, KernelTotals> agg;
+    // the same launches keyed by (subtemplate, kernel); cur_sub tags new records
+    std::map<std::pair<int, std::string>, KernelTotals> agg_sub;
+    int cur_sub = -1;
     double last_total_ms = 0;
     std::string dom_name;
     double dom_ms = 0, dom_bytes = 0;
@@ -437,7 +452,7 @@ void record(pg_ctx* c, const std::string& name, double bytes, bool begin)
 {
     if (!c->timing || !c->cur_recs) return;
     if (begin) {
-        LaunchRec r{name, bytes, nullptr, nullptr};
+        LaunchRec r{name, bytes, c->cur_sub, nullptr, nullptr};
         (void)hipEventCreate(&r.beg);
         (void)hipEventCreate(&r.end);
         (void)hipEventRecord(r.beg, c->stream);
@@ -453,10 +468,11 @@ void accumulate_timing(pg_ctx* c, const std::vector<LaunchRec>& recs)
     for (auto& r : recs) {
         float ms = 0;
         if (hipEventElapsedTime(&ms, r.beg, r.end) != hipSuccess) continue;
-        auto& a = c->agg[r.name];
-        a.ms += ms;
-        a.bytes += r.bytes;
-        a.launches += 1;
+        for (KernelTotals* a : {&c->agg[r.name], &c->agg_sub[{r.sub, r.name}]}) {
+            a->ms += ms;
+            a->bytes += r.bytes;
+            a->launches += 1;
+        }
     }
 }
 
@@ -604,10 +620,18 @@ double csc_bytes(const pg_ctx* c) { return 4.0 * (c->n_act + 1) + 4.0 * c->nnz *
 // One forward step: subtemplate d of B colorings from its children's tables
 // (nullptr for a leaf child) into out_tab or, at the root, into the
 // per-vertex root values and the pass results.
+// Tags the timing records of one subtemplate's launches with its index.
+struct SubTag {
+    pg_ctx* c;
+    SubTag(pg_ctx* ctx, int s) : c(ctx) { c->cur_sub = s; }
+    ~SubTag() { c->cur_sub = -1; }
+};
+
 template <typename T>
 int enqueue_step(pg_ctx* c, const pg_plan* p, const DevSub& d, int B, const PassBufs<T>& pb, const T* main_tab,
                  const T* aux_tab, T* out_tab)
 {
+    const SubTag tag(c, d.s);
     const hipStream_t st = c->stream;
     const double na = (double)c->n_act;
     SubArgs<T> a = pass_args<T>(c, p, B, pb);
@@ -1023,6 +1047,7 @@ int count_impl(pg_ctx* c, const pg_plan* p, int ncol, const uint8_t* colors, uin
         d_in = c->d_colors_in.as<uint8_t>();
     }
     c->agg.clear();
+    c->agg_sub.clear();
     hipEvent_t t0 = nullptr, t1 = nullptr;
     HIP_TRY(c, hipEventCreate(&t0));
     HIP_TRY(c, hipEventCreate(&t1));
@@ -1172,6 +1197,7 @@ int count_multi_impl(pg_ctx* c, int np, const pg_plan* const* plans, const int32
     std::vector<int64_t> first(np, 0), out_off(np, 0);
     for (int i = 1; i < np; ++i) out_off[i] = out_off[i - 1] + ncol[i - 1];
     c->agg.clear();
+    c->agg_sub.clear();
     hipEvent_t t0 = nullptr, t1 = nullptr;
     HIP_TRY(c, hipEventCreate(&t0));
     HIP_TRY(c, hipEventCreate(&t1));
@@ -1358,6 +1384,7 @@ template <typename T>
 int enqueue_sub_backward(pg_ctx* c, const pg_plan* p, const DevSub& d, int B, const PassBufs<T>& pb, T* Ybuf,
                          T* dYbuf, const double* cot, const T* Tm, const T* Ta, const T* G, T* Gm, T* Ga)
 {
+    const SubTag tag(c, d.s);
     const hipStream_t st = c->stream;
     const bool fp32 = sizeof(T) == 4;
     const SubArgs<T> base_args = pass_args<T>(c, p, B, pb);
@@ -1725,6 +1752,7 @@ int count_grad_impl(pg_ctx* c, const pg_plan* p, int ncol, const uint8_t* colors
         d_in = c->d_colors_in.as<uint8_t>();
     }
     c->agg.clear();
+    c->agg_sub.clear();
     hipEvent_t t0 = nullptr, t1 = nullptr;
     HIP_TRY(c, hipEventCreate(&t0));
     HIP_TRY(c, hipEventCreate(&t1));
@@ -2149,6 +2177,7 @@ extern "C" int pg_graph_upload(pg_ctx* c, int32_t n, int64_t nnz, const int32_t*
         }
     }
     c->has_graph = false;
+    c->fascia_acc_ready = false;
     c->n = n;
     c->nnz = nnz;
     c->n_act = na;
@@ -2319,6 +2348,24 @@ extern "C" int pg_plan_create(pg_ctx* c, int k, int nsub, const int32_t* sub_siz
     plan->sizes.assign(sub_size, sub_size + nsub);
     plan->mains.assign(main_idx, main_idx + nsub);
     plan->auxs.assign(aux_idx, aux_idx + nsub);
+    plan->h_c2c.resize(nsub);
+    plan->h_fsplit.resize(nsub);
+    plan->d_c2c.resize(nsub);
+    plan->d_fsplit.resize(nsub);
+    {
+        int64_t co = 0, so = 0;
+        for (int s = 0; s < nsub; ++s) {
+            const int nc = IndexSys::comb_calc(k, sub_size[s]);
+            if (sub_size[s] > 1) {
+                const int P = IndexSys::comb_calc(sub_size[s], sub_size[main_idx[s]]);
+                plan->h_c2c[s].assign(comb_to_count + co, comb_to_count + co + nc);
+                for (int64_t q = 0; q < (int64_t)nc * P; ++q)
+                    plan->h_fsplit[s].push_back(((uint32_t)split_main[so + q] << 16) | (uint32_t)split_aux[so + q]);
+                so += (int64_t)nc * P;
+            }
+            co += nc;
+        }
+    }
     const int K = k - 1;
     const std::vector<int16_t> rank = colex_rank_table(K);
     HIP_TRY(c, plan->rank_tab.alloc(rank.size() * 2));
@@ -2523,6 +2570,127 @@ extern "C" int pg_plan_footprint(pg_ctx* c, const pg_plan* p, int B, int precisi
     return PG_OK;
 }
 
+// ------------------------------------------------------- unpruned mode
+//
+// PG_FLAG_UNPRUNED: the reference's isPruned = 0 path (countNonBottomeOriginal,
+// CountMat.cpp:851-977) with its numerics, colorings one after another (each
+// continues the accumulator the previous one left, as the reference's
+// _bufVec does).  Absolute-set column-major FP32 tables of every non-leaf
+// subtemplate live for the whole coloring; launches run eagerly.
+int count_unpruned(pg_ctx* c, pg_plan* p, int ncol, const uint8_t* colors, uint64_t seed, double* raw_out)
+{
+    const hipStream_t st = c->stream;
+    const size_t na = (size_t)c->n_act;
+    if (!c->fascia_acc_ready) {
+        HIP_TRY(c, c->d_fascia_acc.alloc(na * sizeof(float)));
+        HIP_TRY(c, hipMemsetAsync(c->d_fascia_acc.p, 0, na * sizeof(float), st));
+        c->fascia_acc_ready = true;
+    }
+    for (auto& d : p->subs) {
+        const int s = d->s;
+        if (p->d_c2c[s]) continue;
+        auto c2c = std::make_unique<DevBuf>(), sp = std::make_unique<DevBuf>();
+        HIP_TRY(c, c2c->alloc(p->h_c2c[s].size() * 4));
+        HIP_TRY(c, sp->alloc(p->h_fsplit[s].size() * 4));
+        HIP_TRY(c, hipMemcpy(c2c->p, p->h_c2c[s].data(), p->h_c2c[s].size() * 4, hipMemcpyHostToDevice));
+        HIP_TRY(c, hipMemcpy(sp->p, p->h_fsplit[s].data(), p->h_fsplit[s].size() * 4, hipMemcpyHostToDevice));
+        p->d_c2c[s] = std::move(c2c);
+        p->d_fsplit[s] = std::move(sp);
+    }
+    // layout: tables by subtemplate index, root buffer, caller-order root
+    // values, colors, parameters, the FP64 result
+    std::vector<size_t> off_tab(p->nsub, 0);
+    size_t total = 0;
+    for (auto& d : p->subs) {
+        off_tab[d->s] = total;
+        total += align_up((d->root ? 1 : (size_t)d->Cs) * na * sizeof(float), 256);
+    }
+    const size_t off_rootc = total;
+    total += align_up((size_t)c->n * sizeof(float), 256);
+    const size_t off_colors = total;
+    total += align_up(na, 256);
+    const size_t off_params = total;
+    total += align_up(sizeof(PassParams), 256);
+    const size_t off_out = total;
+    total += 256;
+    HIP_TRY(c, c->d_fascia.alloc(total));
+    char* base = c->d_fascia.as<char>();
+    const uint8_t* d_in = nullptr;
+    if (colors) {
+        if ((size_t)ncol * c->n > c->d_colors_in.bytes) {
+            drop_graphs(c);  // captured passes hold the former colors buffer
+            HIP_TRY(c, c->d_colors_in.alloc((size_t)ncol * c->n));
+        }
+        HIP_TRY(c, hipMemcpyAsync(c->d_colors_in.p, colors, (size_t)ncol * c->n, hipMemcpyHostToDevice, st));
+        d_in = c->d_colors_in.as<uint8_t>();
+    }
+    uint8_t* d_colors = (uint8_t*)(base + off_colors);
+    const PassParams* pp = (const PassParams*)(base + off_params);
+    c->agg.clear();
+    c->agg_sub.clear();
+    std::vector<LaunchRec> recs;
+    c->cur_recs = &recs;
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    HIP_TRY(c, hipEventCreate(&t0));
+    HIP_TRY(c, hipEventCreate(&t1));
+    HIP_TRY(c, hipEventRecord(t0, st));
+    int rc = PG_OK;
+    for (int i = 0; i < ncol && rc == PG_OK; ++i) {
+        const PassParams hp{seed, i};
+        HIP_TRY(c, hipMemcpyAsync(base + off_params, &hp, sizeof hp, hipMemcpyHostToDevice, st));
+        record(c, "k_colors", 0, true);
+        HIP_TRY(c, d_in ? launch_colors_gather(d_colors, d_in, c->d_perm.as<int32_t>(), c->n_act, 1, c->n, pp, st)
+                        : launch_colors_rng(d_colors, c->d_perm.as<int32_t>(), c->n_act, 1, pp, p->k, st));
+        record(c, "", 0, false);
+        for (auto& dp : p->subs) {
+            const DevSub& d = *dp;
+            const SubTag tag(c, d.s);
+            FasciaArgs a{};
+            a.n_act = c->n_act;
+            a.rowptr = c->d_rowptr.as<int32_t>();
+            a.col = c->d_col.as<int32_t>();
+            a.colors = d_colors;
+            a.acc = c->d_fascia_acc.as<float>();
+            a.Cs = d.Cs;
+            a.P = IndexSys::comb_calc(d.size, d.mm);
+            a.c2c = p->d_c2c[d.s]->as<int32_t>();
+            a.split = p->d_fsplit[d.s]->as<uint32_t>();
+            a.main_leaf = d.mm == 1;
+            a.aux_leaf = d.aa == 1;
+            a.tm = d.mm > 1 ? (const float*)(base + off_tab[d.mi]) : nullptr;
+            a.ta = d.aa > 1 ? (const float*)(base + off_tab[d.ai]) : nullptr;
+            a.out = (float*)(base + off_tab[d.s]);
+            // the reference's Fascia byte model per pair: the aux column, the
+            // main column and the output (estimateMemCommFascia)
+            record(c, "k_fascia_step", 4.0 * (double)a.Cs * a.P * (double)(c->nnz + 3 * na), true);
+            HIP_TRY(c, launch_fascia_step(a, c->num_cus, st));
+            record(c, "", 0, false);
+            if (d.root) {
+                record(c, "k_fascia_sum", 4.0 * (double)(na + c->n), true);
+                HIP_TRY(c, launch_fascia_root_sum(a.out, c->d_perm.as<int32_t>(), c->n_act,
+                                                  (float*)(base + off_rootc), c->n, (double*)(base + off_out), st));
+                record(c, "", 0, false);
+            }
+        }
+        // CountMat.cpp:952-955: the FP64 sum is the raw count (no 1e12 here)
+        HIP_TRY(c, hipMemcpyAsync(&raw_out[i], base + off_out, sizeof(double), hipMemcpyDeviceToHost, st));
+        HIP_TRY(c, hipStreamSynchronize(st));
+    }
+    c->cur_recs = nullptr;
+    (void)hipEventRecord(t1, st);
+    (void)hipEventSynchronize(t1);
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, t0, t1);
+    c->last_total_ms = ms;
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    accumulate_timing(c, recs);
+    destroy_recs(recs);
+    finish_timing(c);
+    c->last_tile = 1;
+    return rc;
+}
+
 extern "C" int pg_count(pg_ctx* c, const pg_plan* p, int ncol, const uint8_t* colors, uint64_t seed, int precision,
                         uint32_t flags, double* raw_out)
 {
@@ -2541,6 +2709,11 @@ extern "C" int pg_count(pg_ctx* c, const pg_plan* p, int ncol, const uint8_t* co
         for (int i = 0; i < ncol; ++i) raw_out[i] = 0.0;
         c->last_total_ms = 0;
         return PG_OK;
+    }
+    if (flags & PG_FLAG_UNPRUNED) {
+        if (precision != PG_FP32)
+            return set_err(c, PG_ERR_INVALID, "the unpruned mode has the reference's FP32 numerics only");
+        return count_unpruned(c, const_cast<pg_plan*>(p), ncol, colors, seed, raw_out);
     }
     return precision == PG_FP32 ? count_impl<float>(c, p, ncol, colors, seed, flags, raw_out)
                                 : count_impl<double>(c, p, ncol, colors, seed, flags, raw_out);
@@ -2623,6 +2796,21 @@ extern "C" int pg_last_timing_kernel(const pg_ctx* c, char* buf, int len)
     if (!c || !buf || len <= 0) return PG_ERR_INVALID;
     snprintf(buf, (size_t)len, "%s", c->dom_name.c_str());
     return PG_OK;
+}
+
+extern "C" int pg_last_timing_sub(const pg_ctx* c, int idx, int* sub, char* name, int len, double* ms,
+                                  int64_t* launches, double* bytes)
+{
+    if (!c || idx < 0 || (name && len <= 0)) return PG_ERR_INVALID;
+    if ((size_t)idx >= c->agg_sub.size()) return PG_ERR_RANGE;
+    auto it = c->agg_sub.begin();
+    std::advance(it, idx);
+    if (sub) *sub = it->first.first;
+    if (name) snprintf(name, (size_t)len, "%s", it->first.second.c_str());
+    if (ms) *ms = it->second.ms;
+    if (launches) *launches = it->second.launches;
+    if (bytes) *bytes = it->second.bytes;
+    return (int)c->agg_sub.size();
 }
 
 extern "C" int pg_last_timing_entry(const pg_ctx* c, int idx, char* name, int len, double* ms,

@@ -212,4 +212,29 @@ hipError_t launch_scatter_grad_f64(double* out, const double* grad, const int32_
                                    hipStream_t st);
 hipError_t launch_zero_f64(double* p, int64_t n, hipStream_t st);
 
+// ---- the reference's unpruned mode (isPruned = 0, CountMat.cpp:851-977) ----
+// One non-leaf subtemplate for one coloring, thread per vertex, exactly the
+// reference's FP32 operation sequence (pg_fascia.hip).  Tables are
+// column-major [C(k,|x|)][n_act] over absolute color sets; leaf children
+// read the colors.
+struct FasciaArgs {
+    int32_t n_act;
+    const int32_t* rowptr;   // internal CSR, neighbours in ascending caller id
+    const int32_t* col;
+    const uint8_t* colors;   // [n_act]
+    float* acc;              // [n_act] the reference's _bufVec, carried across calls
+    int32_t Cs, P;
+    const int32_t* c2c;      // [Cs] output column of the i-th set (IndexSys combToCount)
+    const uint32_t* split;   // [Cs][P] (split_main << 16 | split_aux)
+    int32_t main_leaf, aux_leaf;
+    const float* tm;         // [Cm][n_act] (main not a leaf)
+    const float* ta;         // [Ca][n_act] (aux not a leaf)
+    float* out;              // [Cs][n_act]; at the root [n_act] (bufLastSub)
+};
+hipError_t launch_fascia_step(const FasciaArgs& a, int num_cus, hipStream_t st);
+// *out = sum over caller vertices 0..n-1, in order, of (double) the root
+// buffer (internal order, scattered through perm into scratch[n] first)
+hipError_t launch_fascia_root_sum(const float* vals, const int32_t* perm, int32_t n_act, float* scratch, int64_t n,
+                                  double* out, hipStream_t st);
+
 }  // namespace pgbsc

@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <fstream>
+#include <functional>
 #include <numeric>
 
 namespace pgbsc {
@@ -310,4 +311,62 @@ extern "C" int pg_template_tables(int nv, int ne, const int32_t* src, const int3
     std::copy(fp.split_main.begin(), fp.split_main.end(), split_main);
     std::copy(fp.split_aux.begin(), fp.split_aux.end(), split_aux);
     return PG_OK;
+}
+
+// CountMat::automorphismNum (CountMat.cpp:1498-1572).  The reference tries
+// all k! permutations s (calcAutomorphismRecursive builds mappingID, s(i) =
+// mappingID[i]) and accepts s when, for every vertex i, deg(i) = deg(s(i))
+// and every neighbour u of i equals s(w) for some neighbour w of s(i)
+// (calcAutomorphismZero, :1510-1544): N(i) is a subset of s(N(s(i))).  That
+// is not always the automorphism group (on a 3-leg spider it accepts some
+// non-automorphisms and rejects others), so the same predicate is counted.
+// Backtracking assigns s(0), s(1), ... and tests vertex i as soon as s is
+// known on i and on N(s(i)); a branch is cut only when a fully determined
+// test fails, so the count is the reference's over all k! permutations.
+extern "C" int64_t pg_template_automorphisms(int nv, int ne, const int32_t* src, const int32_t* dst)
+{
+    if (nv < 1 || nv > pgbsc::kMaxTemplate || ne < 0 || (ne > 0 && (!src || !dst))) return PG_ERR_INVALID;
+    for (int j = 0; j < ne; ++j)
+        if (src[j] < 0 || src[j] >= nv || dst[j] < 0 || dst[j] >= nv) return PG_ERR_INVALID;
+    pgbsc::Graph g;
+    std::vector<int> sv(src, src + ne), dv(dst, dst + ne);
+    g.build_graph(nv, ne, sv.data(), dv.data());
+    const int n = nv;
+    std::vector<int> map(n, -1), reach(n, 0);
+    std::vector<char> used(n, 0), done(n, 0);
+    // reach[c]: the largest neighbour of c, so s is known on N(c) once s(0..reach[c]) are
+    for (int c = 0; c < n; ++c)
+        for (int j = 0; j < g.get_out_deg(c); ++j) reach[c] = std::max(reach[c], g.get_adj_list(c)[j]);
+    auto holds = [&](int i) {
+        const int c = map[i];
+        for (int j = 0; j < g.get_out_deg(i); ++j) {
+            const int u = g.get_adj_list(i)[j];
+            bool hit = false;
+            for (int q = 0; q < g.get_out_deg(c) && !hit; ++q) hit = map[g.get_adj_list(c)[q]] == u;
+            if (!hit) return false;
+        }
+        return true;
+    };
+    int64_t count = 0;
+    std::function<void(int)> rec = [&](int d) {
+        if (d == n) { ++count; return; }
+        for (int c = 0; c < n; ++c) {
+            if (used[c] || g.get_out_deg(c) != g.get_out_deg(d)) continue;
+            used[c] = 1;
+            map[d] = c;
+            std::vector<int> now;  // tests made at this depth
+            bool ok = true;
+            for (int i = 0; i <= d && ok; ++i)
+                if (!done[i] && reach[map[i]] <= d) {
+                    ok = holds(i);
+                    if (ok) { done[i] = 1; now.push_back(i); }
+                }
+            if (ok) rec(d + 1);
+            for (int i : now) done[i] = 0;
+            map[d] = -1;
+            used[c] = 0;
+        }
+    };
+    rec(0);
+    return count;
 }

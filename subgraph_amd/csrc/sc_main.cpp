@@ -15,10 +15,15 @@
 //   PG_VERBOSE=1   print the cost-model estimates before counting and the
 //                  per-kernel device times and model bandwidth after it
 //   PG_ESTIMATE=1  stop after the estimates (compute(t, isEstimate = true))
+//   PG_AUTOMORPHISMS=1  divide the final count by the template's automorphism
+//                  count (the reference's hard-coded calculate_automorphism,
+//                  sc-main.cpp:1568, which it leaves false)
 // and one switch has no reference counterpart:
 //   PG_AUTOTUNE=1  let the library's autotuner pick kernel variants, degree
 //                  buckets and colorings per pass (one-time, before the first
 //                  pass of the template)
+// isPruned = 0 runs the reference's unpruned mode with its numerics (FP32
+// only, slow by design); useSPMM = 0 forms the same sums as useSPMM = 1.
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -120,7 +125,10 @@ int main(int argc, char** argv)
         return 1;
     }
     CountMat executor;
-    rc = executor.initialization(&g, comp_thds, iterations, isPruned, useSPMM, vtuneStart, false, 0, precision, seed);
+    const char* autom = getenv("PG_AUTOMORPHISMS");
+    const bool calculate_automorphism = autom && atoi(autom) != 0;
+    rc = executor.initialization(&g, comp_thds, iterations, isPruned, useSPMM, vtuneStart, calculate_automorphism, 0,
+                                 precision, seed);
     if (rc) {
         fprintf(stderr, "error: %s\n", executor.error().c_str());
         return 1;

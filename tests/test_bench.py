@@ -15,7 +15,10 @@ sys.path.insert(0, ROOT)
 import bench  # noqa: E402
 
 LINE = os.path.join(ROOT, "profiles", "bench_default.json")
-TRACE_MD = os.path.join(ROOT, "profiles", "k_sub_plain_rmat_s20_ef8_tree7.md")
+
+
+def trace_md(workload):
+    return os.path.join(ROOT, "profiles", f"k_sub_{workload}.md")
 
 
 def test_more_than_one_gpu_is_refused():
@@ -51,14 +54,15 @@ def test_committed_line_keeps_the_contract(line):
         assert k in line, k
     assert line["n_gpus"] == 1 and line["higher_is_better"] is True and line["vs_baseline"] is None
     assert line["unit"] == "colorings/s" and line["dtype"] == "fp32" and line["config"]["workload"]
-    assert line["value"] == pytest.approx(1e3 / line["ms_per_step"], rel=1e-9)
+    per_step = line["config"].get("colorings_per_step", 1)
+    assert line["value"] == pytest.approx(per_step * 1e3 / line["ms_per_step"], rel=1e-9)
     r = line["roofline"]
     assert r["bound"] == "hbm" and r["unit"] == "GB/s" and r["peak"] == 8000.0
     assert r["frac"] == pytest.approx(r["achieved"] / r["peak"], rel=1e-12)
     assert r["achieved"] == pytest.approx(r["algorithmic_bytes_per_launch"] / (r["avg_launch_ms"] * 1e-3) / 1e9,
                                           rel=1e-9)
     # traffic comes from the committed calibrated counter runs of that kernel
-    assert r["traffic"] == bench.measured_traffic(r["kernel"]) and r["traffic"] > 0
+    assert r["traffic"] == bench.measured_traffic(r["kernel"], line["config"]["workload"]) and r["traffic"] > 0
     cb = line["cpu_baseline"]
     assert cb["kind"] in ("reference", "port") and cb["cores"] >= 1 and cb["value"] > 0 and cb["sample"]
     assert cb["parity_max_rel_err"] <= 1e-5
@@ -69,7 +73,7 @@ def test_trace_summary_agrees_with_the_line(line):
     the line's live HIP-event figure name the same dominant kernel, and its
     average duration agrees within 5%."""
     kern = line["roofline"]["kernel"]
-    text = open(TRACE_MD).read()
+    text = open(trace_md(line["config"]["workload"])).read()
     rows = [ln for ln in text.splitlines() if ln.startswith(f"| `{kern}` |")]
     assert len(rows) == 1, kern
     avg_us = float(rows[0].split("|")[3])
@@ -77,7 +81,7 @@ def test_trace_summary_agrees_with_the_line(line):
     # the dominant kernel of the trace table (first row by total time) is the line's
     first = next(ln for ln in text.splitlines() if re.match(r"\| `k_", ln))
     assert first.startswith(f"| `{kern}` |")
-    # settings the summary names are the line's plain settings
+    # settings the summary names are the line's settings
     s = line["config"]["settings"]
     for key in ("stage_rows", "bucket_high", "bucket_mid", "hub_degree", "hub_chunk"):
         assert f"{key}={s[key]}" in text, key

@@ -530,3 +530,40 @@ def test_bucket_shapes_match_oracle(gpu_ctx, hub_degree, bucket_high, bucket_mid
     finally:
         for key, v in zip(keys, saved):
             gpu_ctx.set_option(key, v)
+
+
+@pytest.mark.parametrize("aux_rows,stage_rows", [(0, 0), (0, 1), (1, 0)])
+def test_64_colorings_per_pass_match_oracle(gpu_ctx, aux_rows, stage_rows):
+    """The batching configuration's 64 colorings in ONE pass (extra SpMM
+    right-hand-side columns) on an R-MAT scale-13 graph, every coloring
+    against the oracle: FP32 within the tolerance, FP64 exact below 2^53;
+    the pass width is asserted, not assumed."""
+    s, d = sg.rmat_edges(13, 8, seed=4)
+    n, s, d = sg.remove_holes(1 << 13, s, d)
+    colptr, rowidx = sg.csc_from_edges(n, s, d)
+    keys = ("aux_rows", "stage_rows", "batch_tile")
+    saved = [gpu_ctx.get_option(k) for k in keys]
+    gpu_ctx.upload_graph(colptr, rowidx)
+    gpu_ctx.set_option("aux_rows", aux_rows)
+    gpu_ctx.set_option("stage_rows", stage_rows)
+    gpu_ctx.set_option("batch_tile", 64)
+    try:
+        for k, te in ((5, [(0, 1), (1, 2), (2, 3), (3, 4)]), (7, TREE7)):
+            print(f"[tile64] aux_rows={aux_rows} stage_rows={stage_rows} k={k} n={n} nnz={len(rowidx)}", flush=True)
+            plan = gpu_ctx.plan(k, te)
+            op = oracle.Plan(k, te)
+            cols = sg.colors_fill(77, 0, 64, n, k)
+            g32 = gpu_ctx.count(plan, 64, colors=cols, precision="fp32")
+            assert gpu_ctx.get_option("last_tile") == 64
+            g64 = gpu_ctx.count(plan, 64, colors=cols, precision="fp64")
+            assert gpu_ctx.get_option("last_tile") == 64
+            worst = 0.0
+            for i in range(64):
+                o32 = oracle.count_f32(op, colptr, rowidx, cols[i])
+                worst = max(worst, _rel(g32[i], o32))
+                assert _rel(g32[i], o32) <= TOL32, (k, i, g32[i], o32)
+                assert _same64(g64[i], oracle.count_f64(op, colptr, rowidx, cols[i])), (k, i)
+            print(f"[tile64] k={k}: worst FP32 relative error {worst:.3e}", flush=True)
+    finally:
+        for key, v in zip(keys, saved):
+            gpu_ctx.set_option(key, v)
