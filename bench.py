@@ -184,7 +184,7 @@ def _timed(fn, reps, sync):
 TREE10 = [(0, 1), (1, 2), (2, 3), (3, 4), (1, 5), (5, 6), (5, 7), (7, 8), (7, 9)]
 TREE12 = [(0, 1), (1, 2), (2, 3), (3, 4), (4, 5), (2, 6), (6, 7), (7, 8), (6, 9), (9, 10), (10, 11)]
 U5 = [(0, 1), (1, 2), (2, 3), (3, 4)]
-TUNED_KEYS = ("reg_cols", "stage_rows", "aux_rows", "bucket_high", "bucket_mid", "hub_degree", "hub_chunk",
+TUNED_KEYS = ("reg_cols", "stage_rows", "split_lds", "aux_rows", "bucket_high", "bucket_mid", "hub_degree", "hub_chunk",
               "batch_tile")
 
 

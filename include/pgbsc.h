@@ -81,6 +81,8 @@ const char* pg_last_error(const pg_ctx* ctx);
  *   "graphs"       1 = each pass replays a captured hipGraph (default 1)
  *   "blocks_per_cu" workgroups per CU of persistent launches, 0 = occupancy query
  *   "stage_rows"   1 = gathers stage neighbour rows in LDS, 0 = direct loads (default 0)
+ *   "split_lds"    1 = the eMA combine reads the split table from LDS when it fits (default),
+ *                  0 = from memory
  *   "reg_cols"     values per lane of gathers: 0 = one for Y rows narrower than four values
  *                  (the root), else four (default); 1 or 4 = that for every gather
  *   "aux_rows"     neighbour sums row-wise with LDS atomics: 1 always, 0 never (default),

@@ -150,6 +150,7 @@ struct pg_ctx {
     int64_t blocks_per_cu = 0;  // 0: resident workgroups from the occupancy query
     int64_t aux_rows = 0;       // neighbour sums row-wise: 1 always, 0 never, -1 per subtemplate
     int64_t stage_rows = 0;     // gathers read neighbour rows staged in LDS (1) or directly (0)
+    int64_t split_lds = 1;      // combines read the split table from LDS (1) when it fits, or from memory (0)
     int64_t reg_cols = 0;       // values per lane of gathers: 0 one for rows under kRegCols, else 1 or 4
     int64_t adjoint_recompute = -1;  // adjoint tables: 0 kept, 1 recomputed, -1 recomputed when that fits 4x more
     int64_t vertex_order = 0;   // internal order at the next upload: 0 descending degree, 1 degree classes + BFS
@@ -394,11 +395,25 @@ int stage_pad(const pg_ctx* c, int aux, bool has_gidx, int B, int Wy, int Wa, in
     return pad;
 }
 
+// The combine may read the split table from LDS (option "split_lds"): a
+// main-table step's outputs read it Wo * P times per vertex and coloring.
+// Taken when the table fits kSplitLdsMax and the step's launches still fit
+// kLdsLimit with it.
+constexpr size_t kSplitLdsMax = 24 * 1024;
+constexpr size_t kLdsLimit = 64 * 1024;
+bool split_lds_candidate(const pg_ctx* c, const DevSub& d)
+{
+    return c->split_lds && d.mm > 1 && !d.root && (size_t)4 * d.P * d.Wo <= kSplitLdsMax;
+}
+
 template <typename T>
 size_t lds_of(int team, int aux, int main_mode, int ng, int out_mode, int B, const pg_plan* p, const DevSub& d,
-              int segpad = 0)
+              int segpad, int split_lds)
 {
     SubArgs<T> a{};
+    a.P = d.P;
+    a.Wo = d.Wo;
+    a.split_lds = split_lds;
     a.B = B;
     a.k = p->k;
     a.K = p->K;
@@ -416,34 +431,42 @@ size_t lds_of(int team, int aux, int main_mode, int ng, int out_mode, int B, con
 }
 
 // LDS of the largest launch of subtemplate d for B colorings, over every
-// launch shape launch_gathering (or the aux-leaf read) would use
+// launch shape launch_gathering (or the aux-leaf read) would use, with or
+// without the split table staged
 template <typename T>
-size_t sub_lds_max_t(const pg_ctx* c, const pg_plan* p, const DevSub& d, int B)
+size_t sub_lds_max_t(const pg_ctx* c, const pg_plan* p, const DevSub& d, int B, int split)
 {
     const int mm = d.mm > 1 ? MAIN_TABLE : MAIN_LEAF;
     const int out = d.root ? OUT_ROOT : OUT_EMA;
     const int fin_team = team_for(std::max(B * d.Wy, B * d.Wo));
-    size_t m = lds_of<T>(fin_team, d.aa == 1 ? AUX_READ : AUX_PARTIALS, mm, 1, out, B, p, d);
+    size_t m = lds_of<T>(fin_team, d.aa == 1 ? AUX_READ : AUX_PARTIALS, mm, 1, out, B, p, d, 0, split);
     if (d.aa == 1) return m;
     const int aux = aux_mode_for(c, p, d);
     const int rc = rc_for(c, aux, B, d.Wy);
     const int cl = cl_of(aux, B, p->k, d.Wy, d.Wa, rc);
     const int sp = rc == kRegCols ? stage_pad(c, aux, d.gidx.p != nullptr, B, d.Wy, d.Wa, cl, sizeof(T)) : 0;
     const int aux_l = sp > 0 ? AUX_STAGED : aux;
-    m = std::max(m, lds_of<T>(64, aux_l, mm, 64 / cl, OUT_PARTIAL, B, p, d, sp));
+    m = std::max(m, lds_of<T>(64, aux_l, mm, 64 / cl, OUT_PARTIAL, B, p, d, sp, split));
     for (int lanes : {64, 16, 8}) {
         const int team = std::max(lanes, cl);
-        m = std::max(m, lds_of<T>(team, aux_l, mm, team / cl, out, B, p, d, sp));
+        m = std::max(m, lds_of<T>(team, aux_l, mm, team / cl, out, B, p, d, sp, split));
     }
     return m;
 }
 
-size_t sub_lds_max(const pg_ctx* c, const pg_plan* p, const DevSub& d, int B, size_t tsz)
+// whether subtemplate d's combine reads the split table from LDS at B colorings
+int step_split_lds(const pg_ctx* c, const pg_plan* p, const DevSub& d, int B, size_t tsz)
 {
-    return tsz == 4 ? sub_lds_max_t<float>(c, p, d, B) : sub_lds_max_t<double>(c, p, d, B);
+    if (!split_lds_candidate(c, d)) return 0;
+    const size_t m = tsz == 4 ? sub_lds_max_t<float>(c, p, d, B, 1) : sub_lds_max_t<double>(c, p, d, B, 1);
+    return m <= kLdsLimit;
 }
 
-constexpr size_t kLdsLimit = 64 * 1024;
+size_t sub_lds_max(const pg_ctx* c, const pg_plan* p, const DevSub& d, int B, size_t tsz)
+{
+    const int split = step_split_lds(c, p, d, B, tsz);
+    return tsz == 4 ? sub_lds_max_t<float>(c, p, d, B, split) : sub_lds_max_t<double>(c, p, d, B, split);
+}
 
 // Timing: an event pair around each launch, appended to c->cur_recs.  Passes
 // run eagerly while timing is on: event-record nodes inside an instantiated
@@ -584,10 +607,13 @@ struct PassBufs {
     double* tmp;
 };
 
+// the device memory a pass's buffers are placed in
+char* arena_base(const pg_ctx* c) { return c->d_arena.as<char>(); }
+
 template <typename T>
 PassBufs<T> pass_bufs(pg_ctx* c, const MemPlan& mp)
 {
-    char* base = c->d_arena.as<char>();
+    char* base = arena_base(c);
     return {(const PassParams*)(base + mp.off_params), (uint8_t*)(base + mp.off_colors),
             (uint8_t*)(base + mp.off_ecol),          (T*)(base + mp.off_H),
             (T*)(base + mp.off_part),                (double*)(base + mp.off_root),
@@ -640,6 +666,7 @@ int enqueue_step(pg_ctx* c, const pg_plan* p, const DevSub& d, int B, const Pass
     a.Wo = d.Wo;
     a.P = d.P;
     a.split = d.split.as<uint32_t>();
+    a.split_lds = step_split_lds(c, p, d, B, sizeof(T));
     a.ymask = d.ymask.as<uint16_t>();
     const int main_mode = d.mm == 1 ? MAIN_LEAF : MAIN_TABLE;
     if (d.mm > 1) {
@@ -693,7 +720,7 @@ int enqueue_step(pg_ctx* c, const pg_plan* p, const DevSub& d, int B, const Pass
 template <typename T>
 int enqueue_pass(pg_ctx* c, const pg_plan* p, int B, const uint8_t* d_colors_in, const MemPlan& mp)
 {
-    char* base = c->d_arena.as<char>();
+    char* base = arena_base(c);
     const PassBufs<T> pb = pass_bufs<T>(c, mp);
     const hipStream_t st = c->stream;
     if (d_colors_in)
@@ -878,6 +905,7 @@ const std::vector<Knob>& tune_knobs()
     static const std::vector<Knob> knobs = {
         {"reg_cols", &pg_ctx::reg_cols, {0, 1}},
         {"stage_rows", &pg_ctx::stage_rows, {0, 1}},
+        {"split_lds", &pg_ctx::split_lds, {1, 0}},
         {"aux_rows", &pg_ctx::aux_rows, {0, -1}},
         {"bucket_high", &pg_ctx::bucket_high, {16, 32, 64, 128, 256, 512}},
         {"bucket_mid", &pg_ctx::bucket_mid, {4, 8, 16, 32, 64, 128}},
@@ -2037,6 +2065,12 @@ extern "C" int pg_set_option(pg_ctx* c, const char* key, int64_t v)
             drop_graphs(c);
             c->stage_rows = v;
         }
+    } else if (k == "split_lds") {
+        if (v != 0 && v != 1) return set_err(c, PG_ERR_INVALID, "split_lds must be 0 or 1");
+        if (v != c->split_lds) {
+            drop_graphs(c);
+            c->split_lds = v;
+        }
     } else if (k == "blocks_per_cu") {
         if (v < 0 || v > 64) return set_err(c, PG_ERR_INVALID, "blocks_per_cu must be in [0, 64]");
         if (v != c->blocks_per_cu) {
@@ -2064,6 +2098,7 @@ extern "C" int pg_get_option(const pg_ctx* c, const char* key, int64_t* v)
     else if (k == "blocks_per_cu") *v = c->blocks_per_cu;
     else if (k == "aux_rows") *v = c->aux_rows;
     else if (k == "stage_rows") *v = c->stage_rows;
+    else if (k == "split_lds") *v = c->split_lds;
     else if (k == "reg_cols") *v = c->reg_cols;
     else if (k == "adjoint_recompute") *v = c->adjoint_recompute;
     else if (k == "vertex_order") *v = c->vertex_order;

@@ -88,16 +88,31 @@ __host__ __device__ constexpr bool uses_rank(int aux) { return gathers_rows(aux)
 // launches that walk a vertex's entries
 __host__ __device__ constexpr bool walks_entries(int aux) { return uses_rank(aux) || aux == AUX_COLORS; }
 
-// LDS before the per-team rows: the gather index table (AUX_GATHER with
-// gidx), else the rank table, then (AUX_ROWS) the Wa row masks
+// Index tables before the per-team rows: the gather index table (AUX_GATHER
+// with gidx), else the rank table, then (AUX_ROWS) the Wa row masks
 template <typename T>
-__host__ __device__ size_t head_lds_bytes(int aux, const SubArgs<T>& a)
+__host__ __device__ size_t index_lds_bytes(int aux, const SubArgs<T>& a)
 {
     if ((aux == AUX_GATHER || aux == AUX_STAGED || aux == AUX_GATHER_T || aux == AUX_GATHER_G) && a.gidx)
         return ((size_t)2 * (a.k + 1) * a.k * a.Wy4 + 15) & ~(size_t)15;
     if (aux == AUX_ROWS && a.xidx) return ((size_t)2 * a.k * a.k * a.Wa4 + 15) & ~(size_t)15;
     return !uses_rank(aux) ? 0
                            : rank_lds_bytes(a.K) + (aux == AUX_ROWS ? (((size_t)2 * a.Wa + 15) & ~(size_t)15) : 0);
+}
+
+// the split table staged for the combine, [P][Wo] uint32, after the index tables
+template <typename T>
+__host__ __device__ size_t split_lds_bytes(const SubArgs<T>& a)
+{
+    const bool combine = a.out_mode == OUT_EMA || a.out_mode == OUT_EMA_SCALED;
+    return a.split_lds && combine ? (((size_t)4 * a.P * a.Wo + 15) & ~(size_t)15) : 0;
+}
+
+// LDS before the per-team rows
+template <typename T>
+__host__ __device__ size_t head_lds_bytes(int aux, const SubArgs<T>& a)
+{
+    return index_lds_bytes(aux, a) + split_lds_bytes(a);
 }
 
 }  // namespace
@@ -117,6 +132,63 @@ size_t sub_lds_bytes(int team, int aux, int main_mode, const SubArgs<T>& a)
 
 template size_t sub_lds_bytes<float>(int, int, int, const SubArgs<float>&);
 template size_t sub_lds_bytes<double>(int, int, int, const SubArgs<double>&);
+
+// One output value of the combine: the P splits in the reference's order j
+// (DataTableColMajor.cpp:402-500 add them one after another; :367 for the
+// scaled subtemplate, in FP64 with float(1e-12)).  The loads of 8 splits are
+// issued before their products are added, so the split table and LDS
+// latencies overlap; the additions keep their order.
+template <typename T, typename Split>
+__device__ __forceinline__ T combine_splits(Split split_at, int P, const T* y, const T* m, int mode, float scale)
+{
+    constexpr int U = 8;
+    if (mode == OUT_EMA_SCALED) {
+        T dst = T(0);
+        int j = 0;
+        for (; j + U <= P; j += U) {
+            uint32_t p[U];
+#pragma unroll
+            for (int q = 0; q < U; ++q) p[q] = split_at(j + q);
+            T yy[U], mm[U];
+#pragma unroll
+            for (int q = 0; q < U; ++q) {
+                yy[q] = y[p[q] & 0xffffu];
+                mm[q] = m[p[q] >> 16];
+            }
+#pragma unroll
+            for (int q = 0; q < U; ++q) dst = (T)((double)dst + ((double)yy[q] * (double)mm[q]) * (double)scale);
+        }
+        for (; j < P; ++j) {
+            const uint32_t p = split_at(j);
+            dst = (T)((double)dst + ((double)y[p & 0xffffu] * (double)m[p >> 16]) * (double)scale);
+        }
+        return dst;
+    }
+    T acc = T(0);
+    int j = 0;
+    for (; j + U <= P; j += U) {
+        uint32_t p[U];
+#pragma unroll
+        for (int q = 0; q < U; ++q) p[q] = split_at(j + q);
+        T yy[U], mm[U];
+#pragma unroll
+        for (int q = 0; q < U; ++q) {
+            yy[q] = y[p[q] & 0xffffu];
+            mm[q] = m[p[q] >> 16];
+        }
+#pragma unroll
+        for (int q = 0; q < U; ++q) {
+            const T prod = yy[q] * mm[q];
+            acc = acc + prod;
+        }
+    }
+    for (; j < P; ++j) {
+        const uint32_t p = split_at(j);
+        const T prod = y[p & 0xffffu] * m[p >> 16];
+        acc = acc + prod;
+    }
+    return acc;
+}
 
 template <typename T, int TEAM, int AUX, int MAIN, int RC>
 __global__ __launch_bounds__(kBlock) void k_sub(const SubArgs<T> a)
@@ -149,6 +221,17 @@ __global__ __launch_bounds__(kBlock) void k_sub(const SubArgs<T> a)
     T* sY = reinterpret_cast<T*>(smem_raw + head) + (size_t)team * per_team;
     T* sM = sY + (y_in_lds ? ycopies * nY : 0);
     T* s_stage = sM + nM;
+    // split table of the combine, transposed to [P][Wo] so that lanes on
+    // consecutive outputs read consecutive words
+    uint32_t* s_split = reinterpret_cast<uint32_t*>(smem_raw + index_lds_bytes(AUX, a));
+    const bool use_split_lds = a.split_lds != 0 && MAIN == MAIN_TABLE && (mode == OUT_EMA || mode == OUT_EMA_SCALED);
+    if (use_split_lds) {
+        const int words = a.P * a.Wo;
+        for (int i = threadIdx.x; i < words; i += kBlock) {
+            const int r = i / a.P;
+            s_split[(i - r * a.P) * a.Wo + r] = a.split[i];
+        }
+    }
 
     const bool use_xidx = AUX == AUX_ROWS && a.xidx != nullptr;
     if (use_xidx) {
@@ -581,25 +664,12 @@ __global__ __launch_bounds__(kBlock) void k_sub(const SubArgs<T> a)
                     else res = y[r];
                 } else {
                     const T* m = sM + b * Wm;
-                    const uint32_t* sp = a.split + (size_t)r * P;
-                    if (mode == OUT_EMA_SCALED) {
-                        // DataTableColMajor.cpp:367: dst + (a * (double)b) * scale
-                        T dst = T(0);
-                        for (int j = 0; j < P; ++j) {
-                            const uint32_t p = sp[j];
-                            const double t = ((double)y[p & 0xffffu] * (double)m[p >> 16]) * (double)a.scale;
-                            dst = (T)((double)dst + t);
-                        }
-                        res = dst;
-                    } else {
-                        T acc = T(0);
-                        for (int j = 0; j < P; ++j) {
-                            const uint32_t p = sp[j];
-                            const T prod = y[p & 0xffffu] * m[p >> 16];
-                            acc = acc + prod;
-                        }
-                        res = acc;
-                    }
+                    // split j of output r: s_split[j * Wo + r] (LDS) or a.split[r * P + j]
+                    if (use_split_lds)
+                        res = combine_splits<T>([&](int j) { return s_split[j * Wo + r]; }, P, y, m, mode, a.scale);
+                    else
+                        res = combine_splits<T>([&](int j) { return a.split[(size_t)r * P + j]; }, P, y, m, mode,
+                                                a.scale);
                 }
                 a.out[(int64_t)v * a.ldo + o] = res;
             }
@@ -895,8 +965,40 @@ __global__ void k_bwd_node(const NodeArgs<T> a)
 // stay in global memory, where every workgroup reads the same few KB through
 // the cache.  Per value the same FP64 sum in the same order as bwd_node_item,
 // so the results are identical.
+// sum over a split list, in list order, of G[o] (or g0 at the root) times the
+// row value the entry names: row[r] (skip < 0), or the histogram row read
+// past the vertex's own color, row[r + (r >= skip)].  Entries are (o << 16 |
+// r).  The loads of 8 entries are issued before their terms are added.
+template <typename T, typename List>
+__device__ __forceinline__ double node_sum(List entry_at, int n, const T* G, double g0, const T* row, int skip)
+{
+    constexpr int U = 8;
+    double acc = 0.0;
+    int p = 0;
+    for (; p + U <= n; p += U) {
+        uint32_t e[U];
+#pragma unroll
+        for (int q = 0; q < U; ++q) e[q] = entry_at(p + q);
+        double gv[U], rv[U];
+#pragma unroll
+        for (int q = 0; q < U; ++q) {
+            const int r = (int)(e[q] & 0xffffu);
+            gv[q] = G ? (double)G[e[q] >> 16] : g0;
+            rv[q] = (double)row[r + (skip >= 0 && r >= skip ? 1 : 0)];
+        }
+#pragma unroll
+        for (int q = 0; q < U; ++q) acc += gv[q] * rv[q];
+    }
+    for (; p < n; ++p) {
+        const uint32_t e = entry_at(p);
+        const int r = (int)(e & 0xffffu);
+        acc += (G ? (double)G[e >> 16] : g0) * (double)row[r + (skip >= 0 && r >= skip ? 1 : 0)];
+    }
+    return acc;
+}
+
 template <typename T>
-__global__ __launch_bounds__(256) void k_bwd_node_lds(const NodeArgs<T> a, int vpb)
+__global__ __launch_bounds__(256) void k_bwd_node_lds(const NodeArgs<T> a, int vpb, int lists_lds)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const bool do_dy = a.part != 2;
@@ -905,7 +1007,15 @@ __global__ __launch_bounds__(256) void k_bwd_node_lds(const NodeArgs<T> a, int v
     const int wm = (do_dy && a.Tm) ? a.Wm : 0;
     const int yw = a.y_from_h ? a.k : a.Wy;  // Y row length in global memory
     const int wy = do_gm ? yw : 0;
-    T* sG = reinterpret_cast<T*>(smem_raw);  // [vpb][wg]
+    // lists_lds: the split lists the values read (by_auxT for dY, by_mainT
+    // for G_main) are copied to LDS once per workgroup, ahead of the rows
+    const int la = lists_lds && do_dy && a.Tm ? a.Pa * a.Wy : 0;
+    const int lm = lists_lds && do_gm ? a.Pm * a.Wm : 0;
+    uint32_t* s_la = reinterpret_cast<uint32_t*>(smem_raw);
+    uint32_t* s_lm = s_la + la;
+    for (int i = threadIdx.x; i < la; i += blockDim.x) s_la[i] = a.by_auxT[i];
+    for (int i = threadIdx.x; i < lm; i += blockDim.x) s_lm[i] = a.by_mainT[i];
+    T* sG = reinterpret_cast<T*>(smem_raw + (((size_t)4 * (la + lm) + 15) & ~(size_t)15));  // [vpb][wg]
     T* sM = sG + (size_t)vpb * wg;           // [vpb][wm]
     T* sY = sM + (size_t)vpb * wm;           // [vpb][wy]
     const int ny = do_dy ? a.Wy : 0;
@@ -934,22 +1044,21 @@ __global__ __launch_bounds__(256) void k_bwd_node_lds(const NodeArgs<T> a, int v
                     acc = G ? (double)G[x] : g0;
                 } else {
                     const T* m = sM + lv * wm;
-                    for (int p = 0; p < a.Pa; ++p) {
-                        const uint32_t e = a.by_auxT[(size_t)p * a.Wy + x];
-                        acc += (G ? (double)G[e >> 16] : g0) * (double)m[e & 0xffffu];
-                    }
+                    if (la)
+                        acc = node_sum<T>([&](int p) { return s_la[p * a.Wy + x]; }, a.Pa, G, g0, m, -1);
+                    else
+                        acc = node_sum<T>([&](int p) { return a.by_auxT[(size_t)p * a.Wy + x]; }, a.Pa, G, g0, m, -1);
                 }
                 a.dY[vb * a.Wy + x] = (T)(acc * a.scale);
             } else {
                 const int x = x0 - ny;
                 const int cv = a.colors[vb];
                 const T* y = sY + lv * wy;
-                for (int p = 0; p < a.Pm; ++p) {
-                    const uint32_t e = a.by_mainT[(size_t)p * a.Wm + x];
-                    const int ra = (int)(e & 0xffffu);
-                    const double yv = a.y_from_h ? (double)y[ra + (ra >= cv ? 1 : 0)] : (double)y[ra];
-                    acc += (G ? (double)G[e >> 16] : g0) * yv;
-                }
+                const int skip = a.y_from_h ? cv : -1;
+                if (lm)
+                    acc = node_sum<T>([&](int p) { return s_lm[p * a.Wm + x]; }, a.Pm, G, g0, y, skip);
+                else
+                    acc = node_sum<T>([&](int p) { return a.by_mainT[(size_t)p * a.Wm + x]; }, a.Pm, G, g0, y, skip);
                 a.Gm[vb * a.Wm + x] = (T)(acc * a.scale);
             }
         }
@@ -961,10 +1070,13 @@ __global__ __launch_bounds__(256) void k_bwd_node_lds(const NodeArgs<T> a, int v
 // forward+backward with every main-table launch staged: the 12-vertex tree
 // at R-MAT scale 22 (7.4-21 reads per staged value) ran 8% faster, the
 // 10-vertex tree (2.6-5.4) 4% slower, the 5-star at scale 18 (0.9-1.6) 16%
-// slower.
+// slower.  The split lists go to LDS too when together they fit
+// kBwdListsLds (rows then get kBwdLdsBudget - kBwdListsTrim).
 constexpr size_t kBwdLdsBudget = 32 * 1024;
 constexpr size_t kBwdLdsMax = 48 * 1024;
 constexpr int kBwdStageReuse = 6;
+constexpr size_t kBwdListsLds = 40 * 1024;
+constexpr size_t kBwdListsTrim = 8 * 1024;
 
 template <typename T>
 hipError_t launch_bwd_node(const NodeArgs<T>& a, int num_cus, hipStream_t st)
@@ -983,10 +1095,19 @@ hipError_t launch_bwd_node(const NodeArgs<T>& a, int num_cus, hipStream_t st)
         hipLaunchKernelGGL(k_bwd_node<T>, flat_grid(total), dim3(256), 0, st, a);
         return hipGetLastError();
     }
-    const int vpb = (int)std::max<size_t>(1, std::min<size_t>(64, kBwdLdsBudget / std::max<size_t>(row, 1)));
+    const size_t lists = (size_t)4 * (((do_dy && a.Tm) ? (size_t)a.Pa * a.Wy : 0) + (do_gm ? (size_t)a.Pm * a.Wm : 0));
+    const size_t lists_head = (lists + 15) & ~(size_t)15;
+    auto vpb_for = [&](size_t budget) {
+        return (int)std::max<size_t>(1, std::min<size_t>(64, budget / std::max<size_t>(row, 1)));
+    };
+    // workgroups stay within 64 KB of LDS
+    const int lists_lds = lists > 0 && lists <= kBwdListsLds &&
+                          lists_head + (size_t)vpb_for(kBwdLdsBudget - kBwdListsTrim) * row <= 64 * 1024;
+    const size_t head = lists_lds ? lists_head : 0;
+    const int vpb = vpb_for(lists_lds ? kBwdLdsBudget - kBwdListsTrim : kBwdLdsBudget);
     const int64_t ngroups = ((int64_t)a.n_act * a.B + vpb - 1) / vpb;
     const unsigned grid = (unsigned)std::min<int64_t>(ngroups, (int64_t)std::max(num_cus, 1) * 8);
-    hipLaunchKernelGGL(k_bwd_node_lds<T>, dim3(grid), dim3(256), (size_t)vpb * row, st, a, vpb);
+    hipLaunchKernelGGL(k_bwd_node_lds<T>, dim3(grid), dim3(256), head + (size_t)vpb * row, st, a, vpb, lists_lds);
     return hipGetLastError();
 }
 

@@ -107,6 +107,7 @@ struct SubArgs {
     int32_t Wo;              // output values per coloring
     int32_t P;               // non-zero splits per output set
     const uint32_t* split;   // [Wo][P] (main_rank << 16 | y_rank), reference order
+    int32_t split_lds;       // 1: the workgroup first copies split into LDS as [P][Wo] (combine reads)
     int32_t out_mode;
     double* root_vals;       // [B][n_root] per-vertex root values
     int64_t n_root;
