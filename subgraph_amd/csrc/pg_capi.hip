@@ -113,6 +113,7 @@ struct DevSub {
     // row-wise index table of AUX_ROWS (SubArgs::xidx), when it fits kGidxMaxBytes
     DevBuf xidx;     // [k*k][Wa4] int16
     int Wa4 = 0;
+    DevBuf xidx_g;   // the same table for rows up to 512 values, read from memory (RUNS_SEQ)
     // the adjoint's transposed gather (AUX_GATHER_T) through its own index table
     DevBuf gidxT;    // [(k+1)*k][Wa4] int16
 };
@@ -151,6 +152,8 @@ struct pg_ctx {
     int64_t aux_rows = 0;       // neighbour sums row-wise: 1 always, 0 never, -1 per subtemplate
     int64_t stage_rows = 0;     // gathers read neighbour rows staged in LDS (1) or directly (0)
     int64_t split_lds = 1;      // combines read the split table from LDS (1) when it fits, or from memory (0)
+    int64_t color_runs = -1;    // wide-row neighbour sums by colour runs: 1 always, 0 never, -1 rows of 32+ values
+    int64_t exact_order = 1;    // forward neighbour sums in the reference's order (one group, hubs split by columns)
     int64_t reg_cols = 0;       // values per lane of gathers: 0 one for rows under kRegCols, else 1 or 4
     int64_t adjoint_recompute = -1;  // adjoint tables: 0 kept, 1 recomputed, -1 recomputed when that fits 4x more
     int64_t vertex_order = 0;   // internal order at the next upload: 0 descending degree, 1 degree classes + BFS
@@ -170,6 +173,7 @@ struct pg_ctx {
     int32_t n_hub = 0, hi_end = 0, mid_end = 0;  // [0,n_hub) hubs, [n_hub,hi_end) high, [hi_end,mid_end) mid
     int32_t n_hub_items = 0, n_slots = 0;
     DevBuf d_hub_items, d_slot_begin;
+    DevBuf d_hub_iota;  // [n_hub+1] h -> h: one partial row per hub (column-split hub launches)
     // per-call workspace
     DevBuf d_arena, d_colors_in;
     // captured passes, and what invalidates them
@@ -272,6 +276,12 @@ int build_buckets(pg_ctx* c)
         HIP_TRY(c, hipMemcpy(c->d_hub_items.p, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice));
     HIP_TRY(c, hipMemcpy(c->d_slot_begin.p, slot_begin.data(), slot_begin.size() * sizeof(int32_t),
                          hipMemcpyHostToDevice));
+    {
+        std::vector<int32_t> iota((size_t)nh + 1);
+        std::iota(iota.begin(), iota.end(), 0);
+        HIP_TRY(c, c->d_hub_iota.alloc(iota.size() * sizeof(int32_t)));
+        HIP_TRY(c, hipMemcpy(c->d_hub_iota.p, iota.data(), iota.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    }
     drop_graphs(c);  // captured passes hold the old bucket ranges
     c->generation++;
     c->built_deg = c->hub_degree;
@@ -347,7 +357,7 @@ MemPlan plan_memory(const pg_ctx* c, const pg_plan* p, int B, size_t tsz)
 // -1 by that fraction.
 int aux_mode_for(const pg_ctx* c, const pg_plan* p, const DevSub& d)
 {
-    if (c->aux_rows == 0) return AUX_GATHER;
+    if (c->aux_rows == 0 || c->exact_order) return AUX_GATHER;
     if (c->aux_rows == 1) return AUX_ROWS;
     return 2 * (p->K - d.aa + 1) >= p->K ? AUX_ROWS : AUX_GATHER;
 }
@@ -430,6 +440,10 @@ size_t lds_of(int team, int aux, int main_mode, int ng, int out_mode, int B, con
     return sub_lds_bytes<T>(team, aux, main_mode, a);
 }
 
+int runs_mode_for(const pg_ctx* c, int aux, int wg, int used_per_mille);
+int runs_used_per_mille(const pg_plan* p, const DevSub& d, int aux);
+bool runs_seq_for(const pg_ctx* c, int wa, int used_per_mille);
+
 // LDS of the largest launch of subtemplate d for B colorings, over every
 // launch shape launch_gathering (or the aux-leaf read) would use, with or
 // without the split table staged
@@ -442,7 +456,44 @@ size_t sub_lds_max_t(const pg_ctx* c, const pg_plan* p, const DevSub& d, int B, 
     size_t m = lds_of<T>(fin_team, d.aa == 1 ? AUX_READ : AUX_PARTIALS, mm, 1, out, B, p, d, 0, split);
     if (d.aa == 1) return m;
     const int aux = aux_mode_for(c, p, d);
+    if (runs_mode_for(c, aux, d.Wa, runs_used_per_mille(p, d, AUX_GATHER)) >= 0) {
+        // colour runs: a team per (vertex, colouring), LDS independent of B
+        SubArgs<T> a{};
+        a.B = B;
+        a.K = p->K;
+        a.Wy = d.Wy;
+        a.Wm = d.Wm;
+        a.Wo = d.Wo;
+        a.P = d.P;
+        a.split_lds = split;
+        for (int o : {out, (int)OUT_PARTIAL}) {
+            a.out_mode = o;
+            m = std::max(m, runs_lds_bytes<T>(mm, a));
+        }
+        return m;
+    }
     const int rc = rc_for(c, aux, B, d.Wy);
+    if (c->exact_order && aux == AUX_GATHER && d.xidx_g.p &&
+        runs_seq_for(c, d.Wa, runs_used_per_mille(p, d, AUX_GATHER))) {
+        // RUNS_SEQ: one launch of teams per (vertex, colouring), LDS independent of B
+        SubArgs<T> a{};
+        a.B = B;
+        a.K = p->K;
+        a.Wy = d.Wy;
+        a.Wm = d.Wm;
+        a.Wo = d.Wo;
+        a.P = d.P;
+        a.split_lds = split;
+        a.out_mode = out;
+        return runs_lds_bytes<T>(mm, a);
+    }
+    if (c->exact_order && aux == AUX_GATHER) {
+        // launch_gathering_exact: teams of max(8, cl) lanes, one neighbour group
+        const int cl = std::max(8, cl_of(aux, B, p->k, d.Wy, d.Wa, rc));
+        const int sp = rc == kRegCols ? stage_pad(c, aux, d.gidx.p != nullptr, B, d.Wy, d.Wa, cl, sizeof(T)) : 0;
+        m = std::max(m, lds_of<T>(cl, aux, mm, 1, OUT_PARTIAL, B, p, d, 0, split));
+        return std::max(m, lds_of<T>(cl, sp > 0 ? AUX_STAGED : aux, mm, 1, out, B, p, d, sp, split));
+    }
     const int cl = cl_of(aux, B, p->k, d.Wy, d.Wa, rc);
     const int sp = rc == kRegCols ? stage_pad(c, aux, d.gidx.p != nullptr, B, d.Wy, d.Wa, cl, sizeof(T)) : 0;
     const int aux_l = sp > 0 ? AUX_STAGED : aux;
@@ -532,9 +583,137 @@ struct StepBytes {
 // The step's algorithmic bytes are shared among its launches by the entries
 // and vertices each covers: the CSC by entries + vertices, per-entry bytes by
 // entries, per-vertex bytes by vertices, so per-kernel sums stay exact.
-template <typename T>
-int launch_gathering(pg_ctx* c, int aux, int main_mode, SubArgs<T> a, T* part, int y_cols, StepBytes bytes)
+// Colour-run neighbour sums (pg_runs.hip) for a gathering launch over rows
+// of wg values: RUNS_FWD for the forward gather, RUNS_T for the adjoint's
+// transposed one, -1 for the k_sub kernels.  Option "color_runs": 1 always
+// (rows up to 512 values), 0 never, -1 rows of at least kRunsMinWidth values
+// (the 12-vertex tree's 55- to 462-value rows; narrow rows fit k_sub's
+// per-lane gathers).
+constexpr int kRunsMinWidth = 32;
+int runs_mode_for(const pg_ctx* c, int aux, int wg, int used_per_mille)
 {
+    if (c->color_runs == 0 || runs_values_per_lane(wg) == 0) return -1;
+    if (c->color_runs < 0 && (wg < kRunsMinWidth || used_per_mille < 500)) return -1;
+    if (aux == AUX_GATHER || aux == AUX_ROWS || aux == AUX_STAGED) return c->exact_order ? -1 : RUNS_FWD;
+    if (aux == AUX_GATHER_T) return RUNS_T;
+    if (aux == AUX_GATHER_G) return RUNS_G;
+    return -1;
+}
+
+// Share (per mille) of a gathered row's values a vertex uses: forward and
+// gradient gathers read an aux row at the sets that avoid the vertex's colour,
+// (K - aa + 1) / K of them; the transposed gather reads a dY row at the sets
+// that hold it, aa / K.  Colour runs read whole rows, so by default they are
+// taken when at least half of a row is used.
+int runs_used_per_mille(const pg_plan* p, const DevSub& d, int aux)
+{
+    const int K = p->K;
+    return aux == AUX_GATHER_T ? 1000 * d.aa / K : 1000 * (K - d.aa + 1) / K;
+}
+
+// The exact-order forward sum by whole rows (RUNS_SEQ): with colour runs on
+// (1), for rows up to 512 values; by default (-1), for rows of 32 to 128
+// values of which a vertex uses at least half.  Wider rows keep the per-lane
+// gathers: a whole 462-value row costs 1.8 KB per entry and coloring against
+// the 55% a vertex uses, and a hub's single team adds its 166k rows through
+// one LDS row (the 12-vertex tree's (7;1,6) step at R-MAT s22: 323 ms per
+// 4-coloring launch by rows, 256 ms by gathers; its 55-value steps: 90 ms by
+// rows, 164 ms by gathers).
+bool runs_seq_for(const pg_ctx* c, int wa, int used_per_mille)
+{
+    const int rv = runs_values_per_lane(wa);
+    if (c->color_runs == 0 || rv == 0) return false;
+    return c->color_runs > 0 || (wa >= kRunsMinWidth && rv <= 2 && used_per_mille >= 500);
+}
+
+std::string runs_name(size_t tsz, int mode, int main_mode)
+{
+    char buf[64];
+    snprintf(buf, sizeof buf, "k_runs<%s,%s,%d>", tsz == 4 ? "float" : "double",
+             mode == RUNS_T ? "t" : mode == RUNS_G ? "g" : mode == RUNS_SEQ ? "seq" : "fwd", main_mode);
+    return buf;
+}
+
+// launch_gathering with colour runs: the hub chunks' partial rows, all other
+// vertices in one launch, then the hub rows from their partials
+template <typename T>
+int launch_gathering_runs(pg_ctx* c, int rmode, int main_mode, SubArgs<T> a, T* part, int y_cols, StepBytes bytes)
+{
+    const hipStream_t st = c->stream;
+    const int cus = c->num_cus;
+    const double Z = (double)c->nnz, N = (double)c->n_act;
+    auto share = [&](double ents, double verts) {
+        double b = 0;
+        if (Z + N > 0) b += bytes.csc * (ents + verts) / (Z + N);
+        if (Z > 0) b += bytes.per_entry * ents / Z;
+        if (N > 0) b += bytes.per_vertex * verts / N;
+        return b;
+    };
+    if (c->n_hub > 0) {
+        SubArgs<T> h = a;
+        h.out = part;
+        h.ldo = y_cols;
+        h.out_mode = OUT_PARTIAL;
+        h.items = c->d_hub_items.as<int4>();
+        h.n_items = c->n_hub_items;
+        h.item_base = 0;
+        record(c, runs_name(sizeof(T), rmode, main_mode), share(c->h_rowptr[c->n_hub], 0), true);
+        HIP_TRY(c, launch_runs<T>(rmode, main_mode, h, cus, c->blocks_per_cu, st));
+        record(c, "", 0, false);
+    }
+    if (c->n_hub < c->n_act) {
+        SubArgs<T> r = a;
+        r.items = nullptr;
+        r.item_base = c->n_hub;
+        r.n_items = c->n_act - c->n_hub;
+        record(c, runs_name(sizeof(T), rmode, main_mode),
+               share((double)c->nnz - c->h_rowptr[c->n_hub], c->n_act - c->n_hub), true);
+        HIP_TRY(c, launch_runs<T>(rmode, main_mode, r, cus, c->blocks_per_cu, st));
+        record(c, "", 0, false);
+    }
+    if (c->n_hub > 0) {
+        SubArgs<T> f = a;
+        f.aux = part;
+        f.lda = y_cols;
+        f.items = nullptr;
+        f.item_base = 0;
+        f.n_items = c->n_hub;
+        f.ng = 1;
+        f.Wa = a.Wy;
+        const int team = team_for(std::max(y_cols, a.B * a.Wo));
+        record(c, kname<T>(team, AUX_PARTIALS, main_mode), share(0, c->n_hub), true);
+        HIP_TRY(c, launch_sub<T>(team, AUX_PARTIALS, main_mode, kRegCols, f, cus, c->blocks_per_cu, st));
+        record(c, "", 0, false);
+    }
+    return PG_OK;
+}
+
+template <typename T>
+int launch_gathering_exact(pg_ctx* c, int aux, int main_mode, SubArgs<T> a, T* part, int y_cols, StepBytes bytes);
+
+template <typename T>
+int launch_gathering(pg_ctx* c, int aux, int main_mode, SubArgs<T> a, T* part, int y_cols, StepBytes bytes,
+                     int used_per_mille = 0)
+{
+    {
+        const int rmode = runs_mode_for(c, aux, a.Wa, used_per_mille);
+        if (rmode >= 0) return launch_gathering_runs<T>(c, rmode, main_mode, a, part, y_cols, bytes);
+    }
+    if (c->exact_order && (aux == AUX_GATHER || aux == AUX_STAGED)) {
+        if (a.xidx_g && runs_seq_for(c, a.Wa, used_per_mille)) {
+            // every vertex (hubs included) on a team with its whole neighbour list
+            SubArgs<T> r = a;
+            r.items = nullptr;
+            r.item_base = 0;
+            r.n_items = c->n_act;
+            record(c, runs_name(sizeof(T), RUNS_SEQ, main_mode), bytes.csc + bytes.per_entry + bytes.per_vertex,
+                   true);
+            HIP_TRY(c, launch_runs<T>(RUNS_SEQ, main_mode, r, c->num_cus, c->blocks_per_cu, c->stream));
+            record(c, "", 0, false);
+            return PG_OK;
+        }
+        return launch_gathering_exact<T>(c, AUX_GATHER, main_mode, a, part, y_cols, bytes);
+    }
     const int rc = rc_for(c, aux, a.B, a.Wy);
     const int cl = cl_of(aux, a.B, a.k, a.Wy, a.Wa, rc);
     a.segpad = rc == kRegCols ? stage_pad(c, aux, a.gidx != nullptr, a.B, a.Wy, a.Wa, cl, sizeof(T)) : 0;
@@ -585,6 +764,76 @@ int launch_gathering(pg_ctx* c, int aux, int main_mode, SubArgs<T> a, T* part, i
         f.item_base = 0;
         f.n_items = c->n_hub;
         f.ng = 1;
+        const int team = team_for(std::max(y_cols, a.B * a.Wo));
+        record(c, kname<T>(team, AUX_PARTIALS, main_mode), share(0, c->n_hub), true);
+        HIP_TRY(c, launch_sub<T>(team, AUX_PARTIALS, main_mode, kRegCols, f, cus, c->blocks_per_cu, st));
+        record(c, "", 0, false);
+    }
+    return PG_OK;
+}
+
+// The forward neighbour sums in the reference's order (CSCGraph.hpp:254-307:
+// every Y value adds its neighbours' values one after another, in ascending
+// column order, starting from zero): one neighbour group per team, so a lane
+// adds its values neighbour after neighbour; hubs are split by columns, not
+// by neighbours (a team per chunk of slots, with the whole neighbour list),
+// into one partial row per hub that the finishing launch then combines.
+// The FP32 sums are then the reference's, bit for bit.
+template <typename T>
+int launch_gathering_exact(pg_ctx* c, int aux, int main_mode, SubArgs<T> a, T* part, int y_cols, StepBytes bytes)
+{
+    const int rc = rc_for(c, aux, a.B, a.Wy);
+    const int nslots = slots_for(a.B, a.Wy, rc);
+    const int cl = std::max(8, cl_of(aux, a.B, a.k, a.Wy, a.Wa, rc));
+    a.segpad = rc == kRegCols ? stage_pad(c, aux, a.gidx != nullptr, a.B, a.Wy, a.Wa, cl, sizeof(T)) : 0;
+    const int aux_b = a.segpad > 0 ? AUX_STAGED : aux;
+    const hipStream_t st = c->stream;
+    const int cus = c->num_cus;
+    const double Z = (double)c->nnz, N = (double)c->n_act;
+    auto share = [&](double ents, double verts) {
+        double b = 0;
+        if (Z + N > 0) b += bytes.csc * (ents + verts) / (Z + N);
+        if (Z > 0) b += bytes.per_entry * ents / Z;
+        if (N > 0) b += bytes.per_vertex * verts / N;
+        return b;
+    };
+    a.ng = 1;
+    if (c->n_hub > 0) {
+        // a part per chunk of cl slots: the same per-entry work as a
+        // non-hub vertex's team, whose chunks run one after another
+        const int team = cl;
+        SubArgs<T> h = a;
+        h.segpad = 0;
+        h.out = part;
+        h.ldo = y_cols;
+        h.out_mode = OUT_PARTIAL;
+        h.items = nullptr;
+        h.col_parts = (nslots + team - 1) / team;
+        h.n_items = c->n_hub * h.col_parts;
+        h.item_base = 0;
+        record(c, kname<T>(team, aux, main_mode, rc), share(c->h_rowptr[c->n_hub], 0), true);
+        HIP_TRY(c, launch_sub<T>(team, aux, main_mode, rc, h, cus, c->blocks_per_cu, st));
+        record(c, "", 0, false);
+    }
+    if (c->n_hub < c->n_act) {
+        SubArgs<T> r = a;
+        r.items = nullptr;
+        r.item_base = c->n_hub;
+        r.n_items = c->n_act - c->n_hub;
+        record(c, kname<T>(cl, aux_b, main_mode, rc),
+               share((double)c->nnz - c->h_rowptr[c->n_hub], c->n_act - c->n_hub), true);
+        HIP_TRY(c, launch_sub<T>(cl, aux_b, main_mode, rc, r, cus, c->blocks_per_cu, st));
+        record(c, "", 0, false);
+    }
+    if (c->n_hub > 0) {
+        SubArgs<T> f = a;
+        f.aux = part;
+        f.lda = y_cols;
+        f.items = nullptr;
+        f.item_base = 0;
+        f.n_items = c->n_hub;
+        f.slot_begin = c->d_hub_iota.as<int32_t>();
+        f.segpad = 0;
         const int team = team_for(std::max(y_cols, a.B * a.Wo));
         record(c, kname<T>(team, AUX_PARTIALS, main_mode), share(0, c->n_hub), true);
         HIP_TRY(c, launch_sub<T>(team, AUX_PARTIALS, main_mode, kRegCols, f, cus, c->blocks_per_cu, st));
@@ -705,8 +954,10 @@ int enqueue_step(pg_ctx* c, const pg_plan* p, const DevSub& d, int B, const Pass
         a.gidx = d.gidx.as<int16_t>();
         a.Wy4 = d.Wy4;
         a.xidx = d.xidx.as<int16_t>();
+        a.xidx_g = d.xidx_g.as<int16_t>();
         a.Wa4 = d.Wa4;
-        int rc = launch_gathering<T>(c, aux_mode_for(c, p, d), main_mode, a, pb.part, B * d.Wy, sb);
+        int rc = launch_gathering<T>(c, aux_mode_for(c, p, d), main_mode, a, pb.part, B * d.Wy, sb,
+                                     runs_used_per_mille(p, d, AUX_GATHER));
         if (rc) return rc;
     }
     if (d.root) HIP_TRY(c, launch_root_reduce(pb.rootv, c->n_act, B, pb.tmp, pb.rawd, st));
@@ -906,6 +1157,7 @@ const std::vector<Knob>& tune_knobs()
         {"reg_cols", &pg_ctx::reg_cols, {0, 1}},
         {"stage_rows", &pg_ctx::stage_rows, {0, 1}},
         {"split_lds", &pg_ctx::split_lds, {1, 0}},
+        {"color_runs", &pg_ctx::color_runs, {-1, 0}},
         {"aux_rows", &pg_ctx::aux_rows, {0, -1}},
         {"bucket_high", &pg_ctx::bucket_high, {16, 32, 64, 128, 256, 512}},
         {"bucket_mid", &pg_ctx::bucket_mid, {4, 8, 16, 32, 64, 128}},
@@ -1460,6 +1712,7 @@ int enqueue_sub_backward(pg_ctx* c, const pg_plan* p, const DevSub& d, int B, co
         a.Wy = d.Wy;
         a.Wo = d.Wy;
         a.ymask = d.ymask.as<uint16_t>();
+        a.xmask = d.amask.as<uint16_t>();  // the set of each position of an aux row (colour runs)
         a.gidx = d.gidx.as<int16_t>();
         a.Wy4 = d.Wy4;
         a.out = Ybuf;
@@ -1469,7 +1722,8 @@ int enqueue_sub_backward(pg_ctx* c, const pg_plan* p, const DevSub& d, int B, co
         a.grad = c->d_grad.as<double>();
         // reads T_aux and dY, writes Y
         int rc = launch_gathering<T>(c, AUX_GATHER_G, MAIN_LEAF, a, pb.part, B * d.Wy,
-                                     StepBytes{csc_bytes(c), col * Ca + 16.0 * Z, col * 2.0 * Ca});
+                                     StepBytes{csc_bytes(c), col * Ca + 16.0 * Z, col * 2.0 * Ca},
+                                     runs_used_per_mille(p, d, AUX_GATHER_G));
         if (rc) return rc;
         if (na.Tm) {
             na.part = 2;
@@ -1486,13 +1740,14 @@ int enqueue_sub_backward(pg_ctx* c, const pg_plan* p, const DevSub& d, int B, co
         t.Wy = d.Wa;  // values per coloring: the aux child's row
         t.Wo = d.Wa;
         t.ymask = d.amask.as<uint16_t>();
+        t.xmask = d.ymask.as<uint16_t>();  // the set of each position of a neighbour's dY row (colour runs)
         t.gidx = d.gidxT.as<int16_t>();
         t.Wy4 = d.Wa4;
         t.out = Ga;
         t.ldo = B * d.Wa;
         // reads the neighbours' dY, writes G_aux
         rc = launch_gathering<T>(c, AUX_GATHER_T, MAIN_LEAF, t, pb.part, B * d.Wa,
-                                 StepBytes{csc_bytes(c), col * Ca, col * Ca});
+                                 StepBytes{csc_bytes(c), col * Ca, col * Ca}, runs_used_per_mille(p, d, AUX_GATHER_T));
         if (rc) return rc;
     } else {
         // aux child a leaf: Y is the histogram H; the entry's derivative is
@@ -2065,6 +2320,18 @@ extern "C" int pg_set_option(pg_ctx* c, const char* key, int64_t v)
             drop_graphs(c);
             c->stage_rows = v;
         }
+    } else if (k == "exact_order") {
+        if (v != 0 && v != 1) return set_err(c, PG_ERR_INVALID, "exact_order must be 0 or 1");
+        if (v != c->exact_order) {
+            drop_graphs(c);
+            c->exact_order = v;
+        }
+    } else if (k == "color_runs") {
+        if (v < -1 || v > 1) return set_err(c, PG_ERR_INVALID, "color_runs must be -1, 0 or 1");
+        if (v != c->color_runs) {
+            drop_graphs(c);
+            c->color_runs = v;
+        }
     } else if (k == "split_lds") {
         if (v != 0 && v != 1) return set_err(c, PG_ERR_INVALID, "split_lds must be 0 or 1");
         if (v != c->split_lds) {
@@ -2099,6 +2366,8 @@ extern "C" int pg_get_option(const pg_ctx* c, const char* key, int64_t* v)
     else if (k == "aux_rows") *v = c->aux_rows;
     else if (k == "stage_rows") *v = c->stage_rows;
     else if (k == "split_lds") *v = c->split_lds;
+    else if (k == "color_runs") *v = c->color_runs;
+    else if (k == "exact_order") *v = c->exact_order;
     else if (k == "reg_cols") *v = c->reg_cols;
     else if (k == "adjoint_recompute") *v = c->adjoint_recompute;
     else if (k == "vertex_order") *v = c->vertex_order;
@@ -2506,10 +2775,16 @@ extern "C" int pg_plan_create(pg_ctx* c, int k, int nsub, const int32_t* sub_siz
             }
             d->Wa4 = (d->Wa + 3) / 4 * 4;
             const size_t xbytes = (size_t)2 * k * k * d->Wa4;
-            if (xbytes <= kGidxMaxBytes) {
+            if (xbytes <= kGidxMaxBytes || runs_values_per_lane(d->Wa) > 0) {
                 const std::vector<int16_t> x = rows_index_table(k, am, rank, d->Wa4);
-                HIP_TRY(c, d->xidx.alloc(xbytes));
-                HIP_TRY(c, hipMemcpy(d->xidx.p, x.data(), xbytes, hipMemcpyHostToDevice));
+                if (xbytes <= kGidxMaxBytes) {
+                    HIP_TRY(c, d->xidx.alloc(xbytes));
+                    HIP_TRY(c, hipMemcpy(d->xidx.p, x.data(), xbytes, hipMemcpyHostToDevice));
+                }
+                if (runs_values_per_lane(d->Wa) > 0) {
+                    HIP_TRY(c, d->xidx_g.alloc(xbytes));
+                    HIP_TRY(c, hipMemcpy(d->xidx_g.p, x.data(), xbytes, hipMemcpyHostToDevice));
+                }
             }
             const size_t tbytes = (size_t)2 * (k + 1) * k * d->Wa4;
             if (tbytes <= kGidxMaxBytes) {

@@ -59,6 +59,10 @@ struct SubArgs {
     int32_t n_items;
     int32_t item_base;
     const int4* items;
+    // col_parts > 0: item i is vertex i / col_parts (a hub, internal id =
+    // partial slot) with its whole neighbour list, restricted to the
+    // (i % col_parts)-th chunk of column lanes' slots (gathers, OUT_PARTIAL)
+    int32_t col_parts;
     const int32_t* rowptr;   // [n_act+1] internal CSR
     const int32_t* col;      // [nnz] neighbour (internal id)
     const T* w;              // [nnz] entry weights, nullptr = 1
@@ -90,6 +94,7 @@ struct SubArgs {
     // cv) that value j of a neighbour row (color cu) adds to, or -1
     const int16_t* xidx;
     int32_t Wa4;             // Wa rounded up to 4
+    const int16_t* xidx_g;   // the same table in device memory for any width (RUNS_SEQ reads it there)
     const int32_t* slot_begin;  // hub h's partial slots [slot_begin[h], slot_begin[h+1])
     // AUX_GATHER_G: each neighbour's contribution x to the vertex's Y row is
     // also contracted with dY there, and grad[e] += sum over the row of dY * x
@@ -133,6 +138,29 @@ size_t sub_lds_bytes(int team, int aux, int main_mode, const SubArgs<T>& a);
 template <typename T>
 hipError_t launch_sub(int team, int aux, int main_mode, int rc, const SubArgs<T>& a, int num_cus, int blocks_per_cu,
                       hipStream_t st);
+
+// Neighbour sums over wide rows, read whole, one neighbour colour at a time
+// (pg_runs.hip): a team of 64 lanes per (item, colouring).  RUNS_FWD sums
+// the aux child's rows into Y (SubArgs::Wa values per row, their set masks in
+// xmask; Y has Wy values) and finishes as k_sub does for out_mode; RUNS_T is
+// the adjoint's transposed gather: it sums the neighbours' dY rows (Wa values,
+// masks in xmask) into the vertex's G_aux row (Wy values), OUT_Y or
+// OUT_PARTIAL.  Rows up to 512 values per colouring.
+// RUNS_G is the adjoint's gradient gather: RUNS_FWD's sum written out
+// (OUT_Y / OUT_PARTIAL) and grad[e] += dY[v] . x_u for every entry.
+// RUNS_SEQ is the forward sum in the reference's order: each neighbour row
+// read whole and coalesced, its values added into Y (LDS) neighbour after
+// neighbour through xidx (every (vertex colour, neighbour colour) row of it),
+// a team per (vertex, colouring) with the whole neighbour list.
+enum RunsMode : int { RUNS_FWD = 0, RUNS_T = 1, RUNS_G = 2, RUNS_SEQ = 3 };
+// values per lane for a gathered row of wg values (0: too wide)
+int runs_values_per_lane(int wg);
+template <typename T>
+size_t runs_lds_bytes(int main_mode, const SubArgs<T>& a);
+template <typename T>
+size_t runs_g_lds_bytes(const SubArgs<T>& a);
+template <typename T>
+hipError_t launch_runs(int mode, int main_mode, const SubArgs<T>& a, int num_cus, int blocks_per_cu, hipStream_t st);
 
 // Per-pass values read from device memory, so that one captured graph serves
 // every pass of the same shape (the host writes them before each replay).

@@ -1,0 +1,718 @@
+// Neighbour sums over wide count-table rows, read whole and coalesced, one
+// neighbour colour at a time ("colour runs").
+//
+// The reference's SpMM (CSCGraph::spmmSplit, CSCGraph.hpp:243-310) adds the
+// aux child's table columns of every neighbour u into the vertex's row.  In
+// the colour-relative layout (pg_kernels.hpp) the position of a colour set in
+// u's row depends on u's colour, so the gathering kernel (k_sub) reads, per
+// neighbour, the few values each lane needs at scattered positions.  Here a
+// team of 64 lanes owns one (vertex, colouring) pair and takes the neighbours
+// colour by colour: for colour c it adds the whole rows of the neighbours of
+// colour c (lane l holds positions l, l+64, ...: coalesced loads, no index
+// arithmetic per neighbour), then moves the sum, once, to the positions of
+// the vertex's row it belongs to.  That map depends only on (col(v), c):
+//   forward (RUNS_FWD): position j of a row of colour c is the set
+//     R_j + {c}; it adds to Y set R_j + {c} when that set avoids col(v);
+//   transposed (RUNS_T, the adjoint's G_aux): position t of a neighbour's dY
+//     row of colour c is the set A_t (A_t avoids c); it adds to the vertex's
+//     row entry A_t - {col(v)} when A_t holds col(v).
+// Neighbour sums are therefore formed per colour, in row order, and the
+// colours are added in increasing order: deterministic, and a different
+// FP32 rounding order than the reference's (within the parity tolerance).
+// After the sum the team finishes like k_sub: the eMA combine into the
+// output row (or the root value), a hub chunk's partial row, or the row
+// itself.
+#include "pg_kernels.hpp"
+
+#include <map>
+
+namespace pgbsc {
+
+namespace {
+
+__device__ __forceinline__ uint32_t expand_bit_r(uint32_t m, uint32_t c)
+{
+    return (m & ((1u << c) - 1u)) | ((m >> c) << (c + 1u));
+}
+
+__device__ __forceinline__ uint32_t squeeze_bit_r(uint32_t m, uint32_t c)
+{
+    return (m & ((1u << c) - 1u)) | ((m >> (c + 1u)) << c);
+}
+
+__host__ __device__ constexpr size_t rank_bytes(int K) { return (((size_t)2 << K) + 15) & ~(size_t)15; }
+
+// LDS: rank table, optional split table [P][Wo], then per team the summed
+// row (Wy values) and, for the combine of a main table, the main row
+template <typename T>
+__host__ __device__ size_t runs_head_bytes(const SubArgs<T>& a)
+{
+    const bool combine = a.out_mode == OUT_EMA || a.out_mode == OUT_EMA_SCALED;
+    return rank_bytes(a.K) + (a.split_lds && combine ? (((size_t)4 * a.P * a.Wo + 15) & ~(size_t)15) : 0);
+}
+
+template <typename T>
+__host__ __device__ int runs_main_cols(int main_mode, const SubArgs<T>& a)
+{
+    const bool comb = a.out_mode == OUT_EMA || a.out_mode == OUT_EMA_SCALED || a.out_mode == OUT_ROOT;
+    return main_mode == MAIN_TABLE && comb ? a.Wm : 0;
+}
+
+// one output of the combine, the splits in the reference's order j (the same
+// operations as k_sub's combine)
+template <typename T, typename Split>
+__device__ __forceinline__ T runs_combine(Split split_at, int P, const T* y, const T* m, int mode, float scale)
+{
+    constexpr int U = 8;
+    if (mode == OUT_EMA_SCALED) {
+        T dst = T(0);
+        int j = 0;
+        for (; j + U <= P; j += U) {
+            uint32_t p[U];
+#pragma unroll
+            for (int q = 0; q < U; ++q) p[q] = split_at(j + q);
+            T yy[U], mm[U];
+#pragma unroll
+            for (int q = 0; q < U; ++q) {
+                yy[q] = y[p[q] & 0xffffu];
+                mm[q] = m[p[q] >> 16];
+            }
+#pragma unroll
+            for (int q = 0; q < U; ++q) dst = (T)((double)dst + ((double)yy[q] * (double)mm[q]) * (double)scale);
+        }
+        for (; j < P; ++j) {
+            const uint32_t p = split_at(j);
+            dst = (T)((double)dst + ((double)y[p & 0xffffu] * (double)m[p >> 16]) * (double)scale);
+        }
+        return dst;
+    }
+    T acc = T(0);
+    int j = 0;
+    for (; j + U <= P; j += U) {
+        uint32_t p[U];
+#pragma unroll
+        for (int q = 0; q < U; ++q) p[q] = split_at(j + q);
+        T yy[U], mm[U];
+#pragma unroll
+        for (int q = 0; q < U; ++q) {
+            yy[q] = y[p[q] & 0xffffu];
+            mm[q] = m[p[q] >> 16];
+        }
+#pragma unroll
+        for (int q = 0; q < U; ++q) {
+            const T prod = yy[q] * mm[q];
+            acc = acc + prod;
+        }
+    }
+    for (; j < P; ++j) {
+        const uint32_t p = split_at(j);
+        const T prod = y[p & 0xffffu] * m[p >> 16];
+        acc = acc + prod;
+    }
+    return acc;
+}
+
+}  // namespace
+
+template <typename T>
+size_t runs_lds_bytes(int main_mode, const SubArgs<T>& a)
+{
+    return runs_head_bytes(a) + (size_t)(kBlock / 64) * (a.Wy + runs_main_cols(main_mode, a)) * sizeof(T);
+}
+
+template <typename T>
+size_t runs_g_lds_bytes(const SubArgs<T>& a)
+{
+    return rank_bytes(a.K) + (size_t)(kBlock / 64) * 2 * a.Wy * sizeof(T);
+}
+
+template size_t runs_g_lds_bytes<float>(const SubArgs<float>&);
+template size_t runs_g_lds_bytes<double>(const SubArgs<double>&);
+
+template size_t runs_lds_bytes<float>(int, const SubArgs<float>&);
+template size_t runs_lds_bytes<double>(int, const SubArgs<double>&);
+
+namespace {
+
+// RV: row values per lane (a gathered row has at most 64 * RV values); U:
+// neighbour rows in flight per team
+template <typename T, int RV, int U, int MODE, int MAIN>
+__global__ __launch_bounds__(kBlock) void k_runs(const SubArgs<T> a)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    constexpr int TEAM = 64;
+    constexpr int TEAMS = kBlock / TEAM;
+    const int lane = threadIdx.x % TEAM;
+    const int team = threadIdx.x / TEAM;
+    const int B = a.B;
+    const int mode = a.out_mode;
+    const int Wg = a.Wa;  // values per colouring of a gathered row
+    const int Wt = a.Wy;  // values per colouring of the summed row
+    const int nM = runs_main_cols(MAIN, a);
+    int16_t* s_rank = reinterpret_cast<int16_t*>(smem_raw);
+    uint32_t* s_split = reinterpret_cast<uint32_t*>(smem_raw + rank_bytes(a.K));
+    const bool use_split_lds = a.split_lds && MAIN == MAIN_TABLE && (mode == OUT_EMA || mode == OUT_EMA_SCALED);
+    T* sY = reinterpret_cast<T*>(smem_raw + runs_head_bytes(a)) + (size_t)team * (Wt + nM);
+    T* sM = sY + Wt;
+    for (int i = threadIdx.x; i < (1 << a.K); i += kBlock) s_rank[i] = a.rank_tab[i];
+    if (use_split_lds)
+        for (int i = threadIdx.x; i < a.P * a.Wo; i += kBlock) {
+            const int r = i / a.P;
+            s_split[(i - r * a.P) * a.Wo + r] = a.split[i];
+        }
+    // set mask of each row position this lane gathers
+    uint32_t xm[RV];
+#pragma unroll
+    for (int t = 0; t < RV; ++t) {
+        const int j = lane + TEAM * t;
+        xm[t] = j < Wg ? (uint32_t)a.xmask[j] : 0u;
+    }
+    __syncthreads();
+
+    const int64_t n_work = (int64_t)a.n_items * B;
+    for (int64_t base = (int64_t)blockIdx.x * TEAMS; base < n_work; base += (int64_t)gridDim.x * TEAMS) {
+        const int64_t work = base + team;
+        const bool active = work < n_work;
+        int v = 0, e0 = 0, e1 = 0, slot = 0, b = 0;
+        if (active) {
+            const int64_t item = work / B;
+            b = (int)(work - item * B);
+            if (a.items) {
+                const int4 it = a.items[item];
+                v = it.x; e0 = it.y; e1 = it.z; slot = it.w;
+            } else {
+                v = a.item_base + (int)item;
+                e0 = a.rowptr[v];
+                e1 = a.rowptr[v + 1];
+            }
+        }
+        for (int t = lane; t < Wt; t += TEAM) sY[t] = T(0);
+        __syncthreads();
+        if (active) {
+            const uint32_t cv = a.colors[(int64_t)v * B + b];
+            const T* rows = a.aux + (int64_t)b * Wg;
+            // colours present among the neighbours (one bit each)
+            uint32_t present = 0;
+            for (int e = e0 + lane; e < e1; e += TEAM) present |= 1u << a.ecol[(int64_t)e * B + b];
+            for (int sh = 1; sh < TEAM; sh <<= 1) present |= __shfl_xor(present, sh, TEAM);
+            present &= ~(1u << cv);  // a neighbour of the vertex's own colour adds nothing
+            while (present) {
+                const uint32_t c = (uint32_t)(__ffs(present) - 1);
+                present &= present - 1u;
+                T acc[RV];
+#pragma unroll
+                for (int t = 0; t < RV; ++t) acc[t] = T(0);
+                for (int e = e0; e < e1; e += TEAM) {
+                    const int ee = e + lane;
+                    const bool ok = ee < e1;
+                    const uint32_t cc = ok ? (uint32_t)a.ecol[(int64_t)ee * B + b] : 0xffu;
+                    const int uid = ok ? a.col[ee] : 0;
+                    const T wv = ok && a.w ? a.w[ee] : T(1);
+                    uint64_t m = __ballot(ok && cc == c);
+                    while (m) {
+                        int src[U];
+#pragma unroll
+                        for (int q = 0; q < U; ++q) {
+                            src[q] = m ? __ffsll((unsigned long long)m) - 1 : -1;
+                            m &= m ? m - 1 : 0;
+                        }
+                        T x[U][RV];
+                        T wq[U];
+#pragma unroll
+                        for (int q = 0; q < U; ++q) {
+                            // a slot past the last match reads the first match's row, weight 0
+                            const int s = src[q] >= 0 ? src[q] : src[0];
+                            const int u = __shfl(uid, s, TEAM);
+                            wq[q] = src[q] >= 0 ? __shfl(wv, s, TEAM) : T(0);
+                            const T* row = rows + (int64_t)u * a.lda;
+#pragma unroll
+                            for (int t = 0; t < RV; ++t) {
+                                const int j = lane + TEAM * t;
+                                x[q][t] = j < Wg ? row[j] : T(0);
+                            }
+                        }
+#pragma unroll
+                        for (int q = 0; q < U; ++q) {
+                            if (src[q] < 0) break;
+#pragma unroll
+                            for (int t = 0; t < RV; ++t) acc[t] = a.w ? acc[t] + wq[q] * x[q][t] : acc[t] + x[q][t];
+                        }
+                    }
+                }
+                // move the colour's sum to the vertex's row: distinct targets
+                // per lane and position, one wavefront per team (in order)
+#pragma unroll
+                for (int t = 0; t < RV; ++t) {
+                    const int j = lane + TEAM * t;
+                    if (j >= Wg) continue;
+                    int idx = -1;
+                    if (MODE == RUNS_FWD) {
+                        const uint32_t S = expand_bit_r(xm[t], c) | (1u << c);
+                        if (!((S >> cv) & 1u)) idx = s_rank[squeeze_bit_r(S, cv)];
+                    } else {
+                        const uint32_t A = expand_bit_r(xm[t], c);
+                        if ((A >> cv) & 1u) idx = s_rank[squeeze_bit_r(A & ~(1u << cv), cv)];
+                    }
+                    if (idx >= 0) sY[idx] = sY[idx] + acc[t];
+                }
+            }
+        }
+        __syncthreads();
+        if (mode == OUT_Y || mode == OUT_PARTIAL) {
+            if (active) {
+                T* dst = a.out + (int64_t)(mode == OUT_Y ? v : slot) * a.ldo + (int64_t)b * Wt;
+                for (int t = lane; t < Wt; t += TEAM) dst[t] = sY[t];
+            }
+            __syncthreads();
+            continue;
+        }
+        if (active && nM)
+            for (int t = lane; t < nM; t += TEAM) sM[t] = a.main[(int64_t)v * a.ldm + (int64_t)b * nM + t];
+        __syncthreads();
+        if (active) {
+            const int P = a.P;
+            if (mode == OUT_ROOT) {
+                if (lane == 0) {
+                    double val = 0.0;
+                    if (MAIN == MAIN_LEAF) {
+                        val = (double)sY[0];  // f32 product y * 1, widened (DataTableColMajor.cpp:396)
+                    } else {
+                        for (int j = 0; j < P; ++j) {
+                            const uint32_t p = a.split[j];
+                            const T prod = sY[p & 0xffffu] * sM[p >> 16];
+                            val = val + (double)prod;
+                        }
+                    }
+                    a.root_vals[(int64_t)b * a.n_root + v] = val;
+                }
+            } else {
+                const int Wo = a.Wo;
+                T* dst = a.out + (int64_t)v * a.ldo + (int64_t)b * Wo;
+                for (int r = lane; r < Wo; r += TEAM) {
+                    T res;
+                    if (MAIN == MAIN_LEAF) {
+                        res = mode == OUT_EMA_SCALED ? (T)((double)sY[r] * (double)a.scale) : sY[r];
+                    } else if (use_split_lds) {
+                        res = runs_combine<T>([&](int j) { return s_split[j * Wo + r]; }, P, sY, sM, mode, a.scale);
+                    } else {
+                        res = runs_combine<T>([&](int j) { return a.split[(size_t)r * P + j]; }, P, sY, sM, mode,
+                                              a.scale);
+                    }
+                    dst[r] = res;
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// The adjoint's gradient gather (RUNS_G): the forward sum of Y (written
+// out, OUT_Y or a hub chunk's OUT_PARTIAL) and, for every entry e = (v, u),
+// grad[e] += dY[v] . x_u, the neighbour's row mapped to Y positions
+// (d Y[v] / d w_e = x_u).  The FP32 dot product is summed over the team's
+// lanes by a fixed xor tree and added to the FP64 gradient with one atomic.
+// A team owns an item for every colouring, in order, so each entry's adds
+// happen in program order and the results are deterministic.
+template <typename T, int RV, int U>
+__global__ __launch_bounds__(kBlock) void k_runs_g(const SubArgs<T> a)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    constexpr int TEAM = 64;
+    constexpr int TEAMS = kBlock / TEAM;
+    const int lane = threadIdx.x % TEAM;
+    const int team = threadIdx.x / TEAM;
+    const int B = a.B;
+    const int Wg = a.Wa, Wt = a.Wy;
+    int16_t* s_rank = reinterpret_cast<int16_t*>(smem_raw);
+    T* sY = reinterpret_cast<T*>(smem_raw + rank_bytes(a.K)) + (size_t)team * 2 * Wt;
+    T* sD = sY + Wt;
+    for (int i = threadIdx.x; i < (1 << a.K); i += kBlock) s_rank[i] = a.rank_tab[i];
+    uint32_t xm[RV];
+#pragma unroll
+    for (int t = 0; t < RV; ++t) {
+        const int j = lane + TEAM * t;
+        xm[t] = j < Wg ? (uint32_t)a.xmask[j] : 0u;
+    }
+    __syncthreads();
+    for (int64_t base = (int64_t)blockIdx.x * TEAMS; base < a.n_items; base += (int64_t)gridDim.x * TEAMS) {
+        const int64_t item = base + team;
+        const bool active = item < a.n_items;
+        int v = 0, e0 = 0, e1 = 0, slot = 0;
+        if (active) {
+            if (a.items) {
+                const int4 it = a.items[item];
+                v = it.x; e0 = it.y; e1 = it.z; slot = it.w;
+            } else {
+                v = a.item_base + (int)item;
+                e0 = a.rowptr[v];
+                e1 = a.rowptr[v + 1];
+            }
+        }
+        for (int b = 0; b < B; ++b) {
+            for (int t = lane; t < Wt; t += TEAM) {
+                sY[t] = T(0);
+                sD[t] = active ? a.dy[(int64_t)v * a.ldd + (int64_t)b * Wt + t] : T(0);
+            }
+            __syncthreads();
+            if (active) {
+                const uint32_t cv = a.colors[(int64_t)v * B + b];
+                const T* rows = a.aux + (int64_t)b * Wg;
+                uint32_t present = 0;
+                for (int e = e0 + lane; e < e1; e += TEAM) present |= 1u << a.ecol[(int64_t)e * B + b];
+                for (int sh = 1; sh < TEAM; sh <<= 1) present |= __shfl_xor(present, sh, TEAM);
+                present &= ~(1u << cv);
+                while (present) {
+                    const uint32_t c = (uint32_t)(__ffs(present) - 1);
+                    present &= present - 1u;
+                    // where each of the lane's row positions lands for colour c
+                    int idx[RV];
+                    T dyc[RV];
+                    T acc[RV];
+#pragma unroll
+                    for (int t = 0; t < RV; ++t) {
+                        const int j = lane + TEAM * t;
+                        idx[t] = -1;
+                        if (j < Wg) {
+                            const uint32_t S = expand_bit_r(xm[t], c) | (1u << c);
+                            if (!((S >> cv) & 1u)) idx[t] = s_rank[squeeze_bit_r(S, cv)];
+                        }
+                        dyc[t] = idx[t] >= 0 ? sD[idx[t]] : T(0);
+                        acc[t] = T(0);
+                    }
+                    for (int e = e0; e < e1; e += TEAM) {
+                        const int ee = e + lane;
+                        const bool ok = ee < e1;
+                        const uint32_t cc = ok ? (uint32_t)a.ecol[(int64_t)ee * B + b] : 0xffu;
+                        const int uid = ok ? a.col[ee] : 0;
+                        const T wv = ok && a.w ? a.w[ee] : T(1);
+                        uint64_t m = __ballot(ok && cc == c);
+                        while (m) {
+                            int src[U];
+#pragma unroll
+                            for (int q = 0; q < U; ++q) {
+                                src[q] = m ? __ffsll((unsigned long long)m) - 1 : -1;
+                                m &= m ? m - 1 : 0;
+                            }
+                            T x[U][RV];
+                            T wq[U];
+#pragma unroll
+                            for (int q = 0; q < U; ++q) {
+                                const int s = src[q] >= 0 ? src[q] : src[0];
+                                const int u = __shfl(uid, s, TEAM);
+                                wq[q] = __shfl(wv, s, TEAM);
+                                const T* row = rows + (int64_t)u * a.lda;
+#pragma unroll
+                                for (int t = 0; t < RV; ++t) {
+                                    const int j = lane + TEAM * t;
+                                    x[q][t] = j < Wg ? row[j] : T(0);
+                                }
+                            }
+#pragma unroll
+                            for (int q = 0; q < U; ++q) {
+                                if (src[q] < 0) break;
+                                T dot = T(0);
+#pragma unroll
+                                for (int t = 0; t < RV; ++t) {
+                                    acc[t] = a.w ? acc[t] + wq[q] * x[q][t] : acc[t] + x[q][t];
+                                    dot = dot + dyc[t] * x[q][t];
+                                }
+                                for (int sh = 1; sh < TEAM; sh <<= 1) dot = dot + __shfl_xor(dot, sh, TEAM);
+                                if (lane == 0) (void)unsafeAtomicAdd(&a.grad[e + src[q]], (double)dot);
+                            }
+                        }
+                    }
+#pragma unroll
+                    for (int t = 0; t < RV; ++t)
+                        if (idx[t] >= 0) sY[idx[t]] = sY[idx[t]] + acc[t];
+                }
+            }
+            __syncthreads();
+            if (active) {
+                T* dst = a.out + (int64_t)(a.out_mode == OUT_Y ? v : slot) * a.ldo + (int64_t)b * Wt;
+                for (int t = lane; t < Wt; t += TEAM) dst[t] = sY[t];
+            }
+            __syncthreads();
+        }
+    }
+}
+
+template <typename T, int RV>
+hipError_t launch_runs_g_t(const SubArgs<T>& a, int num_cus, int blocks_per_cu, hipStream_t st)
+{
+    constexpr int U = RV <= 2 ? 8 : 4;
+    const size_t lds = rank_bytes(a.K) + (size_t)(kBlock / 64) * 2 * a.Wy * sizeof(T);
+    const int64_t need = ((int64_t)a.n_items + (kBlock / 64) - 1) / (kBlock / 64);
+    if (need <= 0) return hipSuccess;
+    static thread_local std::map<size_t, int> per_cu_cache;
+    int per_cu = blocks_per_cu;
+    if (per_cu <= 0) {
+        auto it = per_cu_cache.find(lds);
+        if (it != per_cu_cache.end()) {
+            per_cu = it->second;
+        } else {
+            hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_runs_g<T, RV, U>, kBlock, lds);
+            if (e != hipSuccess) return e;
+            if (per_cu < 1) return hipErrorInvalidConfiguration;
+            per_cu_cache[lds] = per_cu;
+        }
+    }
+    const int64_t cap = (int64_t)per_cu * num_cus;
+    const int grid = (int)(need < cap ? need : cap);
+    hipLaunchKernelGGL((k_runs_g<T, RV, U>), dim3(grid), dim3(kBlock), lds, st, a);
+    return hipGetLastError();
+}
+
+// RUNS_SEQ: the forward sum in the reference's order.  A team per (vertex,
+// colouring) reads each neighbour's row whole (coalesced), in the row's
+// column order, and adds every value into the Y value it belongs to (LDS),
+// neighbour after neighbour: each Y value is a running sum over the
+// neighbours in the reference's order, starting from zero (CSCGraph.hpp:
+// 254-307).  The target of row position j of a neighbour of colour cu is
+// xidx[(cv * k + cu) * Wa4 + j] (-1 for none).  Hubs need no partial rows: a
+// team takes a vertex's whole neighbour list.  Then it finishes as k_runs.
+template <typename T, int RV, int U, int MAIN>
+__global__ __launch_bounds__(kBlock) void k_runs_seq(const SubArgs<T> a)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    constexpr int TEAM = 64;
+    constexpr int TEAMS = kBlock / TEAM;
+    const int lane = threadIdx.x % TEAM;
+    const int team = threadIdx.x / TEAM;
+    const int B = a.B;
+    const int mode = a.out_mode;
+    const int Wg = a.Wa, Wt = a.Wy;
+    const int nM = runs_main_cols(MAIN, a);
+    uint32_t* s_split = reinterpret_cast<uint32_t*>(smem_raw + rank_bytes(a.K));
+    const bool use_split_lds = a.split_lds && MAIN == MAIN_TABLE && (mode == OUT_EMA || mode == OUT_EMA_SCALED);
+    T* sY = reinterpret_cast<T*>(smem_raw + runs_head_bytes(a)) + (size_t)team * (Wt + nM);
+    T* sM = sY + Wt;
+    if (use_split_lds)
+        for (int i = threadIdx.x; i < a.P * a.Wo; i += kBlock) {
+            const int r = i / a.P;
+            s_split[(i - r * a.P) * a.Wo + r] = a.split[i];
+        }
+    __syncthreads();
+    const int64_t n_work = (int64_t)a.n_items * B;
+    for (int64_t base = (int64_t)blockIdx.x * TEAMS; base < n_work; base += (int64_t)gridDim.x * TEAMS) {
+        const int64_t work = base + team;
+        const bool active = work < n_work;
+        int v = 0, e0 = 0, e1 = 0, b = 0;
+        if (active) {
+            const int64_t item = work / B;
+            b = (int)(work - item * B);
+            v = a.item_base + (int)item;
+            e0 = a.rowptr[v];
+            e1 = a.rowptr[v + 1];
+        }
+        for (int t = lane; t < Wt; t += TEAM) sY[t] = T(0);
+        __syncthreads();
+        if (active) {
+            const uint32_t cv = a.colors[(int64_t)v * B + b];
+            const T* rows = a.aux + (int64_t)b * Wg;
+            const int16_t* xrow = a.xidx_g + (int64_t)cv * a.k * a.Wa4;
+            for (int e = e0; e < e1; e += U) {
+                int u[U];
+                uint32_t cu[U];
+                T wq[U];
+#pragma unroll
+                for (int q = 0; q < U; ++q) {
+                    const bool ok = e + q < e1;
+                    const int ec = ok ? e + q : e;  // a slot past the end repeats the first (ignored)
+                    u[q] = a.col[ec];
+                    cu[q] = ok ? (uint32_t)a.ecol[(int64_t)ec * B + b] : cv;
+                    wq[q] = a.w ? a.w[ec] : T(1);
+                }
+                T x[U][RV];
+                int16_t id[U][RV];
+#pragma unroll
+                for (int q = 0; q < U; ++q) {
+                    const T* row = rows + (int64_t)u[q] * a.lda;
+                    const int16_t* ix = xrow + cu[q] * a.Wa4;
+#pragma unroll
+                    for (int t = 0; t < RV; ++t) {
+                        const int j = lane + TEAM * t;
+                        x[q][t] = j < Wg ? row[j] : T(0);
+                        id[q][t] = j < Wg ? ix[j] : (int16_t)-1;
+                    }
+                }
+                // in the neighbours' order: every value into its Y value
+#pragma unroll
+                for (int q = 0; q < U; ++q) {
+#pragma unroll
+                    for (int t = 0; t < RV; ++t) {
+                        const int i = id[q][t];
+                        if (i >= 0) sY[i] = a.w ? sY[i] + wq[q] * x[q][t] : sY[i] + x[q][t];
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        if (mode == OUT_Y || mode == OUT_PARTIAL) {
+            if (active) {
+                T* dst = a.out + (int64_t)v * a.ldo + (int64_t)b * Wt;
+                for (int t = lane; t < Wt; t += TEAM) dst[t] = sY[t];
+            }
+            __syncthreads();
+            continue;
+        }
+        if (active && nM)
+            for (int t = lane; t < nM; t += TEAM) sM[t] = a.main[(int64_t)v * a.ldm + (int64_t)b * nM + t];
+        __syncthreads();
+        if (active) {
+            const int P = a.P;
+            if (mode == OUT_ROOT) {
+                if (lane == 0) {
+                    double val = 0.0;
+                    if (MAIN == MAIN_LEAF) {
+                        val = (double)sY[0];
+                    } else {
+                        for (int j = 0; j < P; ++j) {
+                            const uint32_t p = a.split[j];
+                            const T prod = sY[p & 0xffffu] * sM[p >> 16];
+                            val = val + (double)prod;
+                        }
+                    }
+                    a.root_vals[(int64_t)b * a.n_root + v] = val;
+                }
+            } else {
+                const int Wo = a.Wo;
+                T* dst = a.out + (int64_t)v * a.ldo + (int64_t)b * Wo;
+                for (int r = lane; r < Wo; r += TEAM) {
+                    T res;
+                    if (MAIN == MAIN_LEAF) {
+                        res = mode == OUT_EMA_SCALED ? (T)((double)sY[r] * (double)a.scale) : sY[r];
+                    } else if (use_split_lds) {
+                        res = runs_combine<T>([&](int j) { return s_split[j * Wo + r]; }, P, sY, sM, mode, a.scale);
+                    } else {
+                        res = runs_combine<T>([&](int j) { return a.split[(size_t)r * P + j]; }, P, sY, sM, mode,
+                                              a.scale);
+                    }
+                    dst[r] = res;
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+template <typename T, int RV, int MAIN>
+hipError_t launch_runs_seq_t(const SubArgs<T>& a, int num_cus, int blocks_per_cu, hipStream_t st)
+{
+    constexpr int U = RV <= 2 ? 8 : 4;
+    const size_t lds = runs_lds_bytes<T>(MAIN, a);
+    const int64_t need = ((int64_t)a.n_items * a.B + (kBlock / 64) - 1) / (kBlock / 64);
+    if (need <= 0) return hipSuccess;
+    static thread_local std::map<size_t, int> per_cu_cache;
+    int per_cu = blocks_per_cu;
+    if (per_cu <= 0) {
+        auto it = per_cu_cache.find(lds);
+        if (it != per_cu_cache.end()) {
+            per_cu = it->second;
+        } else {
+            hipError_t e =
+                hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_runs_seq<T, RV, U, MAIN>, kBlock, lds);
+            if (e != hipSuccess) return e;
+            if (per_cu < 1) return hipErrorInvalidConfiguration;
+            per_cu_cache[lds] = per_cu;
+        }
+    }
+    const int64_t cap = (int64_t)per_cu * num_cus;
+    const int grid = (int)(need < cap ? need : cap);
+    hipLaunchKernelGGL((k_runs_seq<T, RV, U, MAIN>), dim3(grid), dim3(kBlock), lds, st, a);
+    return hipGetLastError();
+}
+
+template <typename T, int MAIN>
+hipError_t launch_runs_seq_rv(int rv, const SubArgs<T>& a, int num_cus, int blocks_per_cu, hipStream_t st)
+{
+    switch (rv) {
+    case 1: return launch_runs_seq_t<T, 1, MAIN>(a, num_cus, blocks_per_cu, st);
+    case 2: return launch_runs_seq_t<T, 2, MAIN>(a, num_cus, blocks_per_cu, st);
+    case 4: return launch_runs_seq_t<T, 4, MAIN>(a, num_cus, blocks_per_cu, st);
+    case 8: return launch_runs_seq_t<T, 8, MAIN>(a, num_cus, blocks_per_cu, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template <typename T, int RV, int MODE, int MAIN>
+hipError_t launch_runs_t(const SubArgs<T>& a, int num_cus, int blocks_per_cu, hipStream_t st)
+{
+    constexpr int U = RV <= 2 ? 8 : 4;
+    const size_t lds = runs_lds_bytes<T>(MAIN, a);
+    const int64_t need = ((int64_t)a.n_items * a.B + (kBlock / 64) - 1) / (kBlock / 64);
+    if (need <= 0) return hipSuccess;
+    static thread_local std::map<size_t, int> per_cu_cache;
+    int per_cu = blocks_per_cu;
+    if (per_cu <= 0) {
+        auto it = per_cu_cache.find(lds);
+        if (it != per_cu_cache.end()) {
+            per_cu = it->second;
+        } else {
+            hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_runs<T, RV, U, MODE, MAIN>,
+                                                                         kBlock, lds);
+            if (e != hipSuccess) return e;
+            if (per_cu < 1) return hipErrorInvalidConfiguration;
+            per_cu_cache[lds] = per_cu;
+        }
+    }
+    const int64_t cap = (int64_t)per_cu * num_cus;
+    const int grid = (int)(need < cap ? need : cap);
+    hipLaunchKernelGGL((k_runs<T, RV, U, MODE, MAIN>), dim3(grid), dim3(kBlock), lds, st, a);
+    return hipGetLastError();
+}
+
+template <typename T, int MODE, int MAIN>
+hipError_t launch_runs_rv(int rv, const SubArgs<T>& a, int num_cus, int blocks_per_cu, hipStream_t st)
+{
+    switch (rv) {
+    case 1: return launch_runs_t<T, 1, MODE, MAIN>(a, num_cus, blocks_per_cu, st);
+    case 2: return launch_runs_t<T, 2, MODE, MAIN>(a, num_cus, blocks_per_cu, st);
+    case 4: return launch_runs_t<T, 4, MODE, MAIN>(a, num_cus, blocks_per_cu, st);
+    case 8: return launch_runs_t<T, 8, MODE, MAIN>(a, num_cus, blocks_per_cu, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace
+
+int runs_values_per_lane(int wg)
+{
+    const int rv = (wg + 63) / 64;
+    return rv <= 1 ? 1 : rv <= 2 ? 2 : rv <= 4 ? 4 : rv <= 8 ? 8 : 0;
+}
+
+template <typename T>
+hipError_t launch_runs(int mode, int main_mode, const SubArgs<T>& a, int num_cus, int blocks_per_cu, hipStream_t st)
+{
+    if (a.n_items <= 0) return hipSuccess;
+    const int rv = runs_values_per_lane(a.Wa);
+    if (rv == 0 || a.K < 1 || a.K > kMaxK - 1 || a.k > 27) return hipErrorInvalidValue;
+    if (mode == RUNS_SEQ) {
+        if (a.items || !a.xidx_g || a.out_mode == OUT_PARTIAL) return hipErrorInvalidValue;
+        return main_mode == MAIN_LEAF ? launch_runs_seq_rv<T, MAIN_LEAF>(rv, a, num_cus, blocks_per_cu, st)
+                                      : launch_runs_seq_rv<T, MAIN_TABLE>(rv, a, num_cus, blocks_per_cu, st);
+    }
+    if (mode == RUNS_G) {
+        if (main_mode != MAIN_LEAF || (a.out_mode != OUT_Y && a.out_mode != OUT_PARTIAL) || !a.dy || !a.grad)
+            return hipErrorInvalidValue;
+        switch (rv) {
+        case 1: return launch_runs_g_t<T, 1>(a, num_cus, blocks_per_cu, st);
+        case 2: return launch_runs_g_t<T, 2>(a, num_cus, blocks_per_cu, st);
+        case 4: return launch_runs_g_t<T, 4>(a, num_cus, blocks_per_cu, st);
+        case 8: return launch_runs_g_t<T, 8>(a, num_cus, blocks_per_cu, st);
+        default: return hipErrorInvalidValue;
+        }
+    }
+    if (mode == RUNS_T) {
+        if (main_mode != MAIN_LEAF || (a.out_mode != OUT_Y && a.out_mode != OUT_PARTIAL)) return hipErrorInvalidValue;
+        return launch_runs_rv<T, RUNS_T, MAIN_LEAF>(rv, a, num_cus, blocks_per_cu, st);
+    }
+    if (mode != RUNS_FWD) return hipErrorInvalidValue;
+    return main_mode == MAIN_LEAF ? launch_runs_rv<T, RUNS_FWD, MAIN_LEAF>(rv, a, num_cus, blocks_per_cu, st)
+                                  : launch_runs_rv<T, RUNS_FWD, MAIN_TABLE>(rv, a, num_cus, blocks_per_cu, st);
+}
+
+template hipError_t launch_runs<float>(int, int, const SubArgs<float>&, int, int, hipStream_t);
+template hipError_t launch_runs<double>(int, int, const SubArgs<double>&, int, int, hipStream_t);
+
+}  // namespace pgbsc

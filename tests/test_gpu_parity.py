@@ -130,8 +130,10 @@ def test_kernel_variants_match_oracle(gpu_ctx, aux_rows, stage_rows, reg_cols, B
     one or several colorings per chunk), against the oracle: FP64 exact,
     FP32 within the tolerance; the hub split path included."""
     rng = np.random.default_rng(500 + 10 * B + 3 * aux_rows + stage_rows + reg_cols)
-    keys = ("aux_rows", "stage_rows", "reg_cols", "hub_degree", "hub_chunk", "bucket_high", "bucket_mid")
+    keys = ("aux_rows", "stage_rows", "reg_cols", "hub_degree", "hub_chunk", "bucket_high", "bucket_mid",
+            "exact_order")
     saved = [gpu_ctx.get_option(k) for k in keys]
+    gpu_ctx.set_option("exact_order", 0)   # the variants of the fast (reordered) neighbour sums
     gpu_ctx.set_option("bucket_high", 32)  # all three team shapes on these small graphs
     gpu_ctx.set_option("bucket_mid", 8)
     gpu_ctx.set_option("aux_rows", aux_rows)
@@ -502,14 +504,15 @@ def test_bucket_shapes_match_oracle(gpu_ctx, hub_degree, bucket_high, bucket_mid
     """The degree buckets the autotuner may choose, including orderings that
     leave a bucket empty, on a power-law graph with hubs: FP64 exact, FP32
     within the tolerance, one and several colorings per pass."""
-    keys = ("hub_degree", "bucket_high", "bucket_mid", "hub_chunk")
+    keys = ("hub_degree", "bucket_high", "bucket_mid", "hub_chunk", "exact_order")
     saved = [gpu_ctx.get_option(k) for k in keys]
+    gpu_ctx.set_option("exact_order", 0)   # bucket shapes of the fast neighbour sums
     s, d = sg.rmat_edges(12, 16, seed=7)
     n = 1 << 12
     colptr, rowidx = sg.csc_from_edges(n, s, d)
     assert np.diff(colptr).max() > 1024
     try:
-        for key, v in zip(keys, (hub_degree, bucket_high, bucket_mid, hub_chunk)):
+        for key, v in zip(keys[:4], (hub_degree, bucket_high, bucket_mid, hub_chunk)):
             gpu_ctx.set_option(key, v)
         gpu_ctx.upload_graph(colptr, rowidx)
         rng = np.random.default_rng(hub_degree + bucket_high + bucket_mid)
@@ -541,8 +544,9 @@ def test_64_colorings_per_pass_match_oracle(gpu_ctx, aux_rows, stage_rows):
     s, d = sg.rmat_edges(13, 8, seed=4)
     n, s, d = sg.remove_holes(1 << 13, s, d)
     colptr, rowidx = sg.csc_from_edges(n, s, d)
-    keys = ("aux_rows", "stage_rows", "batch_tile")
+    keys = ("aux_rows", "stage_rows", "batch_tile", "exact_order")
     saved = [gpu_ctx.get_option(k) for k in keys]
+    gpu_ctx.set_option("exact_order", 0 if aux_rows or stage_rows else 1)
     gpu_ctx.upload_graph(colptr, rowidx)
     gpu_ctx.set_option("aux_rows", aux_rows)
     gpu_ctx.set_option("stage_rows", stage_rows)
@@ -555,8 +559,10 @@ def test_64_colorings_per_pass_match_oracle(gpu_ctx, aux_rows, stage_rows):
             cols = sg.colors_fill(77, 0, 64, n, k)
             g32 = gpu_ctx.count(plan, 64, colors=cols, precision="fp32")
             assert gpu_ctx.get_option("last_tile") == 64
+            # FP64 rows are twice as wide: the widest pass that fits LDS
+            gpu_ctx.set_option("batch_tile", 0)
             g64 = gpu_ctx.count(plan, 64, colors=cols, precision="fp64")
-            assert gpu_ctx.get_option("last_tile") == 64
+            gpu_ctx.set_option("batch_tile", 64)
             worst = 0.0
             for i in range(64):
                 o32 = oracle.count_f32(op, colptr, rowidx, cols[i])
@@ -564,6 +570,50 @@ def test_64_colorings_per_pass_match_oracle(gpu_ctx, aux_rows, stage_rows):
                 assert _rel(g32[i], o32) <= TOL32, (k, i, g32[i], o32)
                 assert _same64(g64[i], oracle.count_f64(op, colptr, rowidx, cols[i])), (k, i)
             print(f"[tile64] k={k}: worst FP32 relative error {worst:.3e}", flush=True)
+    finally:
+        for key, v in zip(keys, saved):
+            gpu_ctx.set_option(key, v)
+
+
+EXACT_TEMPLATES = [(5, [(0, 1), (1, 2), (2, 3), (3, 4)]), (7, TREE7),
+                   (10, [(0, 1), (1, 2), (2, 3), (3, 4), (1, 5), (5, 6), (5, 7), (7, 8), (7, 9)]),
+                   (12, [(0, 1), (1, 2), (2, 3), (3, 4), (4, 5), (2, 6), (6, 7), (7, 8), (6, 9), (9, 10), (10, 11)])]
+
+
+@pytest.mark.parametrize("hub_degree,color_runs", [(1 << 30, -1), (16, -1), (16, 0), (16, 1)])
+def test_exact_order_is_the_reference_arithmetic(gpu_ctx, hub_degree, color_runs):
+    """Option exact_order (the default): every FP32 operation of the DP is the
+    reference's, in its order (neighbour sums one after another in column
+    order, splits in IndexSys order, the scaled step in FP64), so the counts
+    equal the oracle's bit-exact restatement of the reference up to the order
+    of the final FP64 sum over vertices (1e-12 relative here); hubs (split by
+    columns when hub_degree is small) and one or several colorings per pass,
+    with the neighbour sums gathered per lane (color_runs 0), by whole rows
+    (color_runs 1: every row of up to 512 values) or as chosen by width."""
+    keys = ("hub_degree", "exact_order", "color_runs")
+    saved = [gpu_ctx.get_option(k) for k in keys]
+    gpu_ctx.set_option("exact_order", 1)
+    gpu_ctx.set_option("hub_degree", hub_degree)
+    gpu_ctx.set_option("color_runs", color_runs)
+    rng = np.random.default_rng(91 + (hub_degree & 0xff))
+    try:
+        s, d = sg.rmat_edges(12, 16, seed=7)
+        n = 1 << 12
+        colptr, rowidx = sg.csc_from_edges(n, s, d)
+        gpu_ctx.upload_graph(colptr, rowidx)
+        worst = 0.0
+        for k, te in EXACT_TEMPLATES:
+            plan = gpu_ctx.plan(k, te)
+            op = oracle.Plan(k, te)
+            for B in (1, 5):
+                print(f"[exact] hub_degree={hub_degree} color_runs={color_runs} k={k} B={B}", flush=True)
+                cols = rng.integers(0, k, size=(B, n)).astype(np.uint8)
+                g32 = gpu_ctx.count(plan, B, colors=cols, precision="fp32")
+                for i in range(B):
+                    o32 = oracle.count_f32(op, colptr, rowidx, cols[i])
+                    worst = max(worst, _rel(g32[i], o32))
+                    assert _rel(g32[i], o32) <= 1e-12, (k, B, i, g32[i], o32)
+        print(f"[exact] worst relative difference to the oracle: {worst:.3e}", flush=True)
     finally:
         for key, v in zip(keys, saved):
             gpu_ctx.set_option(key, v)
