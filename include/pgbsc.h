@@ -190,6 +190,24 @@ void pg_colors_fill(uint64_t seed, int64_t first_coloring, int ncolorings, int32
 int pg_count(pg_ctx* ctx, const pg_plan* plan, int ncolorings, const uint8_t* colors,
              uint64_t seed, int precision, uint32_t flags, double* raw_out);
 
+/* No reference counterpart: pg_count for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Same arguments and results as pg_count (PG_FLAG_UNPRUNED
+ * excluded), but every buffer of a pass lives in a static device array of
+ * 64 MiB that the library holds on each device it runs on until the process
+ * ends: the call makes no hipMalloc, hipFree or other allocation, captures
+ * no graph and records no event.  A graph and template whose one-coloring
+ * pass (plus the caller's colors) needs more than 64 MiB are rejected with
+ * PG_ERR_RANGE; up to 64 colorings per pass are used when they fit.  The
+ * graph upload and plan creation that precede it allocate as usual, and the
+ * degree-bucket options must not have changed since the last allocating call
+ * (PG_ERR_STATE).  Every call on a device shares that device's array, so
+ * calls that overlap on one GPU (from several contexts or threads) race;
+ * each call writes every buffer it reads, so nothing depends on what an
+ * earlier call left there. */
+int pg_count_noalloc(pg_ctx* ctx, const pg_plan* plan, int ncolorings, const uint8_t* colors,
+                     uint64_t seed, int precision, uint32_t flags, double* raw_out);
+
 /* Mixed batch: plans[p] is counted for ncolorings[p] colorings drawn on the
  * device from seeds[p]; raw_out is the concatenation of the per-plan results,
  * equal to what pg_count gives each plan with the same seed.  With graphs on

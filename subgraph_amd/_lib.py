@@ -98,6 +98,8 @@ def lib() -> C.CDLL:
         "pg_last_timing_kernel": (C.c_int, [_vp, C.c_char_p, C.c_int]),
         "pg_last_timing_entry": (C.c_int, [_vp, C.c_int, C.c_char_p, C.c_int, C.POINTER(C.c_double),
                                            C.POINTER(C.c_int64), C.POINTER(C.c_double)]),
+        "pg_count_noalloc": (C.c_int, [_vp, _vp, C.c_int, _vp, C.c_uint64, C.c_int, C.c_uint32,
+                                       C.POINTER(C.c_double)]),
         "pg_last_timing_sub": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int), C.c_char_p, C.c_int,
                                          C.POINTER(C.c_double), C.POINTER(C.c_int64), C.POINTER(C.c_double)]),
         "pg_rmat_edges": (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_uint64,
@@ -125,7 +127,7 @@ def exported_symbols() -> list[str]:
     """Names the C ABI header declares (checked against the shared object)."""
     return ["pg_create", "pg_destroy", "pg_last_error", "pg_set_option", "pg_get_option",
             "pg_graph_upload", "pg_graph_set_weights", "pg_graph_stats", "pg_plan_create",
-            "pg_plan_from_edges", "pg_plan_destroy", "pg_plan_info", "pg_colors_fill", "pg_count",
+            "pg_plan_from_edges", "pg_plan_destroy", "pg_plan_info", "pg_colors_fill", "pg_count", "pg_count_noalloc",
             "pg_count_multi", "pg_count_grad", "pg_last_timing", "pg_last_timing_kernel",
             "pg_last_timing_entry", "pg_last_timing_sub", "pg_rmat_edges", "pg_csc_from_edges", "pg_edges_remove_holes",
             "pg_template_sizes", "pg_template_tables", "pg_estimate_model", "pg_plan_footprint",

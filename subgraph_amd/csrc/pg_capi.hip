@@ -15,6 +15,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <tuple>
@@ -176,6 +177,10 @@ struct pg_ctx {
     DevBuf d_hub_iota;  // [n_hub+1] h -> h: one partial row per hub (column-split hub launches)
     // per-call workspace
     DevBuf d_arena, d_colors_in;
+    // pg_count_noalloc: the device address of this device's static scratch
+    // array, and (during such a call) the base the pass buffers are placed at
+    char* scratch = nullptr;
+    char* arena_override = nullptr;
     // captured passes, and what invalidates them
     bool use_graphs = true;
     uint64_t generation = 0;
@@ -856,8 +861,9 @@ struct PassBufs {
     double* tmp;
 };
 
-// the device memory a pass's buffers are placed in
-char* arena_base(const pg_ctx* c) { return c->d_arena.as<char>(); }
+// the device memory a pass's buffers are placed in: the context's arena, or
+// the static scratch during pg_count_noalloc
+char* arena_base(const pg_ctx* c) { return c->arena_override ? c->arena_override : c->d_arena.as<char>(); }
 
 template <typename T>
 PassBufs<T> pass_bufs(pg_ctx* c, const MemPlan& mp)
@@ -2230,6 +2236,39 @@ std::vector<uint16_t> masks_of_size(int K, int size)
 // C ABI
 // =========================================================================
 
+// ------------------------------------------------------------ static scratch
+//
+// pg_count_noalloc places a pass's buffers (colors, neighbour colors, the
+// histogram, count tables, hub partial rows, root values, results) in this
+// array instead of the context's arena.  Each device holds its own copy for
+// the life of the process.  Host code must not pass `scratch` itself to a
+// kernel (in host code the name is a stand-in, not a device address): the
+// device address comes from hipGetSymbolAddress, taken once per device with
+// that device current (scratch_setup, from pg_create) and kept per device.
+constexpr size_t kScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+char* g_scratch_dev[kMaxDevices] = {};
+
+// with c->device current
+hipError_t scratch_setup(pg_ctx* c)
+{
+    if (c->device < 0 || c->device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[c->device]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_dev[c->device] = static_cast<char*>(ptr);
+    }
+    c->scratch = g_scratch_dev[c->device];
+    return hipSuccess;
+}
+}  // namespace
+
 extern "C" int pg_create(pg_ctx** out, int device)
 {
     if (!out) return PG_ERR_INVALID;
@@ -2245,6 +2284,11 @@ extern "C" int pg_create(pg_ctx** out, int device)
         c->num_cus <= 0)
         c->num_cus = 256;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return PG_ERR_HIP;
+    }
+    if (scratch_setup(c) != hipSuccess) {
+        (void)hipStreamDestroy(c->stream);
         delete c;
         return PG_ERR_HIP;
     }
@@ -3027,6 +3071,100 @@ extern "C" int pg_count(pg_ctx* c, const pg_plan* p, int ncol, const uint8_t* co
     }
     return precision == PG_FP32 ? count_impl<float>(c, p, ncol, colors, seed, flags, raw_out)
                                 : count_impl<double>(c, p, ncol, colors, seed, flags, raw_out);
+}
+
+// pg_count's passes with every buffer in the static scratch: eager launches
+// (no captured graphs, no events), no device or pinned allocation.
+template <typename T>
+int count_noalloc_impl(pg_ctx* c, const pg_plan* p, int ncol, const uint8_t* colors, uint64_t seed, uint32_t flags,
+                       double* raw_out)
+{
+    if (c->built_deg != c->hub_degree || c->built_chunk != c->hub_chunk || c->built_high != c->bucket_high ||
+        c->built_mid != c->bucket_mid)
+        return set_err(c, PG_ERR_STATE,
+                       "degree-bucket options changed since the graph upload; their work lists are built by the next "
+                       "allocating call (pg_count), not by pg_count_noalloc");
+    const size_t colors_bytes = colors ? align_up((size_t)ncol * c->n, 256) : 0;
+    const int cap = std::min(64, c->batch_tile > 0 ? (int)std::min<int64_t>(c->batch_tile, ncol) : ncol);
+    int tile = 0;
+    MemPlan mp;
+    size_t need1 = 0;
+    for (int B = cap; B >= 1; --B) {
+        size_t lds = 0;
+        for (auto& d : p->subs) lds = std::max(lds, sub_lds_max(c, p, *d, B, sizeof(T)));
+        if (lds > kLdsLimit) continue;
+        const MemPlan m = plan_memory(c, p, B, sizeof(T));
+        if (B == 1) need1 = m.total + colors_bytes;
+        if (m.total + colors_bytes <= kScratchBytes) {
+            tile = B;
+            mp = m;
+            break;
+        }
+    }
+    if (tile == 0)
+        return set_err(c, PG_ERR_RANGE,
+                       "pg_count_noalloc: one coloring of this graph and template needs %zu bytes of device "
+                       "scratch (with the caller's colors), beyond the static %zu",
+                       need1, kScratchBytes);
+    const hipStream_t st = c->stream;
+    const uint8_t* d_in = nullptr;
+    if (colors) {
+        // pageable source: the copy is complete when the call returns
+        HIP_TRY(c, hipMemcpyAsync(c->scratch + mp.total, colors, (size_t)ncol * c->n, hipMemcpyHostToDevice, st));
+        d_in = reinterpret_cast<const uint8_t*>(c->scratch + mp.total);
+    }
+    struct Override {
+        pg_ctx* c;
+        int64_t timing;
+        Override(pg_ctx* ctx) : c(ctx), timing(ctx->timing)
+        {
+            c->arena_override = c->scratch;
+            c->timing = 0;
+        }
+        ~Override()
+        {
+            c->arena_override = nullptr;
+            c->timing = timing;
+        }
+    } guard(c);
+    c->last_tile = tile;
+    double raw[64];
+    for (int first = 0; first < ncol; first += tile) {
+        const int B = std::min(tile, ncol - first);
+        const MemPlan m = B == tile ? mp : plan_memory(c, p, B, sizeof(T));
+        const PassParams hp{seed, first};
+        HIP_TRY(c, hipMemcpyAsync(c->scratch + m.off_params, &hp, sizeof hp, hipMemcpyHostToDevice, st));
+        int rc = enqueue_pass<T>(c, p, B, d_in, m);
+        if (rc) return rc;
+        HIP_TRY(c, hipMemcpyAsync(raw, c->scratch + m.off_rawdev, sizeof(double) * B, hipMemcpyDeviceToHost, st));
+        HIP_TRY(c, hipStreamSynchronize(st));
+        for (int b = 0; b < B; ++b) raw_out[first + b] = finish_raw(p, sizeof(T) == 4, flags, raw[b]);
+    }
+    return PG_OK;
+}
+
+extern "C" int pg_count_noalloc(pg_ctx* c, const pg_plan* p, int ncol, const uint8_t* colors, uint64_t seed,
+                                int precision, uint32_t flags, double* raw_out)
+{
+    if (!c) return PG_ERR_INVALID;
+    if (!p || p->ctx != c || ncol <= 0 || !raw_out) return set_err(c, PG_ERR_INVALID, "invalid count arguments");
+    if (precision != PG_FP32 && precision != PG_FP64)
+        return set_err(c, PG_ERR_INVALID, "precision must be PG_FP32 or PG_FP64");
+    if (flags & ~PG_FLAG_FIX_2VERTEX) return set_err(c, PG_ERR_INVALID, "pg_count_noalloc takes PG_FLAG_FIX_2VERTEX only");
+    if (!c->has_graph) return set_err(c, PG_ERR_STATE, "no graph uploaded");
+    if (!c->scratch) return set_err(c, PG_ERR_STATE, "no static scratch on this device");
+    if (colors) {
+        const size_t total = (size_t)ncol * c->n;
+        for (size_t i = 0; i < total; ++i)
+            if (colors[i] >= p->k) return set_err(c, PG_ERR_INVALID, "color %d >= k at %zu", colors[i], i);
+    }
+    HIP_TRY(c, hipSetDevice(c->device));
+    if (c->n_act == 0) {
+        for (int i = 0; i < ncol; ++i) raw_out[i] = 0.0;
+        return PG_OK;
+    }
+    return precision == PG_FP32 ? count_noalloc_impl<float>(c, p, ncol, colors, seed, flags, raw_out)
+                                : count_noalloc_impl<double>(c, p, ncol, colors, seed, flags, raw_out);
 }
 
 extern "C" int pg_count_multi(pg_ctx* c, int nplans, const pg_plan* const* plans, const int32_t* ncolorings,
