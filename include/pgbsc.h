@@ -87,6 +87,14 @@ const char* pg_last_error(const pg_ctx* ctx);
  *                  (the root), else four (default); 1 or 4 = that for every gather
  *   "aux_rows"     neighbour sums row-wise with LDS atomics: 1 always, 0 never (default),
  *                  -1 when most of a neighbour row is used
+ *   "exact_order"  1 = the forward neighbour sums in the reference's order, so FP32 counts
+ *                  reproduce the reference's rounding (default); 0 = reordered sums (hub
+ *                  chunks, neighbour groups, colour runs), faster, as accurate, but a
+ *                  different rounding
+ *   "hub_seq"      exact order: 1 = hub rows by the batched hub kernel (default), 0 = by
+ *                  column parts of the gathering kernel (same results)
+ *   "color_runs"   whole-row neighbour sums: 1 always (rows up to 512 values), 0 never,
+ *                  -1 by row width and used share (default)
  *   "vertex_order" internal vertex order set at the next pg_graph_upload: 0 = descending
  *                  degree (default); 1 = degree classes aligned with the bucket thresholds,
  *                  breadth-first within a class (one-coloring passes of smaller graphs gain

@@ -155,6 +155,7 @@ struct pg_ctx {
     int64_t split_lds = 1;      // combines read the split table from LDS (1) when it fits, or from memory (0)
     int64_t color_runs = -1;    // wide-row neighbour sums by colour runs: 1 always, 0 never, -1 rows of 32+ values
     int64_t exact_order = 1;    // forward neighbour sums in the reference's order (one group, hubs split by columns)
+    int64_t hub_seq = 1;        // exact order: hub rows by k_hub_seq (1) or by column parts of k_sub (0)
     int64_t reg_cols = 0;       // values per lane of gathers: 0 one for rows under kRegCols, else 1 or 4
     int64_t adjoint_recompute = -1;  // adjoint tables: 0 kept, 1 recomputed, -1 recomputed when that fits 4x more
     int64_t vertex_order = 0;   // internal order at the next upload: 0 descending degree, 1 degree classes + BFS
@@ -478,25 +479,26 @@ size_t sub_lds_max_t(const pg_ctx* c, const pg_plan* p, const DevSub& d, int B, 
         return m;
     }
     const int rc = rc_for(c, aux, B, d.Wy);
-    if (c->exact_order && aux == AUX_GATHER && d.xidx_g.p &&
-        runs_seq_for(c, d.Wa, runs_used_per_mille(p, d, AUX_GATHER))) {
-        // RUNS_SEQ: one launch of teams per (vertex, colouring), LDS independent of B
-        SubArgs<T> a{};
-        a.B = B;
-        a.K = p->K;
-        a.Wy = d.Wy;
-        a.Wm = d.Wm;
-        a.Wo = d.Wo;
-        a.P = d.P;
-        a.split_lds = split;
-        a.out_mode = out;
-        return runs_lds_bytes<T>(mm, a);
-    }
     if (c->exact_order && aux == AUX_GATHER) {
-        // launch_gathering_exact: teams of max(8, cl) lanes, one neighbour group
+        // launch_gathering_exact: hub rows (k_hub_seq, or column parts of
+        // max(8, cl)-lane teams), the other vertices by whole rows (RUNS_SEQ,
+        // LDS independent of B) or by teams of max(8, cl) lanes, one group
         const int cl = std::max(8, cl_of(aux, B, p->k, d.Wy, d.Wa, rc));
-        const int sp = rc == kRegCols ? stage_pad(c, aux, d.gidx.p != nullptr, B, d.Wy, d.Wa, cl, sizeof(T)) : 0;
+        m = std::max(m, hub_seq_lds_bytes(p->K));
         m = std::max(m, lds_of<T>(cl, aux, mm, 1, OUT_PARTIAL, B, p, d, 0, split));
+        if (d.xidx_g.p && runs_seq_for(c, d.Wa, runs_used_per_mille(p, d, AUX_GATHER))) {
+            SubArgs<T> a{};
+            a.B = B;
+            a.K = p->K;
+            a.Wy = d.Wy;
+            a.Wm = d.Wm;
+            a.Wo = d.Wo;
+            a.P = d.P;
+            a.split_lds = split;
+            a.out_mode = out;
+            return std::max(m, runs_lds_bytes<T>(mm, a));
+        }
+        const int sp = rc == kRegCols ? stage_pad(c, aux, d.gidx.p != nullptr, B, d.Wy, d.Wa, cl, sizeof(T)) : 0;
         return std::max(m, lds_of<T>(cl, sp > 0 ? AUX_STAGED : aux, mm, 1, out, B, p, d, sp, split));
     }
     const int cl = cl_of(aux, B, p->k, d.Wy, d.Wa, rc);
@@ -694,7 +696,8 @@ int launch_gathering_runs(pg_ctx* c, int rmode, int main_mode, SubArgs<T> a, T* 
 }
 
 template <typename T>
-int launch_gathering_exact(pg_ctx* c, int aux, int main_mode, SubArgs<T> a, T* part, int y_cols, StepBytes bytes);
+int launch_gathering_exact(pg_ctx* c, int aux, int main_mode, SubArgs<T> a, T* part, int y_cols, StepBytes bytes,
+                           bool rows);
 
 template <typename T>
 int launch_gathering(pg_ctx* c, int aux, int main_mode, SubArgs<T> a, T* part, int y_cols, StepBytes bytes,
@@ -704,21 +707,9 @@ int launch_gathering(pg_ctx* c, int aux, int main_mode, SubArgs<T> a, T* part, i
         const int rmode = runs_mode_for(c, aux, a.Wa, used_per_mille);
         if (rmode >= 0) return launch_gathering_runs<T>(c, rmode, main_mode, a, part, y_cols, bytes);
     }
-    if (c->exact_order && (aux == AUX_GATHER || aux == AUX_STAGED)) {
-        if (a.xidx_g && runs_seq_for(c, a.Wa, used_per_mille)) {
-            // every vertex (hubs included) on a team with its whole neighbour list
-            SubArgs<T> r = a;
-            r.items = nullptr;
-            r.item_base = 0;
-            r.n_items = c->n_act;
-            record(c, runs_name(sizeof(T), RUNS_SEQ, main_mode), bytes.csc + bytes.per_entry + bytes.per_vertex,
-                   true);
-            HIP_TRY(c, launch_runs<T>(RUNS_SEQ, main_mode, r, c->num_cus, c->blocks_per_cu, c->stream));
-            record(c, "", 0, false);
-            return PG_OK;
-        }
-        return launch_gathering_exact<T>(c, AUX_GATHER, main_mode, a, part, y_cols, bytes);
-    }
+    if (c->exact_order && (aux == AUX_GATHER || aux == AUX_STAGED))
+        return launch_gathering_exact<T>(c, AUX_GATHER, main_mode, a, part, y_cols, bytes,
+                                         a.xidx_g && runs_seq_for(c, a.Wa, used_per_mille));
     const int rc = rc_for(c, aux, a.B, a.Wy);
     const int cl = cl_of(aux, a.B, a.k, a.Wy, a.Wa, rc);
     a.segpad = rc == kRegCols ? stage_pad(c, aux, a.gidx != nullptr, a.B, a.Wy, a.Wa, cl, sizeof(T)) : 0;
@@ -779,13 +770,19 @@ int launch_gathering(pg_ctx* c, int aux, int main_mode, SubArgs<T> a, T* part, i
 
 // The forward neighbour sums in the reference's order (CSCGraph.hpp:254-307:
 // every Y value adds its neighbours' values one after another, in ascending
-// column order, starting from zero): one neighbour group per team, so a lane
-// adds its values neighbour after neighbour; hubs are split by columns, not
-// by neighbours (a team per chunk of slots, with the whole neighbour list),
-// into one partial row per hub that the finishing launch then combines.
-// The FP32 sums are then the reference's, bit for bit.
+// column order, starting from zero), so the FP32 sums are the reference's,
+// bit for bit:
+//  - hubs: one partial row per hub, each value one running sum over the whole
+//    neighbour list, by k_hub_seq (a wavefront per 64 values with 32 row
+//    loads in flight per lane) or, with option hub_seq 0, by column parts of
+//    the gathering kernel (a team per chunk of slots);
+//  - the other vertices: a team with the whole neighbour list, one
+//    neighbour group, by whole rows (RUNS_SEQ, when `rows`) or by gathers;
+//  - the finishing launch adds each hub's single partial row to zero and
+//    combines it with the main table.
 template <typename T>
-int launch_gathering_exact(pg_ctx* c, int aux, int main_mode, SubArgs<T> a, T* part, int y_cols, StepBytes bytes)
+int launch_gathering_exact(pg_ctx* c, int aux, int main_mode, SubArgs<T> a, T* part, int y_cols, StepBytes bytes,
+                           bool rows)
 {
     const int rc = rc_for(c, aux, a.B, a.Wy);
     const int nslots = slots_for(a.B, a.Wy, rc);
@@ -804,20 +801,27 @@ int launch_gathering_exact(pg_ctx* c, int aux, int main_mode, SubArgs<T> a, T* p
     };
     a.ng = 1;
     if (c->n_hub > 0) {
-        // a part per chunk of cl slots: the same per-entry work as a
-        // non-hub vertex's team, whose chunks run one after another
-        const int team = cl;
         SubArgs<T> h = a;
         h.segpad = 0;
         h.out = part;
         h.ldo = y_cols;
         h.out_mode = OUT_PARTIAL;
         h.items = nullptr;
-        h.col_parts = (nslots + team - 1) / team;
-        h.n_items = c->n_hub * h.col_parts;
         h.item_base = 0;
-        record(c, kname<T>(team, aux, main_mode, rc), share(c->h_rowptr[c->n_hub], 0), true);
-        HIP_TRY(c, launch_sub<T>(team, aux, main_mode, rc, h, cus, c->blocks_per_cu, st));
+        if (c->hub_seq) {
+            h.n_items = c->n_hub;
+            record(c, sizeof(T) == 4 ? "k_hub_seq<float>" : "k_hub_seq<double>", share(c->h_rowptr[c->n_hub], 0),
+                   true);
+            HIP_TRY(c, launch_hub_seq<T>(h, cus, c->blocks_per_cu, st));
+        } else {
+            // a part per chunk of cl slots: the same per-entry work as a
+            // non-hub vertex's team, whose chunks run one after another
+            const int team = cl;
+            h.col_parts = (nslots + team - 1) / team;
+            h.n_items = c->n_hub * h.col_parts;
+            record(c, kname<T>(team, aux, main_mode, rc), share(c->h_rowptr[c->n_hub], 0), true);
+            HIP_TRY(c, launch_sub<T>(team, aux, main_mode, rc, h, cus, c->blocks_per_cu, st));
+        }
         record(c, "", 0, false);
     }
     if (c->n_hub < c->n_act) {
@@ -825,9 +829,14 @@ int launch_gathering_exact(pg_ctx* c, int aux, int main_mode, SubArgs<T> a, T* p
         r.items = nullptr;
         r.item_base = c->n_hub;
         r.n_items = c->n_act - c->n_hub;
-        record(c, kname<T>(cl, aux_b, main_mode, rc),
-               share((double)c->nnz - c->h_rowptr[c->n_hub], c->n_act - c->n_hub), true);
-        HIP_TRY(c, launch_sub<T>(cl, aux_b, main_mode, rc, r, cus, c->blocks_per_cu, st));
+        const double sh = share((double)c->nnz - c->h_rowptr[c->n_hub], c->n_act - c->n_hub);
+        if (rows) {
+            record(c, runs_name(sizeof(T), RUNS_SEQ, main_mode), sh, true);
+            HIP_TRY(c, launch_runs<T>(RUNS_SEQ, main_mode, r, cus, c->blocks_per_cu, st));
+        } else {
+            record(c, kname<T>(cl, aux_b, main_mode, rc), sh, true);
+            HIP_TRY(c, launch_sub<T>(cl, aux_b, main_mode, rc, r, cus, c->blocks_per_cu, st));
+        }
         record(c, "", 0, false);
     }
     if (c->n_hub > 0) {
@@ -2370,6 +2379,12 @@ extern "C" int pg_set_option(pg_ctx* c, const char* key, int64_t v)
             drop_graphs(c);
             c->exact_order = v;
         }
+    } else if (k == "hub_seq") {
+        if (v != 0 && v != 1) return set_err(c, PG_ERR_INVALID, "hub_seq must be 0 or 1");
+        if (v != c->hub_seq) {
+            drop_graphs(c);
+            c->hub_seq = v;
+        }
     } else if (k == "color_runs") {
         if (v < -1 || v > 1) return set_err(c, PG_ERR_INVALID, "color_runs must be -1, 0 or 1");
         if (v != c->color_runs) {
@@ -2412,6 +2427,7 @@ extern "C" int pg_get_option(const pg_ctx* c, const char* key, int64_t* v)
     else if (k == "split_lds") *v = c->split_lds;
     else if (k == "color_runs") *v = c->color_runs;
     else if (k == "exact_order") *v = c->exact_order;
+    else if (k == "hub_seq") *v = c->hub_seq;
     else if (k == "reg_cols") *v = c->reg_cols;
     else if (k == "adjoint_recompute") *v = c->adjoint_recompute;
     else if (k == "vertex_order") *v = c->vertex_order;

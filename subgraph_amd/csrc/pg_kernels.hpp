@@ -162,6 +162,15 @@ size_t runs_g_lds_bytes(const SubArgs<T>& a);
 template <typename T>
 hipError_t launch_runs(int mode, int main_mode, const SubArgs<T>& a, int num_cus, int blocks_per_cu, hipStream_t st);
 
+// Hub rows of the forward sum in the reference's order (GATHER, OUT_PARTIAL
+// into out[h][B*Wy] for hubs item_base + h, h < n_items): a wavefront per
+// (hub, coloring, 64 Y values) walks the whole neighbour list with kHubBatch
+// row loads in flight per lane.  LDS: the rank table.
+constexpr int kHubBatch = 32;
+size_t hub_seq_lds_bytes(int K);
+template <typename T>
+hipError_t launch_hub_seq(const SubArgs<T>& a, int num_cus, int blocks_per_cu, hipStream_t st);
+
 // Per-pass values read from device memory, so that one captured graph serves
 // every pass of the same shape (the host writes them before each replay).
 struct PassParams {
