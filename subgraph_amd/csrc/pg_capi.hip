@@ -156,6 +156,7 @@ struct pg_ctx {
     int64_t color_runs = -1;    // wide-row neighbour sums by colour runs: 1 always, 0 never, -1 rows of 32+ values
     int64_t exact_order = 1;    // forward neighbour sums in the reference's order (one group, hubs split by columns)
     int64_t hub_seq = 1;        // exact order: hub rows by k_hub_seq (1) or by column parts of k_sub (0)
+    int64_t fuse_grad = 1;      // adjoint: forward Y kept and the gradient taken in the transposed gather
     int64_t reg_cols = 0;       // values per lane of gathers: 0 one for rows under kRegCols, else 1 or 4
     int64_t adjoint_recompute = -1;  // adjoint tables: 0 kept, 1 recomputed, -1 recomputed when that fits 4x more
     int64_t vertex_order = 0;   // internal order at the next upload: 0 descending degree, 1 degree classes + BFS
@@ -303,6 +304,9 @@ struct MemPlan {
     size_t off_H = 0, off_part = 0, off_root = 0, off_colors = 0, off_ecol = 0, off_rawdev = 0, off_tmp = 0;
     size_t off_params = 0;
     std::vector<size_t> off_tab;  // per subtemplate index (tables of non-root non-leaf)
+    // per subtemplate index, 0 = none: Y kept by the forward step for the
+    // adjoint (gradient calls with every table kept)
+    std::vector<size_t> off_ykeep;
 };
 
 size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
@@ -637,7 +641,8 @@ std::string runs_name(size_t tsz, int mode, int main_mode)
 {
     char buf[64];
     snprintf(buf, sizeof buf, "k_runs<%s,%s,%d>", tsz == 4 ? "float" : "double",
-             mode == RUNS_T ? "t" : mode == RUNS_G ? "g" : mode == RUNS_SEQ ? "seq" : "fwd", main_mode);
+             mode == RUNS_T ? "t" : mode == RUNS_G ? "g" : mode == RUNS_SEQ ? "seq" : mode == RUNS_TG ? "tg" : "fwd",
+             main_mode);
     return buf;
 }
 
@@ -704,8 +709,10 @@ int launch_gathering(pg_ctx* c, int aux, int main_mode, SubArgs<T> a, T* part, i
                      int used_per_mille = 0)
 {
     {
-        const int rmode = runs_mode_for(c, aux, a.Wa, used_per_mille);
+        int rmode = runs_mode_for(c, aux, a.Wa, used_per_mille);
+        if (rmode == RUNS_T && a.tr) rmode = RUNS_TG;
         if (rmode >= 0) return launch_gathering_runs<T>(c, rmode, main_mode, a, part, y_cols, bytes);
+        if (a.tr) return set_err(c, PG_ERR_STATE, "fused gradient gather without colour runs");
     }
     if (c->exact_order && (aux == AUX_GATHER || aux == AUX_STAGED))
         return launch_gathering_exact<T>(c, AUX_GATHER, main_mode, a, part, y_cols, bytes,
@@ -919,12 +926,16 @@ struct SubTag {
 
 template <typename T>
 int enqueue_step(pg_ctx* c, const pg_plan* p, const DevSub& d, int B, const PassBufs<T>& pb, const T* main_tab,
-                 const T* aux_tab, T* out_tab)
+                 const T* aux_tab, T* out_tab, T* ykeep)
 {
     const SubTag tag(c, d.s);
     const hipStream_t st = c->stream;
     const double na = (double)c->n_act;
     SubArgs<T> a = pass_args<T>(c, p, B, pb);
+    if (ykeep && d.aa > 1) {
+        a.ykeep = ykeep;
+        a.ldy = B * d.Wy;
+    }
     a.Wy = d.Wy;
     a.Wm = d.Wm;
     a.Wo = d.Wo;
@@ -1012,10 +1023,13 @@ int enqueue_pass(pg_ctx* c, const pg_plan* p, int B, const uint8_t* d_colors_in,
 
     // ---- subtemplates, children before parents
     auto tab = [&](int s) { return (T*)(base + mp.off_tab[s]); };
+    auto ykeep = [&](int s) {
+        return (size_t)s < mp.off_ykeep.size() && mp.off_ykeep[s] ? (T*)(base + mp.off_ykeep[s]) : nullptr;
+    };
     for (auto& dp : p->subs) {
         const DevSub& d = *dp;
         int rc = enqueue_step<T>(c, p, d, B, pb, d.mm > 1 ? tab(d.mi) : nullptr, d.aa > 1 ? tab(d.ai) : nullptr,
-                                 d.root ? nullptr : tab(d.s));
+                                 d.root ? nullptr : tab(d.s), ykeep(d.s));
         if (rc) return rc;
     }
     return PG_OK;
@@ -1639,6 +1653,16 @@ size_t table_bytes(const pg_ctx* c, const DevSub& d, int B, size_t tsz)
     return align_up((size_t)c->n_act * B * d.Wo * tsz, 256);
 }
 
+// Whether the adjoint of step d takes its entries' gradients in the
+// transposed gather (RUNS_TG, from the forward's kept Y) instead of a
+// gradient gather that recomputes Y: when the transposed gather of d runs
+// by colour runs, with option fuse_grad on.
+bool fused_grad_step(const pg_ctx* c, const pg_plan* p, const DevSub& d)
+{
+    if (!c->fuse_grad || d.aa <= 1) return false;
+    return runs_mode_for(c, AUX_GATHER_T, d.Wy, runs_used_per_mille(p, d, AUX_GATHER_T)) == RUNS_T;
+}
+
 GradPlan plan_memory_grad(const pg_ctx* c, const pg_plan* p, int B, size_t tsz)
 {
     GradPlan g;
@@ -1655,6 +1679,12 @@ GradPlan plan_memory_grad(const pg_ctx* c, const pg_plan* p, int B, size_t tsz)
         if (d->root) continue;
         g.off_G[d->s] = off;
         off += table_bytes(c, *d, B, tsz);
+    }
+    g.fwd.off_ykeep.assign(p->nsub, 0);
+    for (auto& d : p->subs) {
+        if (!fused_grad_step(c, p, *d)) continue;
+        g.fwd.off_ykeep[d->s] = off;
+        off += align_up((size_t)c->n_act * B * d->Wy * tsz, 256);
     }
     int maxY = 1;
     for (auto& d : p->subs) maxY = std::max(maxY, d->Wy);
@@ -1675,9 +1705,15 @@ GradPlan plan_memory_grad(const pg_ctx* c, const pg_plan* p, int B, size_t tsz)
 // main row), one gather pass that recomputes Y and adds dY . T_aux-term to
 // every entry's gradient, G_main (from G and Y), then G_aux by the
 // transposed gather of the neighbours' dY.
+//
+// With Y kept by the forward step (Yk, fused_grad_step), the node launch
+// forms dY and G_main together from it, and the transposed gather also takes
+// the entries' gradients (RUNS_TG): no gradient gather, no second read of
+// the aux rows.
 template <typename T>
 int enqueue_sub_backward(pg_ctx* c, const pg_plan* p, const DevSub& d, int B, const PassBufs<T>& pb, T* Ybuf,
-                         T* dYbuf, const double* cot, const T* Tm, const T* Ta, const T* G, T* Gm, T* Ga)
+                         T* dYbuf, const double* cot, const T* Tm, const T* Ta, const T* G, T* Gm, T* Ga,
+                         const T* Yk = nullptr)
 {
     const SubTag tag(c, d.s);
     const hipStream_t st = c->stream;
@@ -1714,6 +1750,38 @@ int enqueue_sub_backward(pg_ctx* c, const pg_plan* p, const DevSub& d, int B, co
     // per-entry gradient read and written by each launch that adds to it.
     const double col = 4.0 * (double)c->n_act * B, Z = (double)c->nnz;
     const double Cs = d.root ? 0.0 : d.Cs, Cm = d.mm > 1 ? d.Cm : 0.0, Ca = d.Ca;
+    if (d.aa > 1 && Yk) {
+        // dY and G_main from G, the main row and the kept Y
+        na.Y = Yk;
+        na.part = 0;
+        // reads G, the main row and Y, writes dY and G_main
+        record(c, node_name, col * (Cs + 2.0 * Ca + 2.0 * Cm), true);
+        HIP_TRY(c, launch_bwd_node<T>(na, c->num_cus, st));
+        record(c, "", 0, false);
+        SubArgs<T> t = base_args;
+        t.out_mode = OUT_Y;
+        if (c->weighted) t.w = fp32 ? (const T*)c->d_wT32.p : (const T*)c->d_wT64.p;
+        t.aux = dYbuf;
+        t.lda = B * d.Wy;
+        t.Wa = d.Wy;  // per-coloring stride of the neighbours' dY rows
+        t.Wy = d.Wa;  // values per coloring: the aux child's row
+        t.Wo = d.Wa;
+        t.ymask = d.amask.as<uint16_t>();
+        t.xmask = d.ymask.as<uint16_t>();  // the set of each position of a neighbour's dY row
+        t.gidx = d.gidxT.as<int16_t>();
+        t.Wy4 = d.Wa4;
+        t.out = Ga;
+        t.ldo = B * d.Wa;
+        t.own = Ta;
+        t.ldown = B * d.Wa;
+        t.tr = c->d_tr.as<int32_t>();
+        t.grad = c->d_grad.as<double>();
+        // reads the neighbours' dY, the own aux rows and the transposed
+        // entry ids, writes G_aux, reads and writes each entry's gradient
+        return launch_gathering<T>(c, AUX_GATHER_T, MAIN_LEAF, t, pb.part, B * d.Wa,
+                                   StepBytes{csc_bytes(c), col * Ca + 20.0 * Z, col * 2.0 * Ca},
+                                   runs_used_per_mille(p, d, AUX_GATHER_T));
+    }
     if (d.aa > 1) {
         na.part = 1;
         record(c, node_name, col * (Cs + Cm + Ca), true);
@@ -1807,10 +1875,12 @@ int enqueue_backward_kept(pg_ctx* c, const pg_plan* p, int B, const GradPlan& gp
     auto gad = [&](int s) { return (T*)(base + gp.off_G[s]); };
     for (auto it = p->subs.rbegin(); it != p->subs.rend(); ++it) {
         const DevSub& d = **it;
+        const size_t yk = (size_t)d.s < gp.fwd.off_ykeep.size() ? gp.fwd.off_ykeep[d.s] : 0;
         int rc = enqueue_sub_backward<T>(c, p, d, B, pb, (T*)(base + gp.off_Y), (T*)(base + gp.off_dY),
                                          (const double*)(base + gp.off_cot), d.mm > 1 ? tab(d.mi) : nullptr,
                                          d.aa > 1 ? tab(d.ai) : nullptr, d.root ? nullptr : gad(d.s),
-                                         d.mm > 1 ? gad(d.mi) : nullptr, d.aa > 1 ? gad(d.ai) : nullptr);
+                                         d.mm > 1 ? gad(d.mi) : nullptr, d.aa > 1 ? gad(d.ai) : nullptr,
+                                         yk ? (const T*)(base + yk) : nullptr);
         if (rc) return rc;
     }
     return PG_OK;
@@ -1875,7 +1945,7 @@ struct Recompute {
         if (rc) return rc;
         *out = pool.alloc(table_bytes(c, d, B, sizeof(T)));
         if (*out == PoolAlloc::kNone) return PG_ERR_NOMEM;
-        if (!dry) rc = enqueue_step<T>(c, p, d, B, pb, at(m), at(a), at(*out));
+        if (!dry) rc = enqueue_step<T>(c, p, d, B, pb, at(m), at(a), at(*out), nullptr);
         pool.release(m);
         pool.release(a);
         return rc;
@@ -2379,6 +2449,12 @@ extern "C" int pg_set_option(pg_ctx* c, const char* key, int64_t v)
             drop_graphs(c);
             c->exact_order = v;
         }
+    } else if (k == "fuse_grad") {
+        if (v != 0 && v != 1) return set_err(c, PG_ERR_INVALID, "fuse_grad must be 0 or 1");
+        if (v != c->fuse_grad) {
+            drop_graphs(c);
+            c->fuse_grad = v;
+        }
     } else if (k == "hub_seq") {
         if (v != 0 && v != 1) return set_err(c, PG_ERR_INVALID, "hub_seq must be 0 or 1");
         if (v != c->hub_seq) {
@@ -2428,6 +2504,7 @@ extern "C" int pg_get_option(const pg_ctx* c, const char* key, int64_t* v)
     else if (k == "color_runs") *v = c->color_runs;
     else if (k == "exact_order") *v = c->exact_order;
     else if (k == "hub_seq") *v = c->hub_seq;
+    else if (k == "fuse_grad") *v = c->fuse_grad;
     else if (k == "reg_cols") *v = c->reg_cols;
     else if (k == "adjoint_recompute") *v = c->adjoint_recompute;
     else if (k == "vertex_order") *v = c->vertex_order;

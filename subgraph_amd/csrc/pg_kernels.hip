@@ -639,6 +639,8 @@ __global__ __launch_bounds__(kBlock) void k_sub(const SubArgs<T> a)
         if (MAIN == MAIN_TABLE && active)
             for (int c = lane; c < nM; c += TEAM) sM[c] = a.main[(int64_t)v * a.ldm + c];
         __syncthreads();
+        if (a.ykeep && active)  // Y kept for the adjoint
+            for (int c = lane; c < nY; c += TEAM) a.ykeep[(int64_t)v * a.ldy + c] = sY[c];
 
         // ----------------------------------------------- phase 3: combine
         const int Wy = a.Wy, Wm = a.Wm, Wo = a.Wo, P = a.P;

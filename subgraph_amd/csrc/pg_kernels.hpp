@@ -102,6 +102,16 @@ struct SubArgs {
     const T* dy;             // [n_act][ldd]
     int32_t ldd;             // dy row stride (B * Wy)
     double* grad;            // [nnz] internal entry order
+    // RUNS_TG (the transposed gather fused with the gradient): the vertex's
+    // own row of the aux child's table, and the transposed entry of each
+    // entry, whose gradient the neighbour's dY row dotted with it gives
+    const T* own;            // [n_act][ldown]
+    int32_t ldown;           // B * (aux child's row width)
+    const int32_t* tr;       // [nnz] entry (u, v) of entry (v, u)
+    // forward steps, optional: Y of every vertex written out as well
+    // (ykeep[v * ldy + b * Wy + i]) for the adjoint
+    T* ykeep;
+    int32_t ldy;
     // main side
     const T* main;
     int32_t ldm;
@@ -152,7 +162,10 @@ hipError_t launch_sub(int team, int aux, int main_mode, int rc, const SubArgs<T>
 // read whole and coalesced, its values added into Y (LDS) neighbour after
 // neighbour through xidx (every (vertex colour, neighbour colour) row of it),
 // a team per (vertex, colouring) with the whole neighbour list.
-enum RunsMode : int { RUNS_FWD = 0, RUNS_T = 1, RUNS_G = 2, RUNS_SEQ = 3 };
+// RUNS_TG is RUNS_T plus, for every entry (v, u) of the vertex's list, the
+// gradient of its transpose: grad[tr[e]] += dY[u] . (own row of v mapped to
+// u's Y positions), a team per item with every coloring in order.
+enum RunsMode : int { RUNS_FWD = 0, RUNS_T = 1, RUNS_G = 2, RUNS_SEQ = 3, RUNS_TG = 4 };
 // values per lane for a gathered row of wg values (0: too wide)
 int runs_values_per_lane(int wg);
 template <typename T>

@@ -268,6 +268,8 @@ __global__ __launch_bounds__(kBlock) void k_runs(const SubArgs<T> a)
         }
         if (active && nM)
             for (int t = lane; t < nM; t += TEAM) sM[t] = a.main[(int64_t)v * a.ldm + (int64_t)b * nM + t];
+        if (active && a.ykeep)  // Y kept for the adjoint
+            for (int t = lane; t < Wt; t += TEAM) a.ykeep[(int64_t)v * a.ldy + (int64_t)b * Wt + t] = sY[t];
         __syncthreads();
         if (active) {
             const int P = a.P;
@@ -436,6 +438,172 @@ __global__ __launch_bounds__(kBlock) void k_runs_g(const SubArgs<T> a)
     }
 }
 
+// The adjoint's transposed gather fused with the gradient (RUNS_TG).  As
+// RUNS_T, a team sums, colour by colour, the dY rows of the vertex's
+// neighbours into its G_aux row (position t of a dY row of colour c is the
+// set A_t; it adds to the vertex's entry A_t - {col(v)} when A_t holds
+// col(v)).  That entry is also where the vertex's own aux-child row enters
+// the neighbour's Y value A_t, so for each neighbour u the team dots u's dY
+// row with its own row mapped the same way: d Y[u] / d w_(u,v) . dY[u], the
+// gradient of the transposed entry (u, v), added to grad[tr[e]] in FP64.
+// With the forward's Y kept, this replaces the gradient gather (RUNS_G) and
+// its second read of the aux rows.  A team owns an item for every coloring,
+// in order, and each entry belongs to one item, so the adds are
+// deterministic.
+template <typename T, int RV, int U>
+__global__ __launch_bounds__(kBlock) void k_runs_tg(const SubArgs<T> a)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    constexpr int TEAM = 64;
+    constexpr int TEAMS = kBlock / TEAM;
+    const int lane = threadIdx.x % TEAM;
+    const int team = threadIdx.x / TEAM;
+    const int B = a.B;
+    const int Wg = a.Wa;  // values per colouring of a neighbour's dY row
+    const int Wt = a.Wy;  // values per colouring of the G_aux row (and of the own row)
+    int16_t* s_rank = reinterpret_cast<int16_t*>(smem_raw);
+    T* sY = reinterpret_cast<T*>(smem_raw + rank_bytes(a.K)) + (size_t)team * Wt;
+    for (int i = threadIdx.x; i < (1 << a.K); i += kBlock) s_rank[i] = a.rank_tab[i];
+    uint32_t xm[RV];
+#pragma unroll
+    for (int t = 0; t < RV; ++t) {
+        const int j = lane + TEAM * t;
+        xm[t] = j < Wg ? (uint32_t)a.xmask[j] : 0u;
+    }
+    __syncthreads();
+    for (int64_t base = (int64_t)blockIdx.x * TEAMS; base < a.n_items; base += (int64_t)gridDim.x * TEAMS) {
+        const int64_t item = base + team;
+        const bool active = item < a.n_items;
+        int v = 0, e0 = 0, e1 = 0, slot = 0;
+        if (active) {
+            if (a.items) {
+                const int4 it = a.items[item];
+                v = it.x; e0 = it.y; e1 = it.z; slot = it.w;
+            } else {
+                v = a.item_base + (int)item;
+                e0 = a.rowptr[v];
+                e1 = a.rowptr[v + 1];
+            }
+        }
+        for (int b = 0; b < B; ++b) {
+            for (int t = lane; t < Wt; t += TEAM) sY[t] = T(0);
+            __syncthreads();
+            if (active) {
+                const uint32_t cv = a.colors[(int64_t)v * B + b];
+                const T* rows = a.aux + (int64_t)b * Wg;
+                const T* own = a.own + (int64_t)v * a.ldown + (int64_t)b * Wt;
+                uint32_t present = 0;
+                for (int e = e0 + lane; e < e1; e += TEAM) present |= 1u << a.ecol[(int64_t)e * B + b];
+                for (int sh = 1; sh < TEAM; sh <<= 1) present |= __shfl_xor(present, sh, TEAM);
+                present &= ~(1u << cv);  // no set of a neighbour of the vertex's colour holds it
+                while (present) {
+                    const uint32_t c = (uint32_t)(__ffs(present) - 1);
+                    present &= present - 1u;
+                    // each of the lane's dY positions: its entry of the
+                    // vertex's row (-1: none) and the own value there
+                    int idx[RV];
+                    T xo[RV];
+                    T acc[RV];
+#pragma unroll
+                    for (int t = 0; t < RV; ++t) {
+                        const int j = lane + TEAM * t;
+                        idx[t] = -1;
+                        if (j < Wg) {
+                            const uint32_t A = expand_bit_r(xm[t], c);
+                            if ((A >> cv) & 1u) idx[t] = s_rank[squeeze_bit_r(A & ~(1u << cv), cv)];
+                        }
+                        xo[t] = idx[t] >= 0 ? own[idx[t]] : T(0);
+                        acc[t] = T(0);
+                    }
+                    for (int e = e0; e < e1; e += TEAM) {
+                        const int ee = e + lane;
+                        const bool ok = ee < e1;
+                        const uint32_t cc = ok ? (uint32_t)a.ecol[(int64_t)ee * B + b] : 0xffu;
+                        const int uid = ok ? a.col[ee] : 0;
+                        const T wv = ok && a.w ? a.w[ee] : T(1);
+                        const int tre = ok ? a.tr[ee] : 0;
+                        uint64_t m = __ballot(ok && cc == c);
+                        while (m) {
+                            int src[U];
+#pragma unroll
+                            for (int q = 0; q < U; ++q) {
+                                src[q] = m ? __ffsll((unsigned long long)m) - 1 : -1;
+                                m &= m ? m - 1 : 0;
+                            }
+                            T x[U][RV];
+                            T wq[U];
+                            int tq[U];
+#pragma unroll
+                            for (int q = 0; q < U; ++q) {
+                                // a slot past the last match reads the first match's row
+                                const int s = src[q] >= 0 ? src[q] : src[0];
+                                const int u = __shfl(uid, s, TEAM);
+                                wq[q] = __shfl(wv, s, TEAM);
+                                tq[q] = __shfl(tre, s, TEAM);
+                                const T* row = rows + (int64_t)u * a.lda;
+#pragma unroll
+                                for (int t = 0; t < RV; ++t) {
+                                    const int j = lane + TEAM * t;
+                                    x[q][t] = j < Wg ? row[j] : T(0);
+                                }
+                            }
+#pragma unroll
+                            for (int q = 0; q < U; ++q) {
+                                if (src[q] < 0) break;
+                                T dot = T(0);
+#pragma unroll
+                                for (int t = 0; t < RV; ++t) {
+                                    acc[t] = a.w ? acc[t] + wq[q] * x[q][t] : acc[t] + x[q][t];
+                                    dot = dot + xo[t] * x[q][t];
+                                }
+                                for (int sh = 1; sh < TEAM; sh <<= 1) dot = dot + __shfl_xor(dot, sh, TEAM);
+                                if (lane == 0) (void)unsafeAtomicAdd(&a.grad[tq[q]], (double)dot);
+                            }
+                        }
+                    }
+                    // the colour's sum into the vertex's row: distinct targets
+                    // per lane and position, one wavefront per team (in order)
+#pragma unroll
+                    for (int t = 0; t < RV; ++t)
+                        if (idx[t] >= 0) sY[idx[t]] = sY[idx[t]] + acc[t];
+                }
+            }
+            __syncthreads();
+            if (active) {
+                T* dst = a.out + (int64_t)(a.out_mode == OUT_Y ? v : slot) * a.ldo + (int64_t)b * Wt;
+                for (int t = lane; t < Wt; t += TEAM) dst[t] = sY[t];
+            }
+            __syncthreads();
+        }
+    }
+}
+
+template <typename T, int RV>
+hipError_t launch_runs_tg_t(const SubArgs<T>& a, int num_cus, int blocks_per_cu, hipStream_t st)
+{
+    constexpr int U = RV <= 2 ? 8 : 4;
+    const size_t lds = rank_bytes(a.K) + (size_t)(kBlock / 64) * a.Wy * sizeof(T);
+    const int64_t need = ((int64_t)a.n_items + (kBlock / 64) - 1) / (kBlock / 64);
+    if (need <= 0) return hipSuccess;
+    static thread_local std::map<size_t, int> per_cu_cache;
+    int per_cu = blocks_per_cu;
+    if (per_cu <= 0) {
+        auto it = per_cu_cache.find(lds);
+        if (it != per_cu_cache.end()) {
+            per_cu = it->second;
+        } else {
+            hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_runs_tg<T, RV, U>, kBlock, lds);
+            if (e != hipSuccess) return e;
+            if (per_cu < 1) return hipErrorInvalidConfiguration;
+            per_cu_cache[lds] = per_cu;
+        }
+    }
+    const int64_t cap = (int64_t)per_cu * num_cus;
+    const int grid = (int)(need < cap ? need : cap);
+    hipLaunchKernelGGL((k_runs_tg<T, RV, U>), dim3(grid), dim3(kBlock), lds, st, a);
+    return hipGetLastError();
+}
+
 template <typename T, int RV>
 hipError_t launch_runs_g_t(const SubArgs<T>& a, int num_cus, int blocks_per_cu, hipStream_t st)
 {
@@ -557,6 +725,8 @@ __global__ __launch_bounds__(kBlock) void k_runs_seq(const SubArgs<T> a)
         }
         if (active && nM)
             for (int t = lane; t < nM; t += TEAM) sM[t] = a.main[(int64_t)v * a.ldm + (int64_t)b * nM + t];
+        if (active && a.ykeep)  // Y kept for the adjoint
+            for (int t = lane; t < Wt; t += TEAM) a.ykeep[(int64_t)v * a.ldy + (int64_t)b * Wt + t] = sY[t];
         __syncthreads();
         if (active) {
             const int P = a.P;
@@ -706,6 +876,18 @@ hipError_t launch_runs(int mode, int main_mode, const SubArgs<T>& a, int num_cus
     if (mode == RUNS_T) {
         if (main_mode != MAIN_LEAF || (a.out_mode != OUT_Y && a.out_mode != OUT_PARTIAL)) return hipErrorInvalidValue;
         return launch_runs_rv<T, RUNS_T, MAIN_LEAF>(rv, a, num_cus, blocks_per_cu, st);
+    }
+    if (mode == RUNS_TG) {
+        if (main_mode != MAIN_LEAF || (a.out_mode != OUT_Y && a.out_mode != OUT_PARTIAL) || !a.own || !a.tr ||
+            !a.grad)
+            return hipErrorInvalidValue;
+        switch (rv) {
+        case 1: return launch_runs_tg_t<T, 1>(a, num_cus, blocks_per_cu, st);
+        case 2: return launch_runs_tg_t<T, 2>(a, num_cus, blocks_per_cu, st);
+        case 4: return launch_runs_tg_t<T, 4>(a, num_cus, blocks_per_cu, st);
+        case 8: return launch_runs_tg_t<T, 8>(a, num_cus, blocks_per_cu, st);
+        default: return hipErrorInvalidValue;
+        }
     }
     if (mode != RUNS_FWD) return hipErrorInvalidValue;
     return main_mode == MAIN_LEAF ? launch_runs_rv<T, RUNS_FWD, MAIN_LEAF>(rv, a, num_cus, blocks_per_cu, st)

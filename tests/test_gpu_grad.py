@@ -183,6 +183,7 @@ def test_recompute_matches_kept_tables(gpu_ctx, tv, te):
     lo, hi = np.minimum(rows, rowidx), np.maximum(rows, rowidx)
     w = (0.5 + ((lo * 7919 + hi * 104729) % 1000) / 1000.0).astype(np.float32)  # symmetric
     saved = gpu_ctx.get_option("adjoint_recompute")
+    gpu_ctx.set_option("fuse_grad", 0)  # the same gradient gathers in both sweeps
     try:
         for weighted in (False, True):
             gpu_ctx.upload_graph(colptr, rowidx, vals=w if weighted else None)
@@ -203,6 +204,61 @@ def test_recompute_matches_kept_tables(gpu_ctx, tv, te):
                 assert np.any(g0 != 0)
     finally:
         gpu_ctx.set_option("adjoint_recompute", saved)
+        gpu_ctx.set_option("fuse_grad", 1)
+
+
+@pytest.mark.parametrize("color_runs", [-1, 1])
+def test_fused_gradient_matches_gradient_gather(gpu_ctx, color_runs):
+    """Option fuse_grad (default 1, kept tables): the forward keeps Y of the
+    steps whose transposed gather runs by colour runs, and that gather also
+    takes the entries' gradients (dY of each neighbour dotted with the
+    vertex's own row).  Against the gradient gather that recomputes Y: FP64
+    gradients exactly equal for unit weights (integer partial sums), to FP64
+    rounding with weights; FP32 within 1e-6 in norm (the dot products sum in
+    another order); counts bit for bit.  color_runs 1 fuses every step with a
+    wide enough row; small hubs exercise the hub chunks of the fused gather."""
+    n = 1 << 11
+    s, d = sg.rmat_edges(11, 16, seed=23)
+    colptr, rowidx = sg.csc_from_edges(n, s, d)
+    rows = np.repeat(np.arange(n), np.diff(colptr))
+    lo, hi = np.minimum(rows, rowidx), np.maximum(rows, rowidx)
+    w = (0.5 + ((lo * 7919 + hi * 104729) % 1000) / 1000.0).astype(np.float32)  # symmetric
+    keys = ("adjoint_recompute", "fuse_grad", "color_runs", "hub_degree", "hub_chunk")
+    saved = [gpu_ctx.get_option(k) for k in keys]
+    gpu_ctx.set_option("adjoint_recompute", 0)
+    gpu_ctx.set_option("color_runs", color_runs)
+    gpu_ctx.set_option("hub_degree", 48)
+    gpu_ctx.set_option("hub_chunk", 32)
+    try:
+        for weighted in (False, True):
+            gpu_ctx.upload_graph(colptr, rowidx, vals=w if weighted else None)
+            for tv, te in ((12, TREE12), (10, TREE10)):
+                plan = gpu_ctx.plan(tv, te)
+                for prec in ("fp64", "fp32"):
+                    out = {}
+                    for fuse in (0, 1):
+                        gpu_ctx.set_option("fuse_grad", fuse)
+                        print(f"[fused-grad] color_runs={color_runs} weighted={weighted} k={tv} {prec} fuse={fuse}",
+                              flush=True)
+                        out[fuse] = gpu_ctx.count_grad(plan, 5, seed=tv, precision=prec)
+                        if fuse and (tv == 12 or color_runs == 1):
+                            gpu_ctx.set_option("timing", 1)
+                            gpu_ctx.count_grad(plan, 5, seed=tv, precision=prec)
+                            gpu_ctx.set_option("timing", 0)
+                            names = list(gpu_ctx.last_timing_kernels())
+                            assert any(",tg," in k for k in names), names
+                    (r0, g0), (r1, g1) = out[0], out[1]
+                    assert np.array_equal(r0, r1)
+                    assert np.any(g0 != 0)
+                    if prec == "fp64" and not weighted and np.abs(g0).max() < 2.0 ** 50:
+                        assert np.array_equal(g0, g1), np.flatnonzero(g0 != g1)[:10]
+                    elif prec == "fp64":
+                        assert np.allclose(g0, g1, rtol=1e-12, atol=0.0)
+                    else:
+                        assert np.linalg.norm(g1 - g0) <= 1e-6 * np.linalg.norm(g0)
+    finally:
+        for k, v in zip(keys, saved):
+            gpu_ctx.set_option(k, v)
 
 
 def test_default_adjoint_mode(gpu_ctx):
