@@ -112,6 +112,38 @@ __device__ __forceinline__ T runs_combine(Split split_at, int P, const T* y, con
     return acc;
 }
 
+// The team-wide sums of U per-lane values (U a power of two, <= 64) by a
+// butterfly reduce-scatter: at the step of lane bit s < U each lane keeps
+// half of its values (the half its bit selects) and adds the partner's copy
+// of them, so after log2(U) steps a lane holds one value, index *q (its low
+// lane bits reversed), summed over its U-lane group; xor steps over the
+// higher lane bits finish the sums in every lane.  (U - 1) + 6 - log2(U)
+// shuffles for U sums instead of 6 each; a fixed order, the same in every
+// lane holding a given sum.
+template <typename T, int U>
+__device__ __forceinline__ T team_sums(const T (&d)[U], int lane, int* q)
+{
+    T v[U];
+#pragma unroll
+    for (int j = 0; j < U; ++j) v[j] = d[j];
+    int idx = 0;
+#pragma unroll
+    for (int s = 1, n = U; s < U; s <<= 1, n >>= 1) {
+        const bool h = (lane & s) != 0;
+#pragma unroll
+        for (int j = 0; j < n / 2; ++j) {
+            const T keep = h ? v[j + n / 2] : v[j];
+            const T send = h ? v[j] : v[j + n / 2];
+            v[j] = keep + __shfl_xor(send, s, 64);
+        }
+        idx += h ? n / 2 : 0;
+    }
+#pragma unroll
+    for (int s = U; s < 64; s <<= 1) v[0] = v[0] + __shfl_xor(v[0], s, 64);
+    *q = idx;
+    return v[0];
+}
+
 }  // namespace
 
 template <typename T>
@@ -409,18 +441,24 @@ __global__ __launch_bounds__(kBlock) void k_runs_g(const SubArgs<T> a)
                                     x[q][t] = j < Wg ? row[j] : T(0);
                                 }
                             }
+                            T dot[U];
 #pragma unroll
                             for (int q = 0; q < U; ++q) {
-                                if (src[q] < 0) break;
-                                T dot = T(0);
+                                dot[q] = T(0);
 #pragma unroll
-                                for (int t = 0; t < RV; ++t) {
+                                for (int t = 0; t < RV; ++t) dot[q] = dot[q] + dyc[t] * x[q][t];
+                                if (src[q] < 0) continue;
+#pragma unroll
+                                for (int t = 0; t < RV; ++t)
                                     acc[t] = a.w ? acc[t] + wq[q] * x[q][t] : acc[t] + x[q][t];
-                                    dot = dot + dyc[t] * x[q][t];
-                                }
-                                for (int sh = 1; sh < TEAM; sh <<= 1) dot = dot + __shfl_xor(dot, sh, TEAM);
-                                if (lane == 0) (void)unsafeAtomicAdd(&a.grad[e + src[q]], (double)dot);
                             }
+                            int qs;
+                            const T sum = team_sums<T, U>(dot, lane, &qs);
+                            int sq = -1;
+#pragma unroll
+                            for (int q = 0; q < U; ++q)
+                                if (q == qs) sq = src[q];
+                            if (lane < U && sq >= 0) (void)unsafeAtomicAdd(&a.grad[e + sq], (double)sum);
                         }
                     }
 #pragma unroll
@@ -547,18 +585,27 @@ __global__ __launch_bounds__(kBlock) void k_runs_tg(const SubArgs<T> a)
                                     x[q][t] = j < Wg ? row[j] : T(0);
                                 }
                             }
+                            T dot[U];
 #pragma unroll
                             for (int q = 0; q < U; ++q) {
-                                if (src[q] < 0) break;
-                                T dot = T(0);
+                                dot[q] = T(0);
 #pragma unroll
-                                for (int t = 0; t < RV; ++t) {
+                                for (int t = 0; t < RV; ++t) dot[q] = dot[q] + xo[t] * x[q][t];
+                                if (src[q] < 0) continue;
+#pragma unroll
+                                for (int t = 0; t < RV; ++t)
                                     acc[t] = a.w ? acc[t] + wq[q] * x[q][t] : acc[t] + x[q][t];
-                                    dot = dot + xo[t] * x[q][t];
-                                }
-                                for (int sh = 1; sh < TEAM; sh <<= 1) dot = dot + __shfl_xor(dot, sh, TEAM);
-                                if (lane == 0) (void)unsafeAtomicAdd(&a.grad[tq[q]], (double)dot);
                             }
+                            int qs;
+                            const T sum = team_sums<T, U>(dot, lane, &qs);
+                            int sq = -1, tgt = 0;
+#pragma unroll
+                            for (int q = 0; q < U; ++q)
+                                if (q == qs) {
+                                    sq = src[q];
+                                    tgt = tq[q];
+                                }
+                            if (lane < U && sq >= 0) (void)unsafeAtomicAdd(&a.grad[tgt], (double)sum);
                         }
                     }
                     // the colour's sum into the vertex's row: distinct targets
