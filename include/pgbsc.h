@@ -93,8 +93,15 @@ const char* pg_last_error(const pg_ctx* ctx);
  *                  different rounding
  *   "hub_seq"      exact order: 1 = hub rows by the batched hub kernel (default), 0 = by
  *                  column parts of the gathering kernel (same results)
+ *   "hub_concurrent" 1 = a gathering step's hub launch runs on a second stream beside the
+ *                  other vertices' launch (default); 0 = one after the other (same results)
  *   "color_runs"   whole-row neighbour sums: 1 always (rows up to 512 values), 0 never,
  *                  -1 by row width and used share (default)
+ *   "fuse_grad"    pg_count_grad with kept tables: 1 = the forward keeps Y of the steps whose
+ *                  transposed gather runs by colour runs, and that gather also takes the
+ *                  entries' gradients (default); 0 = a gradient gather recomputes Y
+ *   "node_fp32"    pg_count_grad with FP32 tables: 1 = the adjoint's split sums in FP32
+ *                  (fused multiply-adds, default), 0 = in FP64
  *   "vertex_order" internal vertex order set at the next pg_graph_upload: 0 = descending
  *                  degree (default); 1 = degree classes aligned with the bucket thresholds,
  *                  breadth-first within a class (one-coloring passes of smaller graphs gain

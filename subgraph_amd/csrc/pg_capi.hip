@@ -157,6 +157,7 @@ struct pg_ctx {
     int64_t exact_order = 1;    // forward neighbour sums in the reference's order (one group, hubs split by columns)
     int64_t hub_seq = 1;        // exact order: hub rows by k_hub_seq (1) or by column parts of k_sub (0)
     int64_t fuse_grad = 1;      // adjoint: forward Y kept and the gradient taken in the transposed gather
+    int64_t node_fp32 = 1;      // adjoint node sums of FP32 tables: 1 FP32 fused multiply-adds, 0 FP64
     int64_t reg_cols = 0;       // values per lane of gathers: 0 one for rows under kRegCols, else 1 or 4
     int64_t adjoint_recompute = -1;  // adjoint tables: 0 kept, 1 recomputed, -1 recomputed when that fits 4x more
     int64_t vertex_order = 0;   // internal order at the next upload: 0 descending degree, 1 degree classes + BFS
@@ -194,6 +195,12 @@ struct pg_ctx {
     std::vector<hipStream_t> side;
     std::vector<hipEvent_t> ev_join;
     hipEvent_t ev_fork = nullptr;
+    // hub launches of a gathering step run on hub_stream beside the other
+    // vertices' launch (fork and join events), option "hub_concurrent"
+    hipStream_t hub_stream = nullptr;
+    hipEvent_t ev_hub_fork = nullptr, ev_hub_join = nullptr;
+    int64_t hub_concurrent = 1;
+    bool in_branches = false;  // enqueueing the branches of a mixed-batch capture
     std::map<std::tuple<const pg_plan*, int, int>, std::vector<int64_t>> tuned;  // (plan, B, tsize) -> settings, tile
     // the unpruned mode (PG_FLAG_UNPRUNED): the reference's _bufVec, zeroed
     // at graph upload and then only ever added to, and its tables
@@ -646,6 +653,34 @@ std::string runs_name(size_t tsz, int mode, int main_mode)
     return buf;
 }
 
+// A gathering step's hub launch (hubs(stream)) runs on the hub stream beside
+// the other vertices' launches (rest(stream)) on the context's stream, which
+// then waits for it: the hub rows' long neighbour lists overlap the rest of
+// the step instead of leaving the device to a few wavefronts.  Sequential
+// with option hub_concurrent 0, while timing (per-launch events on one
+// stream), without a hub stream, and inside the branches of a mixed-batch
+// capture (a third stream joined from several capturing branches took the
+// process down in the runtime there; those branches already run side by
+// side).
+template <typename H, typename R>
+int fork_hubs(pg_ctx* c, bool have_hubs, H hubs, R rest)
+{
+    if (!have_hubs) return rest(c->stream);
+    if (!c->hub_concurrent || c->timing || !c->hub_stream || c->in_branches) {
+        const int rc = hubs(c->stream);
+        return rc ? rc : rest(c->stream);
+    }
+    HIP_TRY(c, hipEventRecord(c->ev_hub_fork, c->stream));
+    HIP_TRY(c, hipStreamWaitEvent(c->hub_stream, c->ev_hub_fork, 0));
+    int rc = hubs(c->hub_stream);
+    if (rc) return rc;
+    HIP_TRY(c, hipEventRecord(c->ev_hub_join, c->hub_stream));
+    rc = rest(c->stream);
+    if (rc) return rc;
+    HIP_TRY(c, hipStreamWaitEvent(c->stream, c->ev_hub_join, 0));
+    return PG_OK;
+}
+
 // launch_gathering with colour runs: the hub chunks' partial rows, all other
 // vertices in one launch, then the hub rows from their partials
 template <typename T>
@@ -661,7 +696,7 @@ int launch_gathering_runs(pg_ctx* c, int rmode, int main_mode, SubArgs<T> a, T* 
         if (N > 0) b += bytes.per_vertex * verts / N;
         return b;
     };
-    if (c->n_hub > 0) {
+    auto hubs = [&](hipStream_t hs) -> int {
         SubArgs<T> h = a;
         h.out = part;
         h.ldo = y_cols;
@@ -670,19 +705,24 @@ int launch_gathering_runs(pg_ctx* c, int rmode, int main_mode, SubArgs<T> a, T* 
         h.n_items = c->n_hub_items;
         h.item_base = 0;
         record(c, runs_name(sizeof(T), rmode, main_mode), share(c->h_rowptr[c->n_hub], 0), true);
-        HIP_TRY(c, launch_runs<T>(rmode, main_mode, h, cus, c->blocks_per_cu, st));
+        HIP_TRY(c, launch_runs<T>(rmode, main_mode, h, cus, c->blocks_per_cu, hs));
         record(c, "", 0, false);
-    }
-    if (c->n_hub < c->n_act) {
+        return PG_OK;
+    };
+    auto rest = [&](hipStream_t rs) -> int {
+        if (c->n_hub >= c->n_act) return PG_OK;
         SubArgs<T> r = a;
         r.items = nullptr;
         r.item_base = c->n_hub;
         r.n_items = c->n_act - c->n_hub;
         record(c, runs_name(sizeof(T), rmode, main_mode),
                share((double)c->nnz - c->h_rowptr[c->n_hub], c->n_act - c->n_hub), true);
-        HIP_TRY(c, launch_runs<T>(rmode, main_mode, r, cus, c->blocks_per_cu, st));
+        HIP_TRY(c, launch_runs<T>(rmode, main_mode, r, cus, c->blocks_per_cu, rs));
         record(c, "", 0, false);
-    }
+        return PG_OK;
+    };
+    const int frc = fork_hubs(c, c->n_hub > 0, hubs, rest);
+    if (frc) return frc;
     if (c->n_hub > 0) {
         SubArgs<T> f = a;
         f.aux = part;
@@ -731,7 +771,7 @@ int launch_gathering(pg_ctx* c, int aux, int main_mode, SubArgs<T> a, T* part, i
         if (N > 0) b += bytes.per_vertex * verts / N;
         return b;
     };
-    if (c->n_hub > 0) {
+    auto hubs = [&](hipStream_t hs) -> int {
         SubArgs<T> h = a;
         h.out = part;
         h.ldo = y_cols;
@@ -741,24 +781,30 @@ int launch_gathering(pg_ctx* c, int aux, int main_mode, SubArgs<T> a, T* part, i
         h.item_base = 0;
         h.ng = 64 / cl;
         record(c, kname<T>(64, aux, main_mode, rc), share(c->h_rowptr[c->n_hub], 0), true);
-        HIP_TRY(c, launch_sub<T>(64, aux, main_mode, rc, h, cus, c->blocks_per_cu, st));
+        HIP_TRY(c, launch_sub<T>(64, aux, main_mode, rc, h, cus, c->blocks_per_cu, hs));
         record(c, "", 0, false);
-    }
-    struct Bucket { int32_t lo, hi; int lanes; };
-    const Bucket buckets[3] = {{c->n_hub, c->hi_end, 64}, {c->hi_end, c->mid_end, 16}, {c->mid_end, c->n_act, 8}};
-    for (const Bucket& bk : buckets) {
-        if (bk.hi <= bk.lo) continue;
-        const int team = std::max(bk.lanes, cl);
-        SubArgs<T> r = a;
-        r.items = nullptr;
-        r.item_base = bk.lo;
-        r.n_items = bk.hi - bk.lo;
-        r.ng = team / cl;
-        record(c, kname<T>(team, aux, main_mode, rc),
-               share((double)c->h_rowptr[bk.hi] - c->h_rowptr[bk.lo], bk.hi - bk.lo), true);
-        HIP_TRY(c, launch_sub<T>(team, aux, main_mode, rc, r, cus, c->blocks_per_cu, st));
-        record(c, "", 0, false);
-    }
+        return PG_OK;
+    };
+    auto rest = [&](hipStream_t rs) -> int {
+        struct Bucket { int32_t lo, hi; int lanes; };
+        const Bucket buckets[3] = {{c->n_hub, c->hi_end, 64}, {c->hi_end, c->mid_end, 16}, {c->mid_end, c->n_act, 8}};
+        for (const Bucket& bk : buckets) {
+            if (bk.hi <= bk.lo) continue;
+            const int team = std::max(bk.lanes, cl);
+            SubArgs<T> r = a;
+            r.items = nullptr;
+            r.item_base = bk.lo;
+            r.n_items = bk.hi - bk.lo;
+            r.ng = team / cl;
+            record(c, kname<T>(team, aux, main_mode, rc),
+                   share((double)c->h_rowptr[bk.hi] - c->h_rowptr[bk.lo], bk.hi - bk.lo), true);
+            HIP_TRY(c, launch_sub<T>(team, aux, main_mode, rc, r, cus, c->blocks_per_cu, rs));
+            record(c, "", 0, false);
+        }
+        return PG_OK;
+    };
+    const int frc = fork_hubs(c, c->n_hub > 0, hubs, rest);
+    if (frc) return frc;
     if (c->n_hub > 0) {
         SubArgs<T> f = a;
         f.aux = part;
@@ -807,7 +853,7 @@ int launch_gathering_exact(pg_ctx* c, int aux, int main_mode, SubArgs<T> a, T* p
         return b;
     };
     a.ng = 1;
-    if (c->n_hub > 0) {
+    auto hubs = [&](hipStream_t hs) -> int {
         SubArgs<T> h = a;
         h.segpad = 0;
         h.out = part;
@@ -819,7 +865,7 @@ int launch_gathering_exact(pg_ctx* c, int aux, int main_mode, SubArgs<T> a, T* p
             h.n_items = c->n_hub;
             record(c, sizeof(T) == 4 ? "k_hub_seq<float>" : "k_hub_seq<double>", share(c->h_rowptr[c->n_hub], 0),
                    true);
-            HIP_TRY(c, launch_hub_seq<T>(h, cus, c->blocks_per_cu, st));
+            HIP_TRY(c, launch_hub_seq<T>(h, cus, c->blocks_per_cu, hs));
         } else {
             // a part per chunk of cl slots: the same per-entry work as a
             // non-hub vertex's team, whose chunks run one after another
@@ -827,11 +873,13 @@ int launch_gathering_exact(pg_ctx* c, int aux, int main_mode, SubArgs<T> a, T* p
             h.col_parts = (nslots + team - 1) / team;
             h.n_items = c->n_hub * h.col_parts;
             record(c, kname<T>(team, aux, main_mode, rc), share(c->h_rowptr[c->n_hub], 0), true);
-            HIP_TRY(c, launch_sub<T>(team, aux, main_mode, rc, h, cus, c->blocks_per_cu, st));
+            HIP_TRY(c, launch_sub<T>(team, aux, main_mode, rc, h, cus, c->blocks_per_cu, hs));
         }
         record(c, "", 0, false);
-    }
-    if (c->n_hub < c->n_act) {
+        return PG_OK;
+    };
+    auto rest = [&](hipStream_t rs) -> int {
+        if (c->n_hub >= c->n_act) return PG_OK;
         SubArgs<T> r = a;
         r.items = nullptr;
         r.item_base = c->n_hub;
@@ -839,13 +887,16 @@ int launch_gathering_exact(pg_ctx* c, int aux, int main_mode, SubArgs<T> a, T* p
         const double sh = share((double)c->nnz - c->h_rowptr[c->n_hub], c->n_act - c->n_hub);
         if (rows) {
             record(c, runs_name(sizeof(T), RUNS_SEQ, main_mode), sh, true);
-            HIP_TRY(c, launch_runs<T>(RUNS_SEQ, main_mode, r, cus, c->blocks_per_cu, st));
+            HIP_TRY(c, launch_runs<T>(RUNS_SEQ, main_mode, r, cus, c->blocks_per_cu, rs));
         } else {
             record(c, kname<T>(cl, aux_b, main_mode, rc), sh, true);
-            HIP_TRY(c, launch_sub<T>(cl, aux_b, main_mode, rc, r, cus, c->blocks_per_cu, st));
+            HIP_TRY(c, launch_sub<T>(cl, aux_b, main_mode, rc, r, cus, c->blocks_per_cu, rs));
         }
         record(c, "", 0, false);
-    }
+        return PG_OK;
+    };
+    const int frc = fork_hubs(c, c->n_hub > 0, hubs, rest);
+    if (frc) return frc;
     if (c->n_hub > 0) {
         SubArgs<T> f = a;
         f.aux = part;
@@ -1534,6 +1585,7 @@ int count_multi_impl(pg_ctx* c, int np, const pg_plan* const* plans, const int32
         if (it == c->multi_graphs.end()) {
             const hipStream_t main = c->stream;
             HIP_TRY(c, hipStreamBeginCapture(main, hipStreamCaptureModeThreadLocal));
+            c->in_branches = true;
             hipError_t e = hipEventRecord(c->ev_fork, main);
             for (int i = 0; i < np && rc == PG_OK && e == hipSuccess; ++i) {
                 if (B[i] <= 0) continue;
@@ -1548,6 +1600,7 @@ int count_multi_impl(pg_ctx* c, int np, const pg_plan* const* plans, const int32
                 if (e == hipSuccess) e = hipStreamWaitEvent(main, c->ev_join[i], 0);
             }
             put_settings(c, knob[0]);
+            c->in_branches = false;
             hipGraph_t graph = nullptr;
             const hipError_t ee = hipStreamEndCapture(main, &graph);
             if (rc || e != hipSuccess || ee != hipSuccess) {
@@ -1742,6 +1795,7 @@ int enqueue_sub_backward(pg_ctx* c, const pg_plan* p, const DevSub& d, int B, co
     na.by_mainT = d.by_mainT.as<uint32_t>();
     na.P = d.P;
     na.scale = (d.scaled && fp32) ? (double)1.0e-12f : 1.0;
+    na.sums32 = fp32 && c->node_fp32 ? 1 : 0;
     const std::string node_name = std::string("k_bwd_node<") + (fp32 ? "float>" : "double>");
     // Algorithmic bytes, in the forward's units (full-width FP32 columns of
     // n_act x B values; Y and dY are as wide as the aux child's table; the
@@ -2366,7 +2420,11 @@ extern "C" int pg_create(pg_ctx** out, int device)
         delete c;
         return PG_ERR_HIP;
     }
-    if (scratch_setup(c) != hipSuccess) {
+    if (scratch_setup(c) != hipSuccess || hipStreamCreateWithFlags(&c->hub_stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_hub_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_hub_join, hipEventDisableTiming) != hipSuccess) {
+        if (c->ev_hub_fork) (void)hipEventDestroy(c->ev_hub_fork);
+        if (c->hub_stream) (void)hipStreamDestroy(c->hub_stream);
         (void)hipStreamDestroy(c->stream);
         delete c;
         return PG_ERR_HIP;
@@ -2384,6 +2442,9 @@ extern "C" void pg_destroy(pg_ctx* c)
     for (hipStream_t s : c->side) (void)hipStreamDestroy(s);
     for (hipEvent_t e : c->ev_join) (void)hipEventDestroy(e);
     if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+    if (c->ev_hub_fork) (void)hipEventDestroy(c->ev_hub_fork);
+    if (c->ev_hub_join) (void)hipEventDestroy(c->ev_hub_join);
+    if (c->hub_stream) (void)hipStreamDestroy(c->hub_stream);
     (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -2449,11 +2510,23 @@ extern "C" int pg_set_option(pg_ctx* c, const char* key, int64_t v)
             drop_graphs(c);
             c->exact_order = v;
         }
+    } else if (k == "node_fp32") {
+        if (v != 0 && v != 1) return set_err(c, PG_ERR_INVALID, "node_fp32 must be 0 or 1");
+        if (v != c->node_fp32) {
+            drop_graphs(c);
+            c->node_fp32 = v;
+        }
     } else if (k == "fuse_grad") {
         if (v != 0 && v != 1) return set_err(c, PG_ERR_INVALID, "fuse_grad must be 0 or 1");
         if (v != c->fuse_grad) {
             drop_graphs(c);
             c->fuse_grad = v;
+        }
+    } else if (k == "hub_concurrent") {
+        if (v != 0 && v != 1) return set_err(c, PG_ERR_INVALID, "hub_concurrent must be 0 or 1");
+        if (v != c->hub_concurrent) {
+            drop_graphs(c);
+            c->hub_concurrent = v;
         }
     } else if (k == "hub_seq") {
         if (v != 0 && v != 1) return set_err(c, PG_ERR_INVALID, "hub_seq must be 0 or 1");
@@ -2504,7 +2577,9 @@ extern "C" int pg_get_option(const pg_ctx* c, const char* key, int64_t* v)
     else if (k == "color_runs") *v = c->color_runs;
     else if (k == "exact_order") *v = c->exact_order;
     else if (k == "hub_seq") *v = c->hub_seq;
+    else if (k == "hub_concurrent") *v = c->hub_concurrent;
     else if (k == "fuse_grad") *v = c->fuse_grad;
+    else if (k == "node_fp32") *v = c->node_fp32;
     else if (k == "reg_cols") *v = c->reg_cols;
     else if (k == "adjoint_recompute") *v = c->adjoint_recompute;
     else if (k == "vertex_order") *v = c->vertex_order;

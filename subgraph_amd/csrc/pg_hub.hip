@@ -167,6 +167,11 @@ hipError_t launch_hub_seq(const SubArgs<T>& a, int num_cus, int blocks_per_cu, h
     if (a.n_items <= 0) return hipSuccess;
     if (a.K < 1 || a.K > kMaxK - 1 || a.k > 27 || !a.ymask || !a.rank_tab || a.out_mode != OUT_PARTIAL)
         return hipErrorInvalidValue;
+    // rows of up to 64 values (a wavefront per hub and coloring): the largest
+    // hub's chain bounds the launch, so a deeper batch
+    if constexpr (sizeof(T) == 4) {
+        if (a.Wy <= 64) return launch_hub_seq_t<T, kHubBatchNarrow>(a, num_cus, blocks_per_cu, st);
+    }
     return launch_hub_seq_t<T, kHubBatch>(a, num_cus, blocks_per_cu, st);
 }
 

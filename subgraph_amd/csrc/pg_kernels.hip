@@ -916,10 +916,16 @@ hipError_t launch_gather_weights_f64(double* dst, const float* src, const int32_
 
 // ------------------------------------------------------------- adjoint ---
 
+// One term of a split sum: FP64 multiply and add, or (FP32 sums of FP32
+// tables) one fused multiply-add.
+__device__ __forceinline__ double node_term(double acc, double g, double r) { return acc + g * r; }
+__device__ __forceinline__ float node_term(float acc, float g, float r) { return __builtin_fmaf(g, r, acc); }
+
 // One thread per (vertex, coloring, value): the first n_act*B*Wy threads
 // form dY, the next n_act*B*Wm form G_main.  Sums run over the transposed
-// split lists in their fixed order, in FP64, then round once to T.
-template <typename T>
+// split lists in their fixed order, in Acc (FP64, or FP32 with fused
+// multiply-adds), then round once to T.
+template <typename T, typename Acc>
 __device__ __forceinline__ void bwd_node_item(const NodeArgs<T>& a, int64_t i, int64_t ny)
 {
     const bool is_dy = i < ny;
@@ -930,43 +936,43 @@ __device__ __forceinline__ void bwd_node_item(const NodeArgs<T>& a, int64_t i, i
     const int64_t v = vb / a.B;
     const int b = (int)(vb - v * a.B);
     const T* G = a.G ? a.G + vb * a.Wo : nullptr;
-    const double g0 = a.root_cot[b];  // the root's single "table" value
-    double acc = 0.0;
+    const Acc g0 = (Acc)a.root_cot[b];  // the root's single "table" value
+    Acc acc = Acc(0);
     if (is_dy) {
         if (!a.Tm) {
-            acc = G ? (double)G[x] : g0;  // main leaf: T_s[o] = Y[o]
+            acc = G ? (Acc)G[x] : g0;  // main leaf: T_s[o] = Y[o]
         } else {
             const T* m = a.Tm + vb * a.Wm;
             const uint32_t* lst = a.by_aux + (size_t)x * a.Pa;
             for (int p = 0; p < a.Pa; ++p) {
                 const uint32_t e = lst[p];
-                acc += (G ? (double)G[e >> 16] : g0) * (double)m[e & 0xffffu];
+                acc = node_term(acc, G ? (Acc)G[e >> 16] : g0, (Acc)m[e & 0xffffu]);
             }
         }
-        a.dY[vb * a.Wy + x] = (T)(acc * a.scale);
+        a.dY[vb * a.Wy + x] = (T)(acc * (Acc)a.scale);
     } else {
         const int cv = a.colors[vb];
         const uint32_t* lst = a.by_main + (size_t)x * a.Pm;
         for (int p = 0; p < a.Pm; ++p) {
             const uint32_t e = lst[p];
             const int ra = (int)(e & 0xffffu);
-            double y;
-            if (a.y_from_h) y = (double)a.Y[vb * a.k + ra + (ra >= cv ? 1 : 0)];
-            else y = (double)a.Y[vb * a.Wy + ra];
-            acc += (G ? (double)G[e >> 16] : g0) * y;
+            Acc y;
+            if (a.y_from_h) y = (Acc)a.Y[vb * a.k + ra + (ra >= cv ? 1 : 0)];
+            else y = (Acc)a.Y[vb * a.Wy + ra];
+            acc = node_term(acc, G ? (Acc)G[e >> 16] : g0, y);
         }
-        a.Gm[vb * a.Wm + x] = (T)(acc * a.scale);
+        a.Gm[vb * a.Wm + x] = (T)(acc * (Acc)a.scale);
     }
 }
 
-template <typename T>
+template <typename T, typename Acc>
 __global__ void k_bwd_node(const NodeArgs<T> a)
 {
     const int64_t ny = (int64_t)a.n_act * a.B * a.Wy;
     const int64_t nm = a.Tm ? (int64_t)a.n_act * a.B * a.Wm : 0;
     const int64_t lo = a.part == 2 ? ny : 0;        // part 2: the G_main items only
     const int64_t hi = a.part == 1 ? ny : ny + nm;  // part 1: the dY items only
-    FLAT_LOOP(i, hi - lo) bwd_node_item(a, lo + i, ny);
+    FLAT_LOOP(i, hi - lo) bwd_node_item<T, Acc>(a, lo + i, ny);
 }
 
 // The same values with the rows staged in LDS: one workgroup per group of
@@ -980,35 +986,35 @@ __global__ void k_bwd_node(const NodeArgs<T> a)
 // row value the entry names: row[r] (skip < 0), or the histogram row read
 // past the vertex's own color, row[r + (r >= skip)].  Entries are (o << 16 |
 // r).  The loads of 8 entries are issued before their terms are added.
-template <typename T, typename List>
-__device__ __forceinline__ double node_sum(List entry_at, int n, const T* G, double g0, const T* row, int skip)
+template <typename Acc, typename T, typename List>
+__device__ __forceinline__ Acc node_sum(List entry_at, int n, const T* G, Acc g0, const T* row, int skip)
 {
     constexpr int U = 8;
-    double acc = 0.0;
+    Acc acc = Acc(0);
     int p = 0;
     for (; p + U <= n; p += U) {
         uint32_t e[U];
 #pragma unroll
         for (int q = 0; q < U; ++q) e[q] = entry_at(p + q);
-        double gv[U], rv[U];
+        Acc gv[U], rv[U];
 #pragma unroll
         for (int q = 0; q < U; ++q) {
             const int r = (int)(e[q] & 0xffffu);
-            gv[q] = G ? (double)G[e[q] >> 16] : g0;
-            rv[q] = (double)row[r + (skip >= 0 && r >= skip ? 1 : 0)];
+            gv[q] = G ? (Acc)G[e[q] >> 16] : g0;
+            rv[q] = (Acc)row[r + (skip >= 0 && r >= skip ? 1 : 0)];
         }
 #pragma unroll
-        for (int q = 0; q < U; ++q) acc += gv[q] * rv[q];
+        for (int q = 0; q < U; ++q) acc = node_term(acc, gv[q], rv[q]);
     }
     for (; p < n; ++p) {
         const uint32_t e = entry_at(p);
         const int r = (int)(e & 0xffffu);
-        acc += (G ? (double)G[e >> 16] : g0) * (double)row[r + (skip >= 0 && r >= skip ? 1 : 0)];
+        acc = node_term(acc, G ? (Acc)G[e >> 16] : g0, (Acc)row[r + (skip >= 0 && r >= skip ? 1 : 0)]);
     }
     return acc;
 }
 
-template <typename T>
+template <typename T, typename Acc>
 __global__ __launch_bounds__(256) void k_bwd_node_lds(const NodeArgs<T> a, int vpb, int lists_lds)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -1047,30 +1053,32 @@ __global__ __launch_bounds__(256) void k_bwd_node_lds(const NodeArgs<T> a, int v
             const int64_t vb = vb0 + lv;
             const int b = (int)(vb % a.B);
             const T* G = wg ? sG + lv * wg : nullptr;
-            const double g0 = a.root_cot[b];
-            double acc = 0.0;
+            const Acc g0 = (Acc)a.root_cot[b];
+            Acc acc = Acc(0);
             if (x0 < ny) {
                 const int x = x0;
                 if (!a.Tm) {
-                    acc = G ? (double)G[x] : g0;
+                    acc = G ? (Acc)G[x] : g0;
                 } else {
                     const T* m = sM + lv * wm;
                     if (la)
-                        acc = node_sum<T>([&](int p) { return s_la[p * a.Wy + x]; }, a.Pa, G, g0, m, -1);
+                        acc = node_sum<Acc>([&](int p) { return s_la[p * a.Wy + x]; }, a.Pa, G, g0, m, -1);
                     else
-                        acc = node_sum<T>([&](int p) { return a.by_auxT[(size_t)p * a.Wy + x]; }, a.Pa, G, g0, m, -1);
+                        acc = node_sum<Acc>([&](int p) { return a.by_auxT[(size_t)p * a.Wy + x]; }, a.Pa, G, g0, m,
+                                            -1);
                 }
-                a.dY[vb * a.Wy + x] = (T)(acc * a.scale);
+                a.dY[vb * a.Wy + x] = (T)(acc * (Acc)a.scale);
             } else {
                 const int x = x0 - ny;
                 const int cv = a.colors[vb];
                 const T* y = sY + lv * wy;
                 const int skip = a.y_from_h ? cv : -1;
                 if (lm)
-                    acc = node_sum<T>([&](int p) { return s_lm[p * a.Wm + x]; }, a.Pm, G, g0, y, skip);
+                    acc = node_sum<Acc>([&](int p) { return s_lm[p * a.Wm + x]; }, a.Pm, G, g0, y, skip);
                 else
-                    acc = node_sum<T>([&](int p) { return a.by_mainT[(size_t)p * a.Wm + x]; }, a.Pm, G, g0, y, skip);
-                a.Gm[vb * a.Wm + x] = (T)(acc * a.scale);
+                    acc = node_sum<Acc>([&](int p) { return a.by_mainT[(size_t)p * a.Wm + x]; }, a.Pm, G, g0, y,
+                                        skip);
+                a.Gm[vb * a.Wm + x] = (T)(acc * (Acc)a.scale);
             }
         }
     }
@@ -1089,8 +1097,8 @@ constexpr int kBwdStageReuse = 6;
 constexpr size_t kBwdListsLds = 40 * 1024;
 constexpr size_t kBwdListsTrim = 8 * 1024;
 
-template <typename T>
-hipError_t launch_bwd_node(const NodeArgs<T>& a, int num_cus, hipStream_t st)
+template <typename T, typename Acc>
+hipError_t launch_bwd_node_t(const NodeArgs<T>& a, int num_cus, hipStream_t st)
 {
     const int64_t ny = (int64_t)a.n_act * a.B * a.Wy;
     const int64_t nm = a.Tm ? (int64_t)a.n_act * a.B * a.Wm : 0;
@@ -1103,7 +1111,7 @@ hipError_t launch_bwd_node(const NodeArgs<T>& a, int num_cus, hipStream_t st)
     // main tables have split lists)
     const int64_t reads = a.Tm ? (int64_t)a.Wo * a.P * ((do_dy ? 1 : 0) + (do_gm ? 1 : 0)) : 0;
     if (row > kBwdLdsMax || reads < (int64_t)kBwdStageReuse * (int64_t)(row / sizeof(T))) {
-        hipLaunchKernelGGL(k_bwd_node<T>, flat_grid(total), dim3(256), 0, st, a);
+        hipLaunchKernelGGL((k_bwd_node<T, Acc>), flat_grid(total), dim3(256), 0, st, a);
         return hipGetLastError();
     }
     const size_t lists = (size_t)4 * (((do_dy && a.Tm) ? (size_t)a.Pa * a.Wy : 0) + (do_gm ? (size_t)a.Pm * a.Wm : 0));
@@ -1118,8 +1126,18 @@ hipError_t launch_bwd_node(const NodeArgs<T>& a, int num_cus, hipStream_t st)
     const int vpb = vpb_for(lists_lds ? kBwdLdsBudget - kBwdListsTrim : kBwdLdsBudget);
     const int64_t ngroups = ((int64_t)a.n_act * a.B + vpb - 1) / vpb;
     const unsigned grid = (unsigned)std::min<int64_t>(ngroups, (int64_t)std::max(num_cus, 1) * 8);
-    hipLaunchKernelGGL(k_bwd_node_lds<T>, dim3(grid), dim3(256), head + (size_t)vpb * row, st, a, vpb, lists_lds);
+    hipLaunchKernelGGL((k_bwd_node_lds<T, Acc>), dim3(grid), dim3(256), head + (size_t)vpb * row, st, a, vpb,
+                       lists_lds);
     return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_bwd_node(const NodeArgs<T>& a, int num_cus, hipStream_t st)
+{
+    if constexpr (sizeof(T) == 4) {
+        if (a.sums32) return launch_bwd_node_t<T, float>(a, num_cus, st);
+    }
+    return launch_bwd_node_t<T, double>(a, num_cus, st);
 }
 
 template hipError_t launch_bwd_node<float>(const NodeArgs<float>&, int, hipStream_t);

@@ -180,6 +180,7 @@ hipError_t launch_runs(int mode, int main_mode, const SubArgs<T>& a, int num_cus
 // (hub, coloring, 64 Y values) walks the whole neighbour list with kHubBatch
 // row loads in flight per lane.  LDS: the rank table.
 constexpr int kHubBatch = 32;
+constexpr int kHubBatchNarrow = 56;  // FP32 rows of up to 64 values
 size_t hub_seq_lds_bytes(int K);
 template <typename T>
 hipError_t launch_hub_seq(const SubArgs<T>& a, int num_cus, int blocks_per_cu, hipStream_t st);
@@ -231,6 +232,7 @@ struct NodeArgs {
     const uint32_t* by_mainT;// [Pm][Wm]
     double scale;            // 1, or float(1e-12) for the scaled subtemplate
     int32_t part;            // 0: dY and G_main, 1: dY only, 2: G_main only
+    int32_t sums32;          // FP32 tables: 1 = split sums in FP32 (fused multiply-add), 0 = in FP64
 };
 template <typename T>
 hipError_t launch_bwd_node(const NodeArgs<T>& a, int num_cus, hipStream_t st);
