@@ -37,6 +37,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import re
 import subprocess
 import sys
 import tempfile
@@ -462,7 +463,11 @@ def plain_s20(args, sync):
     R-MAT s20/ef8 (seed 1), the 7-vertex tree, one coloring per step drawn
     on the device, fixed settings (the autotuner's choice for this shape),
     with its dominant kernel's roofline and the compiled reference on host
-    cores; FP64 tables and 64 colorings per pass beside it."""
+    cores; FP64 tables and 64 colorings per pass beside it.  The neighbour
+    sums run in the reordered (hub chunk, neighbour group) order: this
+    workload's FP32 counts stay within 2e-6 of the reference's (its golden
+    vectors, and the parity printed beside the CPU baseline), so the
+    reference's own summation order is not needed for the 1e-5 bar here."""
     import subgraph_amd as sg
     t0 = time.perf_counter()
     src, dst = sg.rmat_edges(20, 8, seed=1)
@@ -591,14 +596,30 @@ def main():
                          "first calls; 0 = fixed settings (TREE12_SETTINGS, plus --settings)")
     ap.add_argument("--settings", default="", metavar="KEY=VAL,...", help="primary line context options")
     ap.add_argument("--plain-settings",
-                    default="stage_rows=1,bucket_high=512,bucket_mid=64,hub_degree=512,hub_chunk=1024",
+                    default="stage_rows=1,bucket_high=512,bucket_mid=64,hub_degree=512,hub_chunk=1024,exact_order=0",
                     metavar="KEY=VAL,...", help="context options of the plain_s20 line")
     ap.add_argument("--only", default="", choices=("", "headline", "plain"),
                     help="run just that measurement (a short run for profilers); headline skips its comparisons")
+    ap.add_argument("--eager", action="store_true",
+                    help="with --only headline: run the steps eagerly with a HIP event pair around every launch "
+                         "(one stream), as the roofline figure is measured, for a kernel trace that matches it")
+    ap.add_argument("--summarize-stats", default="", metavar="CSV",
+                    help="print a rocprofv3 --stats kernel summary (kernel_stats.csv) as a markdown table by the "
+                         "library's kernel names, and exit")
+    ap.add_argument("--summarize-pmc", nargs=2, default=None, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="turn two rocprofv3 --pmc counter_collection.csv files (FETCH_SIZE, WRITE_SIZE; the first "
+                         "dispatch a --copy-calibration kernel) into the traffic JSON for --workload, and exit")
+    ap.add_argument("--workload", default=HEADLINE, help="workload name of --summarize-pmc")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run one elementwise kernel that reads and writes MIB MiB (the known-size "
                          "kernel rocprofv3 FETCH_SIZE / WRITE_SIZE readings are scaled by)")
     args = ap.parse_args()
+    if args.summarize_stats:
+        print(stats_table(args.summarize_stats))
+        return
+    if args.summarize_pmc:
+        print(json.dumps(pmc_traffic(args.summarize_pmc[0], args.summarize_pmc[1], args.workload), indent=1))
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print(json.dumps({"error": "this build measures one GPU: bench.py supports --gpus 1 only; "
                                    "colorings would shard across GPUs, each rank counting its share of the "
@@ -652,12 +673,104 @@ def headline_only(args, sync):
     plan = ctx.plan(12, TREE12)
     gbuf = torch.empty(len(rowidx), dtype=torch.float32, device="cuda")
     B = args.colorings
+    if args.eager:
+        ctx.set_option("timing", 1)
     for k in range(args.warmup + args.steps):
         t0 = time.perf_counter()
         ctx.count_grad_device(plan, B, gbuf.data_ptr(), seed=5000 + k)
         sync()
         log(f"step {k}: {B / (time.perf_counter() - t0):.2f} colorings/s ({ctx.get_option('last_tile')} per pass)")
-    return {"workload": HEADLINE, "settings": {k: ctx.get_option(k) for k in TUNED_KEYS}}
+    ctx.set_option("timing", 0)
+    return {"workload": HEADLINE, "eager": bool(args.eager), "settings": {k: ctx.get_option(k) for k in TUNED_KEYS}}
+
+
+def record_name(symbol):
+    """The library's timing name of a kernel symbol as rocprofv3 prints it
+    (pg_capi.hip names launches by kernel family and the template arguments
+    that matter: k_sub's values per lane only when 1, k_hub_seq without its
+    batch depth, the colour-run kernels by mode and main-table kind, the node
+    kernels' variants as one)."""
+    s = symbol.replace("(anonymous namespace)::", "").split("(")[0]
+    for junk in ("void ", "pgbsc::", " "):
+        s = s.replace(junk, "")
+    m = re.match(r"(k_\w+)<(.*)>$", s)
+    if not m:
+        return s
+    name, a = m.group(1), m.group(2).split(",")
+    if name == "k_sub" and len(a) == 5:
+        return "k_sub<" + ",".join(a[:4] + ([] if a[4] == "4" else ["1"])) + ">"
+    if name == "k_hub_seq":
+        return f"k_hub_seq<{a[0]}>"
+    if name == "k_runs" and len(a) == 5:
+        return f"k_runs<{a[0]},{ {'0': 'fwd', '1': 't'}.get(a[3], a[3]) },{a[4]}>"
+    if name in ("k_runs_g", "k_runs_tg"):
+        return f"k_runs<{a[0]},{name[7:]},1>"
+    if name == "k_runs_seq" and len(a) == 4:
+        return f"k_runs<{a[0]},seq,{a[3]}>"
+    if name in ("k_bwd_node", "k_bwd_node_lds", "k_bwd_node_rows"):
+        return f"k_bwd_node<{a[0]}>"
+    if name in ("k_edge_grad", "k_fascia_step"):
+        return f"{name}<{a[0]}>"
+    return s
+
+
+def stats_table(path):
+    """A rocprofv3 kernel_stats.csv as markdown rows by record_name(),
+    largest total first: | `name` | calls | avg us | total ms | share |."""
+    import csv
+    agg = {}
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            name = record_name(r["Name"])
+            a = agg.setdefault(name, [0, 0.0])
+            a[0] += int(r["Calls"])
+            a[1] += float(r["TotalDurationNs"])
+    tot = sum(v[1] for v in agg.values())
+    lines = ["| kernel | calls | avg us | total ms | share |", "|---|---|---|---|---|"]
+    for name, (calls, ns) in sorted(agg.items(), key=lambda kv: -kv[1][1]):
+        lines.append(f"| `{name}` | {calls} | {ns / calls / 1e3:.1f} | {ns / 1e6:.3f} | {100 * ns / tot:.1f}% |")
+    return "\n".join(lines)
+
+
+def pmc_traffic(fetch_csv, write_csv, workload):
+    """HBM bytes per launch by record_name() from one FETCH_SIZE and one
+    WRITE_SIZE rocprofv3 --pmc run of the same command, each counter scaled
+    by what it reads for the run's first dispatch, a --copy-calibration
+    kernel that reads and writes a known number of bytes."""
+    import csv
+    runs = []
+    for path in (fetch_csv, write_csv):
+        with open(path) as f:
+            rows = sorted(csv.DictReader(f), key=lambda r: int(r["Dispatch_Id"]))
+        runs.append(rows)
+    fetch, write = runs
+    cal = [i for i, r in enumerate(fetch) if "add" in r["Kernel_Name"] and "elementwise" in r["Kernel_Name"]]
+    if not cal:
+        raise SystemExit("no copy-calibration dispatch in the FETCH_SIZE run")
+    ci = cal[0]
+    mib_kib = float(fetch[ci]["Grid_Size"]) * 4 / 1024  # one float per work-item, read and written
+    ff = mib_kib / float(fetch[ci]["Counter_Value"])
+    wf = mib_kib / float(write[ci]["Counter_Value"])
+    agg = {}
+    for fr, wr in zip(fetch[ci + 1:], write[ci + 1:]):
+        if fr["Kernel_Name"] != wr["Kernel_Name"]:
+            raise SystemExit("the two counter runs dispatched different kernels")
+        a = agg.setdefault(record_name(fr["Kernel_Name"]), [0, 0.0, 0.0])
+        a[0] += 1
+        a[1] += float(fr["Counter_Value"])
+        a[2] += float(wr["Counter_Value"])
+    kernels = {}
+    for name, (n, fk, wk) in sorted(agg.items(), key=lambda kv: -(kv[1][1] * ff + kv[1][2] * wf)):
+        kernels[name] = {"fetch_kib_per_launch": round(fk / n, 1), "write_kib_per_launch": round(wk / n, 1),
+                         "hbm_bytes_per_launch": int((fk * ff + wk * wf) * 1024 / n), "launches_in_run": n}
+    return {"workload": workload,
+            "calibration": {"kernel": f"torch.add of a float32 tensor of {mib_kib / 1024:.0f} MiB (reads it, "
+                                      "writes as much)",
+                            "fetch_kib": float(fetch[ci]["Counter_Value"]), "write_kib": float(write[ci]["Counter_Value"]),
+                            "fetch_factor": round(ff, 4), "write_factor": round(wf, 4)},
+            "note": "hbm_bytes_per_launch = (FETCH_SIZE * fetch_factor + WRITE_SIZE * write_factor) KiB * 1024, "
+                    "averaged over every launch of that kernel name in the run",
+            "kernels": kernels}
 
 
 if __name__ == "__main__":

@@ -158,6 +158,7 @@ struct pg_ctx {
     int64_t hub_seq = 1;        // exact order: hub rows by k_hub_seq (1) or by column parts of k_sub (0)
     int64_t fuse_grad = 1;      // adjoint: forward Y kept and the gradient taken in the transposed gather
     int64_t node_fp32 = 1;      // adjoint node sums of FP32 tables: 1 FP32 fused multiply-adds, 0 FP64
+    int64_t node_rows = 1;      // adjoint node launches: a thread per value over a group's rows (1) or per (row, value)
     int64_t reg_cols = 0;       // values per lane of gathers: 0 one for rows under kRegCols, else 1 or 4
     int64_t adjoint_recompute = -1;  // adjoint tables: 0 kept, 1 recomputed, -1 recomputed when that fits 4x more
     int64_t vertex_order = 0;   // internal order at the next upload: 0 descending degree, 1 degree classes + BFS
@@ -1796,6 +1797,7 @@ int enqueue_sub_backward(pg_ctx* c, const pg_plan* p, const DevSub& d, int B, co
     na.P = d.P;
     na.scale = (d.scaled && fp32) ? (double)1.0e-12f : 1.0;
     na.sums32 = fp32 && c->node_fp32 ? 1 : 0;
+    na.rows_inner = c->node_rows ? 1 : 0;
     const std::string node_name = std::string("k_bwd_node<") + (fp32 ? "float>" : "double>");
     // Algorithmic bytes, in the forward's units (full-width FP32 columns of
     // n_act x B values; Y and dY are as wide as the aux child's table; the
@@ -2510,6 +2512,12 @@ extern "C" int pg_set_option(pg_ctx* c, const char* key, int64_t v)
             drop_graphs(c);
             c->exact_order = v;
         }
+    } else if (k == "node_rows") {
+        if (v != 0 && v != 1) return set_err(c, PG_ERR_INVALID, "node_rows must be 0 or 1");
+        if (v != c->node_rows) {
+            drop_graphs(c);
+            c->node_rows = v;
+        }
     } else if (k == "node_fp32") {
         if (v != 0 && v != 1) return set_err(c, PG_ERR_INVALID, "node_fp32 must be 0 or 1");
         if (v != c->node_fp32) {
@@ -2580,6 +2588,7 @@ extern "C" int pg_get_option(const pg_ctx* c, const char* key, int64_t* v)
     else if (k == "hub_concurrent") *v = c->hub_concurrent;
     else if (k == "fuse_grad") *v = c->fuse_grad;
     else if (k == "node_fp32") *v = c->node_fp32;
+    else if (k == "node_rows") *v = c->node_rows;
     else if (k == "reg_cols") *v = c->reg_cols;
     else if (k == "adjoint_recompute") *v = c->adjoint_recompute;
     else if (k == "vertex_order") *v = c->vertex_order;

@@ -32,6 +32,8 @@ __device__ __forceinline__ uint32_t hub_squeeze_bit(uint32_t m, uint32_t c)
     return (m & ((1u << c) - 1u)) | ((m >> (c + 1u)) << c);
 }
 
+__host__ __device__ constexpr size_t hub_rank_bytes(int K) { return (((size_t)2 << K) + 15) & ~(size_t)15; }
+
 template <typename T>
 __device__ __forceinline__ T lane_value(T x, int j);
 
@@ -64,6 +66,9 @@ __global__ __launch_bounds__(kBlock) void k_hub_seq(const SubArgs<T> a)
     constexpr int WAVES = kBlock / 64;
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    // per wavefront: for each neighbour colour cu, the row index the lane's
+    // Y value reads (-1: none), [cu][lane]
+    int16_t* s_tab = reinterpret_cast<int16_t*>(smem_raw + hub_rank_bytes(a.K)) + (size_t)wave * kMaxK * 64;
     const int B = a.B, Wy = a.Wy;
     const int cpb = (Wy + 63) >> 6;
     const int64_t per_hub = (int64_t)B * cpb;
@@ -77,6 +82,12 @@ __global__ __launch_bounds__(kBlock) void k_hub_seq(const SubArgs<T> a)
         const uint32_t cv = a.colors[(int64_t)v * B + b];
         // a lane past the row's end has the empty set: it adds nothing
         const uint32_t amask = i < Wy ? hub_expand_bit(a.ymask[i], cv) : 0u;
+        // the lane's index for each neighbour colour, once per work unit (a
+        // lane writes and reads only its own column: no barrier)
+        for (int cu = 0; cu < a.k; ++cu)
+            s_tab[cu * 64 + lane] = (amask >> cu) & 1u
+                                        ? s_rank[hub_squeeze_bit(amask & ~(1u << (uint32_t)cu), (uint32_t)cu)]
+                                        : (int16_t)-1;
         const int e0 = a.rowptr[v], e1 = a.rowptr[v + 1];
         const T* rows = a.aux + (int64_t)b * a.Wa;
         T acc = T(0);
@@ -95,15 +106,13 @@ __global__ __launch_bounds__(kBlock) void k_hub_seq(const SubArgs<T> a)
         };
         if (e0 < e1) meta(e0);
         for (int e = e0; e < e1; e += NB) {
-            // ranks first, without branches: (set - cu) squeezed at cu is a
-            // K-bit mask for any colour cu < k, so every lane reads the table;
-            // -1 marks "cu not in the set"
+            // the batch's row indices from the lane's table (-1: cu is not
+            // in the set)
             int idx[NB];
 #pragma unroll
             for (int j = 0; j < NB; ++j) {
                 const uint32_t cu = (uint32_t)__builtin_amdgcn_readlane((int)cm, j);
-                const int r = s_rank[hub_squeeze_bit(amask & ~(1u << cu), cu)];
-                idx[j] = (amask >> cu) & 1u ? r : -1;
+                idx[j] = s_tab[cu * 64 + lane];
             }
             T x[NB];
 #pragma unroll
@@ -159,7 +168,7 @@ hipError_t launch_hub_seq_t(const SubArgs<T>& a, int num_cus, int blocks_per_cu,
 
 }  // namespace
 
-size_t hub_seq_lds_bytes(int K) { return (((size_t)2 << K) + 15) & ~(size_t)15; }
+size_t hub_seq_lds_bytes(int K) { return hub_rank_bytes(K) + (size_t)(kBlock / 64) * kMaxK * 64 * sizeof(int16_t); }
 
 template <typename T>
 hipError_t launch_hub_seq(const SubArgs<T>& a, int num_cus, int blocks_per_cu, hipStream_t st)

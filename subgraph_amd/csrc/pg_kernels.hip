@@ -1084,6 +1084,83 @@ __global__ __launch_bounds__(256) void k_bwd_node_lds(const NodeArgs<T> a, int v
     }
 }
 
+// The staged kernel with the loops the other way round: a thread takes one
+// value index for every staged row of the group, so each split-list entry is
+// read once per group instead of once per row, and the group's rows share it
+// (two LDS reads per term instead of three).  Per row the same sum, in the
+// same order, as k_bwd_node_lds.  vpb <= kNodeRows; steps below the root
+// (a G table) whose Y is not the histogram.
+constexpr int kNodeRows = 16;
+
+template <typename T, typename Acc>
+__global__ __launch_bounds__(256) void k_bwd_node_rows(const NodeArgs<T> a, int vpb, int lists_lds)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const bool do_dy = a.part != 2;
+    const bool do_gm = a.part != 1 && a.Tm != nullptr;
+    const int wg = a.G ? a.Wo : 0;
+    const int wm = (do_dy && a.Tm) ? a.Wm : 0;
+    const int yw = a.y_from_h ? a.k : a.Wy;
+    const int wy = do_gm ? yw : 0;
+    const int la = lists_lds && do_dy && a.Tm ? a.Pa * a.Wy : 0;
+    const int lm = lists_lds && do_gm ? a.Pm * a.Wm : 0;
+    uint32_t* s_la = reinterpret_cast<uint32_t*>(smem_raw);
+    uint32_t* s_lm = s_la + la;
+    for (int i = threadIdx.x; i < la; i += blockDim.x) s_la[i] = a.by_auxT[i];
+    for (int i = threadIdx.x; i < lm; i += blockDim.x) s_lm[i] = a.by_mainT[i];
+    T* sG = reinterpret_cast<T*>(smem_raw + (((size_t)4 * (la + lm) + 15) & ~(size_t)15));  // [vpb][wg]
+    T* sM = sG + (size_t)vpb * wg;           // [vpb][wm]
+    T* sY = sM + (size_t)vpb * wm;           // [vpb][wy]
+    const int ny = do_dy ? a.Wy : 0;
+    const int per = ny + (do_gm ? a.Wm : 0);
+    const int64_t nvb = (int64_t)a.n_act * a.B;
+    const int64_t ngroups = (nvb + vpb - 1) / vpb;
+    for (int64_t g = blockIdx.x; g < ngroups; g += gridDim.x) {
+        const int64_t vb0 = g * vpb;
+        const int nv = (int)(nvb - vb0 < vpb ? nvb - vb0 : vpb);
+        __syncthreads();
+        for (int i = threadIdx.x; i < nv * wg; i += blockDim.x) sG[i] = a.G[vb0 * wg + i];
+        for (int i = threadIdx.x; i < nv * wm; i += blockDim.x) sM[i] = a.Tm[vb0 * wm + i];
+        for (int i = threadIdx.x; i < nv * wy; i += blockDim.x) sY[i] = a.Y[vb0 * yw + i];
+        __syncthreads();
+        for (int x0 = threadIdx.x; x0 < per; x0 += blockDim.x) {
+            const bool is_dy = x0 < ny;
+            const int x = is_dy ? x0 : x0 - ny;
+            Acc acc[kNodeRows];
+#pragma unroll
+            for (int r = 0; r < kNodeRows; ++r) acc[r] = Acc(0);
+            if (is_dy && !a.Tm) {
+#pragma unroll
+                for (int r = 0; r < kNodeRows; ++r)
+                    if (r < nv) acc[r] = (Acc)sG[r * wg + x];
+            } else {
+                const int n = is_dy ? a.Pa : a.Pm;
+                const int stride = is_dy ? a.Wy : a.Wm;
+                const uint32_t* lst = is_dy ? (la ? s_la : a.by_auxT) : (lm ? s_lm : a.by_mainT);
+                const T* rows = is_dy ? sM : sY;
+                const int rw = is_dy ? wm : wy;
+                for (int p = 0; p < n; ++p) {
+                    const uint32_t e = lst[(size_t)p * stride + x];
+                    const T* gp = sG + (int)(e >> 16);
+                    const T* rp = rows + (int)(e & 0xffffu);
+#pragma unroll
+                    for (int r = 0; r < kNodeRows; ++r) {
+                        if (r >= nv) break;
+                        acc[r] = node_term(acc[r], (Acc)gp[r * wg], (Acc)rp[r * rw]);
+                    }
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < kNodeRows; ++r) {
+                if (r >= nv) break;
+                const int64_t vb = vb0 + r;
+                if (is_dy) a.dY[vb * a.Wy + x] = (T)(acc[r] * (Acc)a.scale);
+                else a.Gm[vb * a.Wm + x] = (T)(acc[r] * (Acc)a.scale);
+            }
+        }
+    }
+}
+
 // LDS per workgroup of the staged kernel, the largest row set it takes, and
 // the split reads per staged value from which it pays.  Measured in FP32,
 // forward+backward with every main-table launch staged: the 12-vertex tree
@@ -1123,9 +1200,21 @@ hipError_t launch_bwd_node_t(const NodeArgs<T>& a, int num_cus, hipStream_t st)
     const int lists_lds = lists > 0 && lists <= kBwdListsLds &&
                           lists_head + (size_t)vpb_for(kBwdLdsBudget - kBwdListsTrim) * row <= 64 * 1024;
     const size_t head = lists_lds ? lists_head : 0;
-    const int vpb = vpb_for(lists_lds ? kBwdLdsBudget - kBwdListsTrim : kBwdLdsBudget);
+    int vpb = vpb_for(lists_lds ? kBwdLdsBudget - kBwdListsTrim : kBwdLdsBudget);
+    // rows inner only where a row's values fill a workgroup: with fewer, the
+    // per (row, value) threads keep more of them busy (the (6;2,4) step of
+    // the 12-vertex tree, 176 values of 14-210 terms: 272 ms as against 458
+    // ms rows inner; the (10;7,3) step, 627 values: 213 as against 283)
+    const int per_row = (do_dy ? a.Wy : 0) + (do_gm ? a.Wm : 0);
+    const bool rows_inner = a.rows_inner && a.G && !a.y_from_h && per_row >= 256;
+    if (rows_inner) vpb = std::min(vpb, kNodeRows);
     const int64_t ngroups = ((int64_t)a.n_act * a.B + vpb - 1) / vpb;
     const unsigned grid = (unsigned)std::min<int64_t>(ngroups, (int64_t)std::max(num_cus, 1) * 8);
+    if (rows_inner) {
+        hipLaunchKernelGGL((k_bwd_node_rows<T, Acc>), dim3(grid), dim3(256), head + (size_t)vpb * row, st, a, vpb,
+                           lists_lds);
+        return hipGetLastError();
+    }
     hipLaunchKernelGGL((k_bwd_node_lds<T, Acc>), dim3(grid), dim3(256), head + (size_t)vpb * row, st, a, vpb,
                        lists_lds);
     return hipGetLastError();

@@ -233,6 +233,7 @@ struct NodeArgs {
     double scale;            // 1, or float(1e-12) for the scaled subtemplate
     int32_t part;            // 0: dY and G_main, 1: dY only, 2: G_main only
     int32_t sums32;          // FP32 tables: 1 = split sums in FP32 (fused multiply-add), 0 = in FP64
+    int32_t rows_inner;      // staged launches: a thread per value index over the group's rows
 };
 template <typename T>
 hipError_t launch_bwd_node(const NodeArgs<T>& a, int num_cus, hipStream_t st);

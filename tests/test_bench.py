@@ -17,8 +17,9 @@ import bench  # noqa: E402
 LINE = os.path.join(ROOT, "profiles", "bench_default.json")
 
 
-def trace_md(workload):
-    return os.path.join(ROOT, "profiles", f"k_sub_{workload}.md")
+def trace_md(kernel, workload):
+    """The kernel-trace summary named for the dominant kernel's family and the workload."""
+    return os.path.join(ROOT, "profiles", f"{kernel.split('<')[0]}_{workload}.md")
 
 
 def test_more_than_one_gpu_is_refused():
@@ -26,6 +27,23 @@ def test_more_than_one_gpu_is_refused():
                          text=True, timeout=120)
     assert out.returncode == 2
     assert "one GPU" in json.loads(out.stdout.strip().splitlines()[-1])["error"]
+
+
+def test_record_names_of_kernel_symbols():
+    """rocprofv3's kernel symbols map to the library's timing names (the
+    trace summaries and traffic files are keyed by them)."""
+    cases = {
+        "void pgbsc::(anonymous namespace)::k_sub<float, 64, 0, 1, 4>(pgbsc::SubArgs<float>)": "k_sub<float,64,0,1>",
+        "void pgbsc::(anonymous namespace)::k_sub<float, 8, 0, 1, 1>(pgbsc::SubArgs<float>)": "k_sub<float,8,0,1,1>",
+        "void pgbsc::(anonymous namespace)::k_hub_seq<float, 56>(pgbsc::SubArgs<float>)": "k_hub_seq<float>",
+        "void pgbsc::(anonymous namespace)::k_runs<float, 8, 4, 1, 1>(pgbsc::SubArgs<float>)": "k_runs<float,t,1>",
+        "void pgbsc::(anonymous namespace)::k_runs_tg<float, 8, 4>(pgbsc::SubArgs<float>)": "k_runs<float,tg,1>",
+        "void pgbsc::(anonymous namespace)::k_runs_g<float, 1, 8>(pgbsc::SubArgs<float>)": "k_runs<float,g,1>",
+        "void pgbsc::(anonymous namespace)::k_runs_seq<float, 1, 8, 0>(pgbsc::SubArgs<float>)": "k_runs<float,seq,0>",
+        "void pgbsc::k_bwd_node_rows<float, float>(pgbsc::NodeArgs<float>, int, int)": "k_bwd_node<float>",
+    }
+    for sym, want in cases.items():
+        assert bench.record_name(sym) == want, sym
 
 
 def test_hashed_weights_symmetric_and_in_range():
@@ -69,11 +87,11 @@ def test_committed_line_keeps_the_contract(line):
 
 
 def test_trace_summary_agrees_with_the_line(line):
-    """The committed rocprofv3 kernel-trace summary of the plain workload and
+    """The committed rocprofv3 kernel-trace summary of the line's workload and
     the line's live HIP-event figure name the same dominant kernel, and its
     average duration agrees within 5%."""
     kern = line["roofline"]["kernel"]
-    text = open(trace_md(line["config"]["workload"])).read()
+    text = open(trace_md(kern, line["config"]["workload"])).read()
     rows = [ln for ln in text.splitlines() if ln.startswith(f"| `{kern}` |")]
     assert len(rows) == 1, kern
     avg_us = float(rows[0].split("|")[3])

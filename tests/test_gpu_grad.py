@@ -371,7 +371,9 @@ def test_autotuned_gradient(gpu_ctx):
                     print(f"[tuned-grad] k={tv} weighted={weighted} mode={mode}", flush=True)
                     for rep in range(2):  # tuning, then the tuned settings recalled
                         r1, g1 = gpu_ctx.count_grad(plan, 4, seed=tv, precision="fp64")
-                        assert 1 <= gpu_ctx.get_option("last_tile") <= tile0
+                        # the tuned settings change the LDS a pass needs, so the widest
+                        # tile that fits can differ from the untuned call's
+                        assert 1 <= gpu_ctx.get_option("last_tile") <= 4
                         if not weighted and np.abs(g0).max() < 2.0 ** 50:
                             assert np.array_equal(r1, r0) and np.array_equal(g1, g0)
                         else:
