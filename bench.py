@@ -283,10 +283,17 @@ def s22_graph():
     return n, int(len(src)), colptr, rowidx
 
 
-# the 12-vertex tree's settings when the autotuner is off (--autotune 0): what
-# it settles on for this template at s22 (every gather of this template runs
-# 64-lane teams, so the bucket thresholds matter little)
-TREE12_SETTINGS = {"stage_rows": 1, "bucket_high": 32, "bucket_mid": 8, "hub_degree": 512, "hub_chunk": 1024}
+# the 12-vertex tree's fixed settings (--autotune 0, the default).  Measured
+# at s22, forward+backward of 16 colorings replayed from captured graphs:
+# 4 colorings per pass 4.85-4.89 s, 2 per pass 4.95-4.98 s, 3 per pass
+# 5.07-5.09 s (five passes of 3 and a remainder pass of 1); hubs from degree
+# 512, 1024 or 2048 within 1%.  The autotuner settles on 3 per pass (it
+# times forward passes per coloring, not the call's remainder), so the line
+# fixes the tile.  In the reference's summation order every forward gather
+# of this template runs teams of its own width, so the bucket thresholds
+# matter little.
+TREE12_SETTINGS = {"stage_rows": 1, "bucket_high": 32, "bucket_mid": 8, "hub_degree": 2048, "hub_chunk": 1024,
+                   "batch_tile": 4}
 HEADLINE = "tree12_fwd_bwd_rmat_s22_ef16"
 
 
@@ -418,6 +425,7 @@ def s22_batching(ctx, graph, sync):
     n, colptr, rowidx = graph
     gold = golden_s22()
     out = {"graph": "R-MAT scale 22 edge factor 16 (the headline graph)", "dtype": "fp32", "templates": {}}
+    ctx.set_option("batch_tile", 0)  # the headline's fixed tile off: the tuner picks each template's
     ctx.set_option("autotune", 1)
     plans = {}
     for name, nv, te, reps in (("u5", 5, U5, 3), ("tree7", 7, TREE7, 2), ("tree10", 10, TREE10, 1)):
@@ -591,9 +599,9 @@ def main():
     ap.add_argument("--variants", default="plain,s22,grad",
                     help="comma list of extra measurements: plain (s20 plain method), s22 (batching), "
                          "grad (s18 differentiation), none")
-    ap.add_argument("--autotune", type=int, default=1, choices=(0, 1),
-                    help="primary line: 1 = the autotuner picks settings and colorings per pass in the untimed "
-                         "first calls; 0 = fixed settings (TREE12_SETTINGS, plus --settings)")
+    ap.add_argument("--autotune", type=int, default=0, choices=(0, 1),
+                    help="primary line: 0 = fixed settings (TREE12_SETTINGS, plus --settings); 1 = the autotuner "
+                         "picks settings and colorings per pass in the untimed first calls")
     ap.add_argument("--settings", default="", metavar="KEY=VAL,...", help="primary line context options")
     ap.add_argument("--plain-settings",
                     default="stage_rows=1,bucket_high=512,bucket_mid=64,hub_degree=512,hub_chunk=1024,exact_order=0",
@@ -607,8 +615,9 @@ def main():
                     help="print a rocprofv3 --stats kernel summary (kernel_stats.csv) as a markdown table by the "
                          "library's kernel names, and exit")
     ap.add_argument("--summarize-pmc", nargs=2, default=None, metavar=("FETCH_CSV", "WRITE_CSV"),
-                    help="turn two rocprofv3 --pmc counter_collection.csv files (FETCH_SIZE, WRITE_SIZE; the first "
-                         "dispatch a --copy-calibration kernel) into the traffic JSON for --workload, and exit")
+                    help="turn two rocprofv3 --pmc counter_collection.csv files (FETCH_SIZE, WRITE_SIZE; runs with "
+                         "--copy-calibration MIB, the same MIB given here) into the traffic JSON for --workload, "
+                         "and exit")
     ap.add_argument("--workload", default=HEADLINE, help="workload name of --summarize-pmc")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run one elementwise kernel that reads and writes MIB MiB (the known-size "
@@ -618,7 +627,10 @@ def main():
         print(stats_table(args.summarize_stats))
         return
     if args.summarize_pmc:
-        print(json.dumps(pmc_traffic(args.summarize_pmc[0], args.summarize_pmc[1], args.workload), indent=1))
+        if args.copy_calibration <= 0:
+            raise SystemExit("--summarize-pmc needs --copy-calibration MIB, the size the counter runs used")
+        print(json.dumps(pmc_traffic(args.summarize_pmc[0], args.summarize_pmc[1], args.workload,
+                                     args.copy_calibration), indent=1))
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print(json.dumps({"error": "this build measures one GPU: bench.py supports --gpus 1 only; "
@@ -732,11 +744,11 @@ def stats_table(path):
     return "\n".join(lines)
 
 
-def pmc_traffic(fetch_csv, write_csv, workload):
+def pmc_traffic(fetch_csv, write_csv, workload, mib):
     """HBM bytes per launch by record_name() from one FETCH_SIZE and one
     WRITE_SIZE rocprofv3 --pmc run of the same command, each counter scaled
-    by what it reads for the run's first dispatch, a --copy-calibration
-    kernel that reads and writes a known number of bytes."""
+    by what it reads for the --copy-calibration kernel of that command, which
+    reads and writes `mib` MiB."""
     import csv
     runs = []
     for path in (fetch_csv, write_csv):
@@ -748,7 +760,7 @@ def pmc_traffic(fetch_csv, write_csv, workload):
     if not cal:
         raise SystemExit("no copy-calibration dispatch in the FETCH_SIZE run")
     ci = cal[0]
-    mib_kib = float(fetch[ci]["Grid_Size"]) * 4 / 1024  # one float per work-item, read and written
+    mib_kib = float(mib) * 1024
     ff = mib_kib / float(fetch[ci]["Counter_Value"])
     wf = mib_kib / float(write[ci]["Counter_Value"])
     agg = {}

@@ -158,6 +158,7 @@ struct pg_ctx {
     int64_t hub_seq = 1;        // exact order: hub rows by k_hub_seq (1) or by column parts of k_sub (0)
     int64_t fuse_grad = 1;      // adjoint: forward Y kept and the gradient taken in the transposed gather
     int64_t node_fp32 = 1;      // adjoint node sums of FP32 tables: 1 FP32 fused multiply-adds, 0 FP64
+    int64_t runs_g_min = 128;   // gradient gathers by colour runs from this row width (color_runs -1)
     int64_t node_rows = 1;      // adjoint node launches: a thread per value over a group's rows (1) or per (row, value)
     int64_t reg_cols = 0;       // values per lane of gathers: 0 one for rows under kRegCols, else 1 or 4
     int64_t adjoint_recompute = -1;  // adjoint tables: 0 kept, 1 recomputed, -1 recomputed when that fits 4x more
@@ -615,7 +616,7 @@ int runs_mode_for(const pg_ctx* c, int aux, int wg, int used_per_mille)
     if (c->color_runs < 0 && (wg < kRunsMinWidth || used_per_mille < 500)) return -1;
     if (aux == AUX_GATHER || aux == AUX_ROWS || aux == AUX_STAGED) return c->exact_order ? -1 : RUNS_FWD;
     if (aux == AUX_GATHER_T) return RUNS_T;
-    if (aux == AUX_GATHER_G) return RUNS_G;
+    if (aux == AUX_GATHER_G) return c->color_runs > 0 || wg >= c->runs_g_min ? RUNS_G : -1;
     return -1;
 }
 
@@ -2512,6 +2513,12 @@ extern "C" int pg_set_option(pg_ctx* c, const char* key, int64_t v)
             drop_graphs(c);
             c->exact_order = v;
         }
+    } else if (k == "runs_g_min") {
+        if (v < 1 || v > 4096) return set_err(c, PG_ERR_INVALID, "runs_g_min must be in 1..4096");
+        if (v != c->runs_g_min) {
+            drop_graphs(c);
+            c->runs_g_min = v;
+        }
     } else if (k == "node_rows") {
         if (v != 0 && v != 1) return set_err(c, PG_ERR_INVALID, "node_rows must be 0 or 1");
         if (v != c->node_rows) {
@@ -2589,6 +2596,7 @@ extern "C" int pg_get_option(const pg_ctx* c, const char* key, int64_t* v)
     else if (k == "fuse_grad") *v = c->fuse_grad;
     else if (k == "node_fp32") *v = c->node_fp32;
     else if (k == "node_rows") *v = c->node_rows;
+    else if (k == "runs_g_min") *v = c->runs_g_min;
     else if (k == "reg_cols") *v = c->reg_cols;
     else if (k == "adjoint_recompute") *v = c->adjoint_recompute;
     else if (k == "vertex_order") *v = c->vertex_order;
