@@ -100,6 +100,8 @@ const char* pg_last_error(const pg_ctx* ctx);
  *   "fuse_grad"    pg_count_grad with kept tables: 1 = the forward keeps Y of the steps whose
  *                  transposed gather runs by colour runs, and that gather also takes the
  *                  entries' gradients (default); 0 = a gradient gather recomputes Y
+ *   "fuse_min_width" with fuse_grad: also fuse steps whose dY rows have at least this many
+ *                  values, by colour runs whatever share of a row a vertex uses (default 128)
  *   "node_fp32"    pg_count_grad with FP32 tables: 1 = the adjoint's split sums in FP32
  *                  (fused multiply-adds, default), 0 = in FP64
  *   "node_rows"    pg_count_grad: 1 = staged node launches over rows of 256+ values take a

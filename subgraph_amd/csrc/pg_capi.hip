@@ -159,6 +159,7 @@ struct pg_ctx {
     int64_t fuse_grad = 1;      // adjoint: forward Y kept and the gradient taken in the transposed gather
     int64_t node_fp32 = 1;      // adjoint node sums of FP32 tables: 1 FP32 fused multiply-adds, 0 FP64
     int64_t runs_g_min = 128;   // gradient gathers by colour runs from this row width (color_runs -1)
+    int64_t fuse_min_width = 128;   // fused gradient also for dY rows of at least this many values
     int64_t node_rows = 1;      // adjoint node launches: a thread per value over a group's rows (1) or per (row, value)
     int64_t reg_cols = 0;       // values per lane of gathers: 0 one for rows under kRegCols, else 1 or 4
     int64_t adjoint_recompute = -1;  // adjoint tables: 0 kept, 1 recomputed, -1 recomputed when that fits 4x more
@@ -751,10 +752,12 @@ int launch_gathering(pg_ctx* c, int aux, int main_mode, SubArgs<T> a, T* part, i
                      int used_per_mille = 0)
 {
     {
-        int rmode = runs_mode_for(c, aux, a.Wa, used_per_mille);
-        if (rmode == RUNS_T && a.tr) rmode = RUNS_TG;
+        if (a.tr) {  // the fused transposed gather and gradient (fused_grad_step)
+            if (runs_values_per_lane(a.Wa) == 0) return set_err(c, PG_ERR_STATE, "fused gradient gather too wide");
+            return launch_gathering_runs<T>(c, RUNS_TG, main_mode, a, part, y_cols, bytes);
+        }
+        const int rmode = runs_mode_for(c, aux, a.Wa, used_per_mille);
         if (rmode >= 0) return launch_gathering_runs<T>(c, rmode, main_mode, a, part, y_cols, bytes);
-        if (a.tr) return set_err(c, PG_ERR_STATE, "fused gradient gather without colour runs");
     }
     if (c->exact_order && (aux == AUX_GATHER || aux == AUX_STAGED))
         return launch_gathering_exact<T>(c, AUX_GATHER, main_mode, a, part, y_cols, bytes,
@@ -1710,12 +1713,16 @@ size_t table_bytes(const pg_ctx* c, const DevSub& d, int B, size_t tsz)
 
 // Whether the adjoint of step d takes its entries' gradients in the
 // transposed gather (RUNS_TG, from the forward's kept Y) instead of a
-// gradient gather that recomputes Y: when the transposed gather of d runs
-// by colour runs, with option fuse_grad on.
+// gradient gather that recomputes Y, with option fuse_grad on: when the
+// transposed gather of d runs by colour runs, or its dY rows have at least
+// fuse_min_width values (then by colour runs too, whatever share of a row a
+// vertex uses: one whole-row read per entry replaces the gradient gather's
+// read of the aux rows and the transposed gather's partial one).
 bool fused_grad_step(const pg_ctx* c, const pg_plan* p, const DevSub& d)
 {
-    if (!c->fuse_grad || d.aa <= 1) return false;
-    return runs_mode_for(c, AUX_GATHER_T, d.Wy, runs_used_per_mille(p, d, AUX_GATHER_T)) == RUNS_T;
+    if (!c->fuse_grad || d.aa <= 1 || runs_values_per_lane(d.Wy) == 0) return false;
+    return runs_mode_for(c, AUX_GATHER_T, d.Wy, runs_used_per_mille(p, d, AUX_GATHER_T)) == RUNS_T ||
+           (c->color_runs != 0 && d.Wy >= c->fuse_min_width);
 }
 
 GradPlan plan_memory_grad(const pg_ctx* c, const pg_plan* p, int B, size_t tsz)
@@ -2513,6 +2520,12 @@ extern "C" int pg_set_option(pg_ctx* c, const char* key, int64_t v)
             drop_graphs(c);
             c->exact_order = v;
         }
+    } else if (k == "fuse_min_width") {
+        if (v < 1 || v > 4096) return set_err(c, PG_ERR_INVALID, "fuse_min_width must be in 1..4096");
+        if (v != c->fuse_min_width) {
+            drop_graphs(c);
+            c->fuse_min_width = v;
+        }
     } else if (k == "runs_g_min") {
         if (v < 1 || v > 4096) return set_err(c, PG_ERR_INVALID, "runs_g_min must be in 1..4096");
         if (v != c->runs_g_min) {
@@ -2597,6 +2610,7 @@ extern "C" int pg_get_option(const pg_ctx* c, const char* key, int64_t* v)
     else if (k == "node_fp32") *v = c->node_fp32;
     else if (k == "node_rows") *v = c->node_rows;
     else if (k == "runs_g_min") *v = c->runs_g_min;
+    else if (k == "fuse_min_width") *v = c->fuse_min_width;
     else if (k == "reg_cols") *v = c->reg_cols;
     else if (k == "adjoint_recompute") *v = c->adjoint_recompute;
     else if (k == "vertex_order") *v = c->vertex_order;
