@@ -91,8 +91,6 @@ const char* pg_last_error(const pg_ctx* ctx);
  *                  reproduce the reference's rounding (default); 0 = reordered sums (hub
  *                  chunks, neighbour groups, colour runs), faster, as accurate, but a
  *                  different rounding
- *   "hub_seq"      exact order: 1 = hub rows by the batched hub kernel (default), 0 = by
- *                  column parts of the gathering kernel (same results)
  *   "hub_concurrent" 1 = a gathering step's hub launch runs on a second stream beside the
  *                  other vertices' launch (default); 0 = one after the other (same results)
  *   "color_runs"   whole-row neighbour sums: 1 always (rows up to 512 values), 0 never,

@@ -155,7 +155,6 @@ struct pg_ctx {
     int64_t split_lds = 1;      // combines read the split table from LDS (1) when it fits, or from memory (0)
     int64_t color_runs = -1;    // wide-row neighbour sums by colour runs: 1 always, 0 never, -1 rows of 32+ values
     int64_t exact_order = 1;    // forward neighbour sums in the reference's order (one group, hubs split by columns)
-    int64_t hub_seq = 1;        // exact order: hub rows by k_hub_seq (1) or by column parts of k_sub (0)
     int64_t fuse_grad = 1;      // adjoint: forward Y kept and the gradient taken in the transposed gather
     int64_t node_fp32 = 1;      // adjoint node sums of FP32 tables: 1 FP32 fused multiply-adds, 0 FP64
     int64_t runs_g_min = 128;   // gradient gathers by colour runs from this row width (color_runs -1)
@@ -494,12 +493,11 @@ size_t sub_lds_max_t(const pg_ctx* c, const pg_plan* p, const DevSub& d, int B, 
     }
     const int rc = rc_for(c, aux, B, d.Wy);
     if (c->exact_order && aux == AUX_GATHER) {
-        // launch_gathering_exact: hub rows (k_hub_seq, or column parts of
-        // max(8, cl)-lane teams), the other vertices by whole rows (RUNS_SEQ,
-        // LDS independent of B) or by teams of max(8, cl) lanes, one group
+        // launch_gathering_exact: hub rows (k_hub_seq), the other vertices by
+        // whole rows (RUNS_SEQ, LDS independent of B) or by teams of
+        // max(8, cl) lanes, one group
         const int cl = std::max(8, cl_of(aux, B, p->k, d.Wy, d.Wa, rc));
         m = std::max(m, hub_seq_lds_bytes(p->K));
-        m = std::max(m, lds_of<T>(cl, aux, mm, 1, OUT_PARTIAL, B, p, d, 0, split));
         if (d.xidx_g.p && runs_seq_for(c, d.Wa, runs_used_per_mille(p, d, AUX_GATHER))) {
             SubArgs<T> a{};
             a.B = B;
@@ -764,7 +762,9 @@ int launch_gathering(pg_ctx* c, int aux, int main_mode, SubArgs<T> a, T* part, i
                                          a.xidx_g && runs_seq_for(c, a.Wa, used_per_mille));
     const int rc = rc_for(c, aux, a.B, a.Wy);
     const int cl = cl_of(aux, a.B, a.k, a.Wy, a.Wa, rc);
-    a.segpad = rc == kRegCols ? stage_pad(c, aux, a.gidx != nullptr, a.B, a.Wy, a.Wa, cl, sizeof(T)) : 0;
+    // a step whose Y is kept for the adjoint gathers unstaged (the staged
+    // kernel carries no Y store, which keeps its registers at 3 waves per SIMD)
+    a.segpad = rc == kRegCols && !a.ykeep ? stage_pad(c, aux, a.gidx != nullptr, a.B, a.Wy, a.Wa, cl, sizeof(T)) : 0;
     if (a.segpad > 0) aux = AUX_STAGED;
     const hipStream_t st = c->stream;
     const int cus = c->num_cus;
@@ -831,9 +831,8 @@ int launch_gathering(pg_ctx* c, int aux, int main_mode, SubArgs<T> a, T* part, i
 // column order, starting from zero), so the FP32 sums are the reference's,
 // bit for bit:
 //  - hubs: one partial row per hub, each value one running sum over the whole
-//    neighbour list, by k_hub_seq (a wavefront per 64 values with 32 row
-//    loads in flight per lane) or, with option hub_seq 0, by column parts of
-//    the gathering kernel (a team per chunk of slots);
+//    neighbour list, by k_hub_seq (a wavefront per 64 values with 32-56 row
+//    loads in flight per lane);
 //  - the other vertices: a team with the whole neighbour list, one
 //    neighbour group, by whole rows (RUNS_SEQ, when `rows`) or by gathers;
 //  - the finishing launch adds each hub's single partial row to zero and
@@ -843,9 +842,8 @@ int launch_gathering_exact(pg_ctx* c, int aux, int main_mode, SubArgs<T> a, T* p
                            bool rows)
 {
     const int rc = rc_for(c, aux, a.B, a.Wy);
-    const int nslots = slots_for(a.B, a.Wy, rc);
     const int cl = std::max(8, cl_of(aux, a.B, a.k, a.Wy, a.Wa, rc));
-    a.segpad = rc == kRegCols ? stage_pad(c, aux, a.gidx != nullptr, a.B, a.Wy, a.Wa, cl, sizeof(T)) : 0;
+    a.segpad = rc == kRegCols && !a.ykeep ? stage_pad(c, aux, a.gidx != nullptr, a.B, a.Wy, a.Wa, cl, sizeof(T)) : 0;
     const int aux_b = a.segpad > 0 ? AUX_STAGED : aux;
     const hipStream_t st = c->stream;
     const int cus = c->num_cus;
@@ -866,20 +864,9 @@ int launch_gathering_exact(pg_ctx* c, int aux, int main_mode, SubArgs<T> a, T* p
         h.out_mode = OUT_PARTIAL;
         h.items = nullptr;
         h.item_base = 0;
-        if (c->hub_seq) {
-            h.n_items = c->n_hub;
-            record(c, sizeof(T) == 4 ? "k_hub_seq<float>" : "k_hub_seq<double>", share(c->h_rowptr[c->n_hub], 0),
-                   true);
-            HIP_TRY(c, launch_hub_seq<T>(h, cus, c->blocks_per_cu, hs));
-        } else {
-            // a part per chunk of cl slots: the same per-entry work as a
-            // non-hub vertex's team, whose chunks run one after another
-            const int team = cl;
-            h.col_parts = (nslots + team - 1) / team;
-            h.n_items = c->n_hub * h.col_parts;
-            record(c, kname<T>(team, aux, main_mode, rc), share(c->h_rowptr[c->n_hub], 0), true);
-            HIP_TRY(c, launch_sub<T>(team, aux, main_mode, rc, h, cus, c->blocks_per_cu, hs));
-        }
+        h.n_items = c->n_hub;
+        record(c, sizeof(T) == 4 ? "k_hub_seq<float>" : "k_hub_seq<double>", share(c->h_rowptr[c->n_hub], 0), true);
+        HIP_TRY(c, launch_hub_seq<T>(h, cus, c->blocks_per_cu, hs));
         record(c, "", 0, false);
         return PG_OK;
     };
@@ -2556,12 +2543,6 @@ extern "C" int pg_set_option(pg_ctx* c, const char* key, int64_t v)
             drop_graphs(c);
             c->hub_concurrent = v;
         }
-    } else if (k == "hub_seq") {
-        if (v != 0 && v != 1) return set_err(c, PG_ERR_INVALID, "hub_seq must be 0 or 1");
-        if (v != c->hub_seq) {
-            drop_graphs(c);
-            c->hub_seq = v;
-        }
     } else if (k == "color_runs") {
         if (v < -1 || v > 1) return set_err(c, PG_ERR_INVALID, "color_runs must be -1, 0 or 1");
         if (v != c->color_runs) {
@@ -2604,7 +2585,6 @@ extern "C" int pg_get_option(const pg_ctx* c, const char* key, int64_t* v)
     else if (k == "split_lds") *v = c->split_lds;
     else if (k == "color_runs") *v = c->color_runs;
     else if (k == "exact_order") *v = c->exact_order;
-    else if (k == "hub_seq") *v = c->hub_seq;
     else if (k == "hub_concurrent") *v = c->hub_concurrent;
     else if (k == "fuse_grad") *v = c->fuse_grad;
     else if (k == "node_fp32") *v = c->node_fp32;

@@ -258,17 +258,11 @@ __global__ __launch_bounds__(kBlock) void k_sub(const SubArgs<T> a)
     for (int64_t base = (int64_t)blockIdx.x * TEAMS; base < a.n_items; base += (int64_t)gridDim.x * TEAMS) {
         const int64_t item = base + team;
         const bool active = item < a.n_items;
-        int v = 0, e0 = 0, e1 = 0, slot = 0, part = 0;
+        int v = 0, e0 = 0, e1 = 0, slot = 0;
         if (active) {
             if (a.items) {
                 const int4 it = a.items[item];
                 v = it.x; e0 = it.y; e1 = it.z; slot = it.w;
-            } else if (a.col_parts > 0) {
-                v = (int)(item / a.col_parts);
-                part = (int)(item - (int64_t)v * a.col_parts);
-                slot = v;
-                e0 = a.rowptr[v];
-                e1 = a.rowptr[v + 1];
             } else {
                 v = a.item_base + (int)item;
                 if (walks_entries(AUX)) { e0 = a.rowptr[v]; e1 = a.rowptr[v + 1]; }
@@ -367,10 +361,7 @@ __global__ __launch_bounds__(kBlock) void k_sub(const SubArgs<T> a)
             const int W = gathers_rows(AUX) ? a.Wy : a.k;
             const int spb = (W + RC - 1) / RC;
             const int nslots = B * spb;
-            // a column part covers one chunk of cl slots (whole neighbour list)
-            const int s_begin = a.col_parts > 0 ? part * cl : 0;
-            const int s_end = a.col_parts > 0 ? min(nslots, s_begin + cl) : nslots;
-            for (int s0 = s_begin; s0 < s_end; s0 += cl) {
+            for (int s0 = 0; s0 < nslots; s0 += cl) {
                 const int sid = s0 + cln;
                 const bool sv = active && sid < nslots;
                 const int b = sv ? sid / spb : 0;
@@ -639,8 +630,10 @@ __global__ __launch_bounds__(kBlock) void k_sub(const SubArgs<T> a)
         if (MAIN == MAIN_TABLE && active)
             for (int c = lane; c < nM; c += TEAM) sM[c] = a.main[(int64_t)v * a.ldm + c];
         __syncthreads();
-        if (a.ykeep && active)  // Y kept for the adjoint
-            for (int c = lane; c < nY; c += TEAM) a.ykeep[(int64_t)v * a.ldy + c] = sY[c];
+        if constexpr (AUX != AUX_STAGED) {  // (the host never stages a step whose Y is kept)
+            if (a.ykeep && active)  // Y kept for the adjoint
+                for (int c = lane; c < nY; c += TEAM) a.ykeep[(int64_t)v * a.ldy + c] = sY[c];
+        }
 
         // ----------------------------------------------- phase 3: combine
         const int Wy = a.Wy, Wm = a.Wm, Wo = a.Wo, P = a.P;

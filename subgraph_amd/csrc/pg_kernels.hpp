@@ -59,10 +59,6 @@ struct SubArgs {
     int32_t n_items;
     int32_t item_base;
     const int4* items;
-    // col_parts > 0: item i is vertex i / col_parts (a hub, internal id =
-    // partial slot) with its whole neighbour list, restricted to the
-    // (i % col_parts)-th chunk of column lanes' slots (gathers, OUT_PARTIAL)
-    int32_t col_parts;
     const int32_t* rowptr;   // [n_act+1] internal CSR
     const int32_t* col;      // [nnz] neighbour (internal id)
     const T* w;              // [nnz] entry weights, nullptr = 1

@@ -580,9 +580,8 @@ EXACT_TEMPLATES = [(5, [(0, 1), (1, 2), (2, 3), (3, 4)]), (7, TREE7),
                    (12, [(0, 1), (1, 2), (2, 3), (3, 4), (4, 5), (2, 6), (6, 7), (7, 8), (6, 9), (9, 10), (10, 11)])]
 
 
-@pytest.mark.parametrize("hub_degree,color_runs,hub_seq", [(1 << 30, -1, 1), (16, -1, 1), (16, 0, 0), (16, 1, 1),
-                                                            (16, 1, 0)])
-def test_exact_order_is_the_reference_arithmetic(gpu_ctx, hub_degree, color_runs, hub_seq):
+@pytest.mark.parametrize("hub_degree,color_runs", [(1 << 30, -1), (16, -1), (16, 0), (16, 1)])
+def test_exact_order_is_the_reference_arithmetic(gpu_ctx, hub_degree, color_runs):
     """Option exact_order (the default): every FP32 operation of the DP is the
     reference's, in its order (neighbour sums one after another in column
     order, splits in IndexSys order, the scaled step in FP64), so the counts
@@ -591,13 +590,12 @@ def test_exact_order_is_the_reference_arithmetic(gpu_ctx, hub_degree, color_runs
     columns when hub_degree is small) and one or several colorings per pass,
     with the neighbour sums gathered per lane (color_runs 0), by whole rows
     (color_runs 1: every row of up to 512 values) or as chosen by width, and
-    hub rows by the batched hub kernel (hub_seq 1) or by column parts."""
-    keys = ("hub_degree", "exact_order", "color_runs", "hub_seq")
+    hub rows by the batched hub kernel."""
+    keys = ("hub_degree", "exact_order", "color_runs")
     saved = [gpu_ctx.get_option(k) for k in keys]
     gpu_ctx.set_option("exact_order", 1)
     gpu_ctx.set_option("hub_degree", hub_degree)
     gpu_ctx.set_option("color_runs", color_runs)
-    gpu_ctx.set_option("hub_seq", hub_seq)
     rng = np.random.default_rng(91 + (hub_degree & 0xff))
     try:
         s, d = sg.rmat_edges(12, 16, seed=7)
@@ -609,8 +607,7 @@ def test_exact_order_is_the_reference_arithmetic(gpu_ctx, hub_degree, color_runs
             plan = gpu_ctx.plan(k, te)
             op = oracle.Plan(k, te)
             for B in (1, 5):
-                print(f"[exact] hub_degree={hub_degree} color_runs={color_runs} hub_seq={hub_seq} k={k} B={B}",
-                      flush=True)
+                print(f"[exact] hub_degree={hub_degree} color_runs={color_runs} k={k} B={B}", flush=True)
                 cols = rng.integers(0, k, size=(B, n)).astype(np.uint8)
                 g32 = gpu_ctx.count(plan, B, colors=cols, precision="fp32")
                 for i in range(B):
