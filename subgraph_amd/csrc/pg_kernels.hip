@@ -240,20 +240,20 @@ __global__ __launch_bounds__(kBlock) void k_sub(const SubArgs<T> a)
         const int2* src = reinterpret_cast<const int2*>(a.xidx);
         int2* dst = reinterpret_cast<int2*>(s_gidx);
         for (int i = threadIdx.x; i < words; i += kBlock) dst[i] = src[i];
-        __syncthreads();
     } else if (use_gidx) {
         // (k+1)*k rows of Wy4 int16: a whole number of 8-byte words
         const int words = (a.k + 1) * a.k * a.Wy4 / 4;
         const int2* src = reinterpret_cast<const int2*>(a.gidx);
         int2* dst = reinterpret_cast<int2*>(s_gidx);
         for (int i = threadIdx.x; i < words; i += kBlock) dst[i] = src[i];
-        __syncthreads();
     } else if (uses_rank(AUX)) {
         for (int i = threadIdx.x; i < (1 << a.K); i += kBlock) s_rank[i] = a.rank_tab[i];
         if (AUX == AUX_ROWS)
             for (int i = threadIdx.x; i < a.Wa; i += kBlock) s_xmask[i] = a.xmask[i];
-        __syncthreads();
     }
+    // the shared tables are complete; from here on each team touches only
+    // its own LDS region and syncs within its wavefront
+    __syncthreads();
 
     for (int64_t base = (int64_t)blockIdx.x * TEAMS; base < a.n_items; base += (int64_t)gridDim.x * TEAMS) {
         const int64_t item = base + team;
@@ -281,7 +281,7 @@ __global__ __launch_bounds__(kBlock) void k_sub(const SubArgs<T> a)
             // the copies then fold in group order.
             T* acc = sY + grp * nY;
             for (int t = cln; t < nY; t += cl) acc[t] = T(0);
-            __syncthreads();
+            wave_sync();
             if (active) {
                 const int Wa = a.Wa;
                 const int nq = B * Wa;
@@ -346,7 +346,7 @@ __global__ __launch_bounds__(kBlock) void k_sub(const SubArgs<T> a)
                     }
                 }
             }
-            __syncthreads();
+            wave_sync();
             for (int t = lane; t < nY; t += TEAM) {
                 T sum = sY[t];
                 for (int g = 1; g < ng; ++g) sum = sum + sY[g * nY + t];
@@ -629,7 +629,7 @@ __global__ __launch_bounds__(kBlock) void k_sub(const SubArgs<T> a)
         // -------------------------------------------------- phase 2: main
         if (MAIN == MAIN_TABLE && active)
             for (int c = lane; c < nM; c += TEAM) sM[c] = a.main[(int64_t)v * a.ldm + c];
-        __syncthreads();
+        wave_sync();
         if constexpr (AUX != AUX_STAGED) {  // (the host never stages a step whose Y is kept)
             if (a.ykeep && active)  // Y kept for the adjoint
                 for (int c = lane; c < nY; c += TEAM) a.ykeep[(int64_t)v * a.ldy + c] = sY[c];
@@ -678,7 +678,7 @@ __global__ __launch_bounds__(kBlock) void k_sub(const SubArgs<T> a)
                 a.out[(int64_t)v * a.ldo + o] = res;
             }
         }
-        __syncthreads();  // LDS rows are rewritten by the next item
+        wave_sync();  // LDS rows are rewritten by the next item
     }
 }
 

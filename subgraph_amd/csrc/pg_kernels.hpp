@@ -50,6 +50,18 @@ constexpr int kRegCols = 4;  // Y values a lane accumulates per neighbour sweep
 constexpr int kMaxK = 14;      // colors supported on the device (rank table 2^(k-1) int16 in LDS)
 constexpr int kStageRegs = 4;  // staged row values per lane per neighbour
 
+// Orders a team's LDS accesses where the team lies within one wavefront and
+// its LDS region is its own: a wavefront's LDS operations execute in issue
+// order, so only the compiler must keep accesses on their side of this point.
+// Teams then run independently instead of in step with the workgroup's
+// slowest item.
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // A work item is (vertex, first entry, end entry, partial slot).  Items may
 // be explicit (hub chunks) or implicit: item i is vertex item_base + i with
 // its whole neighbour list.  A team of TEAM lanes owns an item; its lanes

@@ -219,7 +219,7 @@ __global__ __launch_bounds__(kBlock) void k_runs(const SubArgs<T> a)
             }
         }
         for (int t = lane; t < Wt; t += TEAM) sY[t] = T(0);
-        __syncthreads();
+        wave_sync();
         if (active) {
             const uint32_t cv = a.colors[(int64_t)v * B + b];
             const T* rows = a.aux + (int64_t)b * Wg;
@@ -231,9 +231,25 @@ __global__ __launch_bounds__(kBlock) void k_runs(const SubArgs<T> a)
             while (present) {
                 const uint32_t c = (uint32_t)(__ffs(present) - 1);
                 present &= present - 1u;
+                // the entry of the vertex's row each of the lane's row
+                // positions adds to for colour c (-1: none; such a position
+                // is never read)
+                int idx[RV];
                 T acc[RV];
 #pragma unroll
-                for (int t = 0; t < RV; ++t) acc[t] = T(0);
+                for (int t = 0; t < RV; ++t) {
+                    const int j = lane + TEAM * t;
+                    acc[t] = T(0);
+                    idx[t] = -1;
+                    if (j >= Wg) continue;
+                    if (MODE == RUNS_FWD) {
+                        const uint32_t S = expand_bit_r(xm[t], c) | (1u << c);
+                        if (!((S >> cv) & 1u)) idx[t] = s_rank[squeeze_bit_r(S, cv)];
+                    } else {
+                        const uint32_t A = expand_bit_r(xm[t], c);
+                        if ((A >> cv) & 1u) idx[t] = s_rank[squeeze_bit_r(A & ~(1u << cv), cv)];
+                    }
+                }
                 for (int e = e0; e < e1; e += TEAM) {
                     const int ee = e + lane;
                     const bool ok = ee < e1;
@@ -260,7 +276,7 @@ __global__ __launch_bounds__(kBlock) void k_runs(const SubArgs<T> a)
 #pragma unroll
                             for (int t = 0; t < RV; ++t) {
                                 const int j = lane + TEAM * t;
-                                x[q][t] = j < Wg ? row[j] : T(0);
+                                x[q][t] = idx[t] >= 0 ? row[j] : T(0);
                             }
                         }
 #pragma unroll
@@ -274,35 +290,24 @@ __global__ __launch_bounds__(kBlock) void k_runs(const SubArgs<T> a)
                 // move the colour's sum to the vertex's row: distinct targets
                 // per lane and position, one wavefront per team (in order)
 #pragma unroll
-                for (int t = 0; t < RV; ++t) {
-                    const int j = lane + TEAM * t;
-                    if (j >= Wg) continue;
-                    int idx = -1;
-                    if (MODE == RUNS_FWD) {
-                        const uint32_t S = expand_bit_r(xm[t], c) | (1u << c);
-                        if (!((S >> cv) & 1u)) idx = s_rank[squeeze_bit_r(S, cv)];
-                    } else {
-                        const uint32_t A = expand_bit_r(xm[t], c);
-                        if ((A >> cv) & 1u) idx = s_rank[squeeze_bit_r(A & ~(1u << cv), cv)];
-                    }
-                    if (idx >= 0) sY[idx] = sY[idx] + acc[t];
-                }
+                for (int t = 0; t < RV; ++t)
+                    if (idx[t] >= 0) sY[idx[t]] = sY[idx[t]] + acc[t];
             }
         }
-        __syncthreads();
+        wave_sync();
         if (mode == OUT_Y || mode == OUT_PARTIAL) {
             if (active) {
                 T* dst = a.out + (int64_t)(mode == OUT_Y ? v : slot) * a.ldo + (int64_t)b * Wt;
                 for (int t = lane; t < Wt; t += TEAM) dst[t] = sY[t];
             }
-            __syncthreads();
+            wave_sync();
             continue;
         }
         if (active && nM)
             for (int t = lane; t < nM; t += TEAM) sM[t] = a.main[(int64_t)v * a.ldm + (int64_t)b * nM + t];
         if (active && a.ykeep)  // Y kept for the adjoint
             for (int t = lane; t < Wt; t += TEAM) a.ykeep[(int64_t)v * a.ldy + (int64_t)b * Wt + t] = sY[t];
-        __syncthreads();
+        wave_sync();
         if (active) {
             const int P = a.P;
             if (mode == OUT_ROOT) {
@@ -336,7 +341,7 @@ __global__ __launch_bounds__(kBlock) void k_runs(const SubArgs<T> a)
                 }
             }
         }
-        __syncthreads();
+        wave_sync();
     }
 }
 
@@ -387,7 +392,7 @@ __global__ __launch_bounds__(kBlock) void k_runs_g(const SubArgs<T> a)
                 sY[t] = T(0);
                 sD[t] = active ? a.dy[(int64_t)v * a.ldd + (int64_t)b * Wt + t] : T(0);
             }
-            __syncthreads();
+            wave_sync();
             if (active) {
                 const uint32_t cv = a.colors[(int64_t)v * B + b];
                 const T* rows = a.aux + (int64_t)b * Wg;
@@ -435,10 +440,11 @@ __global__ __launch_bounds__(kBlock) void k_runs_g(const SubArgs<T> a)
                                 const int u = __shfl(uid, s, TEAM);
                                 wq[q] = __shfl(wv, s, TEAM);
                                 const T* row = rows + (int64_t)u * a.lda;
+                                // positions that land in no Y value are not read
 #pragma unroll
                                 for (int t = 0; t < RV; ++t) {
                                     const int j = lane + TEAM * t;
-                                    x[q][t] = j < Wg ? row[j] : T(0);
+                                    x[q][t] = idx[t] >= 0 ? row[j] : T(0);
                                 }
                             }
                             T dot[U];
@@ -466,12 +472,12 @@ __global__ __launch_bounds__(kBlock) void k_runs_g(const SubArgs<T> a)
                         if (idx[t] >= 0) sY[idx[t]] = sY[idx[t]] + acc[t];
                 }
             }
-            __syncthreads();
+            wave_sync();
             if (active) {
                 T* dst = a.out + (int64_t)(a.out_mode == OUT_Y ? v : slot) * a.ldo + (int64_t)b * Wt;
                 for (int t = lane; t < Wt; t += TEAM) dst[t] = sY[t];
             }
-            __syncthreads();
+            wave_sync();
         }
     }
 }
@@ -525,7 +531,7 @@ __global__ __launch_bounds__(kBlock) void k_runs_tg(const SubArgs<T> a)
         }
         for (int b = 0; b < B; ++b) {
             for (int t = lane; t < Wt; t += TEAM) sY[t] = T(0);
-            __syncthreads();
+            wave_sync();
             if (active) {
                 const uint32_t cv = a.colors[(int64_t)v * B + b];
                 const T* rows = a.aux + (int64_t)b * Wg;
@@ -579,10 +585,14 @@ __global__ __launch_bounds__(kBlock) void k_runs_tg(const SubArgs<T> a)
                                 wq[q] = __shfl(wv, s, TEAM);
                                 tq[q] = __shfl(tre, s, TEAM);
                                 const T* row = rows + (int64_t)u * a.lda;
+                                // only the positions whose set holds col(v):
+                                // the others add to no entry and dot with a
+                                // zero, and a line none of them touches is
+                                // never fetched
 #pragma unroll
                                 for (int t = 0; t < RV; ++t) {
                                     const int j = lane + TEAM * t;
-                                    x[q][t] = j < Wg ? row[j] : T(0);
+                                    x[q][t] = idx[t] >= 0 ? row[j] : T(0);
                                 }
                             }
                             T dot[U];
@@ -615,12 +625,12 @@ __global__ __launch_bounds__(kBlock) void k_runs_tg(const SubArgs<T> a)
                         if (idx[t] >= 0) sY[idx[t]] = sY[idx[t]] + acc[t];
                 }
             }
-            __syncthreads();
+            wave_sync();
             if (active) {
                 T* dst = a.out + (int64_t)(a.out_mode == OUT_Y ? v : slot) * a.ldo + (int64_t)b * Wt;
                 for (int t = lane; t < Wt; t += TEAM) dst[t] = sY[t];
             }
-            __syncthreads();
+            wave_sync();
         }
     }
 }
@@ -720,7 +730,7 @@ __global__ __launch_bounds__(kBlock) void k_runs_seq(const SubArgs<T> a)
             e1 = a.rowptr[v + 1];
         }
         for (int t = lane; t < Wt; t += TEAM) sY[t] = T(0);
-        __syncthreads();
+        wave_sync();
         if (active) {
             const uint32_t cv = a.colors[(int64_t)v * B + b];
             const T* rows = a.aux + (int64_t)b * Wg;
@@ -761,20 +771,20 @@ __global__ __launch_bounds__(kBlock) void k_runs_seq(const SubArgs<T> a)
                 }
             }
         }
-        __syncthreads();
+        wave_sync();
         if (mode == OUT_Y || mode == OUT_PARTIAL) {
             if (active) {
                 T* dst = a.out + (int64_t)v * a.ldo + (int64_t)b * Wt;
                 for (int t = lane; t < Wt; t += TEAM) dst[t] = sY[t];
             }
-            __syncthreads();
+            wave_sync();
             continue;
         }
         if (active && nM)
             for (int t = lane; t < nM; t += TEAM) sM[t] = a.main[(int64_t)v * a.ldm + (int64_t)b * nM + t];
         if (active && a.ykeep)  // Y kept for the adjoint
             for (int t = lane; t < Wt; t += TEAM) a.ykeep[(int64_t)v * a.ldy + (int64_t)b * Wt + t] = sY[t];
-        __syncthreads();
+        wave_sync();
         if (active) {
             const int P = a.P;
             if (mode == OUT_ROOT) {
@@ -808,7 +818,7 @@ __global__ __launch_bounds__(kBlock) void k_runs_seq(const SubArgs<T> a)
                 }
             }
         }
-        __syncthreads();
+        wave_sync();
     }
 }
 
