@@ -1708,6 +1708,8 @@ size_t table_bytes(const pg_ctx* c, const DevSub& d, int B, size_t tsz)
 bool fused_grad_step(const pg_ctx* c, const pg_plan* p, const DevSub& d)
 {
     if (!c->fuse_grad || d.aa <= 1 || runs_values_per_lane(d.Wy) == 0) return false;
+    // its LDS (the G_aux and own rows per team) must fit at either precision
+    if (runs_tg_lds_bytes<double>(p->K, d.Wa) > kLdsLimit) return false;
     return runs_mode_for(c, AUX_GATHER_T, d.Wy, runs_used_per_mille(p, d, AUX_GATHER_T)) == RUNS_T ||
            (c->color_runs != 0 && d.Wy >= c->fuse_min_width);
 }

@@ -180,6 +180,10 @@ template <typename T>
 size_t runs_lds_bytes(int main_mode, const SubArgs<T>& a);
 template <typename T>
 size_t runs_g_lds_bytes(const SubArgs<T>& a);
+// LDS of the fused transposed gather (RUNS_TG) for rank tables of K bits and
+// G_aux rows of Wt values
+template <typename T>
+size_t runs_tg_lds_bytes(int K, int Wt);
 template <typename T>
 hipError_t launch_runs(int mode, int main_mode, const SubArgs<T>& a, int num_cus, int blocks_per_cu, hipStream_t st);
 

@@ -491,22 +491,45 @@ __global__ __launch_bounds__(kBlock) void k_runs_g(const SubArgs<T> a)
 // row with its own row mapped the same way: d Y[u] / d w_(u,v) . dY[u], the
 // gradient of the transposed entry (u, v), added to grad[tr[e]] in FP64.
 // With the forward's Y kept, this replaces the gradient gather (RUNS_G) and
-// its second read of the aux rows.  A team owns an item for every coloring,
-// in order, and each entry belongs to one item, so the adds are
-// deterministic.
+// its second read of the aux rows.
+//
+// The item's entries are taken 64 at a time (a window).  The team sorts a
+// window by neighbour colour into an LDS queue (ballots give each entry its
+// place: colours ascending, entry order within a colour; entries of col(v)
+// add nothing and are dropped), then streams the queue U rows at a time
+// regardless of colour boundaries: a row load never waits on a colour's
+// setup, and the colour's map (and the own-row values it selects, staged in
+// LDS once per coloring) is set up when the queue reaches a new colour,
+// where the previous colour's sum moves into the vertex's row.  For an item
+// of at most 64 entries the sums are formed exactly as colour runs form
+// them; a longer item moves each colour's sum once per window.  A team owns
+// an item for every coloring, in order, and each entry belongs to one item,
+// so the gradient adds are deterministic.
+template <typename T>
+__host__ __device__ constexpr size_t runs_tg_team_bytes(int Wt)
+{
+    // queue (int4 per entry), its weights, the summed row and the own row
+    return (16 * 64 + 64 * sizeof(T) + 2 * (size_t)Wt * sizeof(T) + 15) & ~(size_t)15;
+}
+
 template <typename T, int RV, int U>
 __global__ __launch_bounds__(kBlock) void k_runs_tg(const SubArgs<T> a)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     constexpr int TEAM = 64;
     constexpr int TEAMS = kBlock / TEAM;
+    constexpr uint32_t kNone = 0xffu;  // queue slot past the window's last entry
     const int lane = threadIdx.x % TEAM;
     const int team = threadIdx.x / TEAM;
     const int B = a.B;
     const int Wg = a.Wa;  // values per colouring of a neighbour's dY row
     const int Wt = a.Wy;  // values per colouring of the G_aux row (and of the own row)
     int16_t* s_rank = reinterpret_cast<int16_t*>(smem_raw);
-    T* sY = reinterpret_cast<T*>(smem_raw + rank_bytes(a.K)) + (size_t)team * Wt;
+    unsigned char* tb = smem_raw + rank_bytes(a.K) + (size_t)team * runs_tg_team_bytes<T>(Wt);
+    int4* sQ = reinterpret_cast<int4*>(tb);  // (neighbour, transposed entry, colour, -)
+    T* sW = reinterpret_cast<T*>(tb + 16 * TEAM);
+    T* sY = sW + TEAM;
+    T* sO = sY + Wt;
     for (int i = threadIdx.x; i < (1 << a.K); i += kBlock) s_rank[i] = a.rank_tab[i];
     uint32_t xm[RV];
 #pragma unroll
@@ -530,99 +553,117 @@ __global__ __launch_bounds__(kBlock) void k_runs_tg(const SubArgs<T> a)
             }
         }
         for (int b = 0; b < B; ++b) {
-            for (int t = lane; t < Wt; t += TEAM) sY[t] = T(0);
+            const T* own = a.own + (int64_t)v * a.ldown + (int64_t)b * Wt;
+            for (int t = lane; t < Wt; t += TEAM) {
+                sY[t] = T(0);
+                sO[t] = active ? own[t] : T(0);
+            }
             wave_sync();
             if (active) {
                 const uint32_t cv = a.colors[(int64_t)v * B + b];
                 const T* rows = a.aux + (int64_t)b * Wg;
-                const T* own = a.own + (int64_t)v * a.ldown + (int64_t)b * Wt;
-                uint32_t present = 0;
-                for (int e = e0 + lane; e < e1; e += TEAM) present |= 1u << a.ecol[(int64_t)e * B + b];
-                for (int sh = 1; sh < TEAM; sh <<= 1) present |= __shfl_xor(present, sh, TEAM);
-                present &= ~(1u << cv);  // no set of a neighbour of the vertex's colour holds it
-                while (present) {
-                    const uint32_t c = (uint32_t)(__ffs(present) - 1);
-                    present &= present - 1u;
-                    // each of the lane's dY positions: its entry of the
-                    // vertex's row (-1: none) and the own value there
+                for (int w0 = e0; w0 < e1; w0 += TEAM) {
+                    const int ee = w0 + lane;
+                    const bool ok = ee < e1;
+                    const uint32_t cc = ok ? (uint32_t)a.ecol[(int64_t)ee * B + b] : kNone;
+                    const bool use = ok && cc != cv;
+                    int pos = 0, n = 0;
+                    for (int c = 0; c < a.k; ++c) {
+                        const bool mine = use && cc == (uint32_t)c;
+                        const uint64_t m = __ballot(mine);
+                        if (mine) pos = n + __popcll(m & ((1ull << lane) - 1ull));
+                        n += __popcll(m);
+                    }
+                    if (use) {
+                        sQ[pos] = make_int4(a.col[ee], a.tr[ee], (int)cc, 0);
+                        sW[pos] = a.w ? a.w[ee] : T(1);
+                    }
+                    wave_sync();
+                    // the current colour: each of the lane's dY positions'
+                    // entry of the vertex's row (-1: none), the own value
+                    // there, and the colour's running sum
+                    uint32_t cur = kNone;
                     int idx[RV];
                     T xo[RV];
                     T acc[RV];
 #pragma unroll
                     for (int t = 0; t < RV; ++t) {
-                        const int j = lane + TEAM * t;
                         idx[t] = -1;
-                        if (j < Wg) {
-                            const uint32_t A = expand_bit_r(xm[t], c);
-                            if ((A >> cv) & 1u) idx[t] = s_rank[squeeze_bit_r(A & ~(1u << cv), cv)];
-                        }
-                        xo[t] = idx[t] >= 0 ? own[idx[t]] : T(0);
+                        xo[t] = T(0);
                         acc[t] = T(0);
                     }
-                    for (int e = e0; e < e1; e += TEAM) {
-                        const int ee = e + lane;
-                        const bool ok = ee < e1;
-                        const uint32_t cc = ok ? (uint32_t)a.ecol[(int64_t)ee * B + b] : 0xffu;
-                        const int uid = ok ? a.col[ee] : 0;
-                        const T wv = ok && a.w ? a.w[ee] : T(1);
-                        const int tre = ok ? a.tr[ee] : 0;
-                        uint64_t m = __ballot(ok && cc == c);
-                        while (m) {
-                            int src[U];
+                    auto flush = [&]() {
+                        // the colour's sum into the vertex's row: distinct
+                        // targets per lane and position, in order
 #pragma unroll
-                            for (int q = 0; q < U; ++q) {
-                                src[q] = m ? __ffsll((unsigned long long)m) - 1 : -1;
-                                m &= m ? m - 1 : 0;
-                            }
-                            T x[U][RV];
-                            T wq[U];
-                            int tq[U];
-#pragma unroll
-                            for (int q = 0; q < U; ++q) {
-                                // a slot past the last match reads the first match's row
-                                const int s = src[q] >= 0 ? src[q] : src[0];
-                                const int u = __shfl(uid, s, TEAM);
-                                wq[q] = __shfl(wv, s, TEAM);
-                                tq[q] = __shfl(tre, s, TEAM);
-                                const T* row = rows + (int64_t)u * a.lda;
-                                // only the positions whose set holds col(v):
-                                // the others add to no entry and dot with a
-                                // zero, and a line none of them touches is
-                                // never fetched
-#pragma unroll
-                                for (int t = 0; t < RV; ++t) {
-                                    const int j = lane + TEAM * t;
-                                    x[q][t] = idx[t] >= 0 ? row[j] : T(0);
-                                }
-                            }
-                            T dot[U];
-#pragma unroll
-                            for (int q = 0; q < U; ++q) {
-                                dot[q] = T(0);
-#pragma unroll
-                                for (int t = 0; t < RV; ++t) dot[q] = dot[q] + xo[t] * x[q][t];
-                                if (src[q] < 0) continue;
-#pragma unroll
-                                for (int t = 0; t < RV; ++t)
-                                    acc[t] = a.w ? acc[t] + wq[q] * x[q][t] : acc[t] + x[q][t];
-                            }
-                            int qs;
-                            const T sum = team_sums<T, U>(dot, lane, &qs);
-                            int sq = -1, tgt = 0;
-#pragma unroll
-                            for (int q = 0; q < U; ++q)
-                                if (q == qs) {
-                                    sq = src[q];
-                                    tgt = tq[q];
-                                }
-                            if (lane < U && sq >= 0) (void)unsafeAtomicAdd(&a.grad[tgt], (double)sum);
+                        for (int t = 0; t < RV; ++t) {
+                            if (idx[t] >= 0) sY[idx[t]] = sY[idx[t]] + acc[t];
+                            acc[t] = T(0);
                         }
-                    }
-                    // the colour's sum into the vertex's row: distinct targets
-                    // per lane and position, one wavefront per team (in order)
+                    };
+                    auto enter = [&](uint32_t c) {
 #pragma unroll
-                    for (int t = 0; t < RV; ++t)
-                        if (idx[t] >= 0) sY[idx[t]] = sY[idx[t]] + acc[t];
+                        for (int t = 0; t < RV; ++t) {
+                            const int j = lane + TEAM * t;
+                            idx[t] = -1;
+                            if (j < Wg) {
+                                const uint32_t A = expand_bit_r(xm[t], c);
+                                if ((A >> cv) & 1u) idx[t] = s_rank[squeeze_bit_r(A & ~(1u << cv), cv)];
+                            }
+                            xo[t] = idx[t] >= 0 ? sO[idx[t]] : T(0);
+                        }
+                    };
+                    for (int i = 0; i < n; i += U) {
+                        uint32_t cq[U];
+                        int tq[U];
+                        T wq[U];
+                        T x[U][RV];
+#pragma unroll
+                        for (int q = 0; q < U; ++q) {
+                            // a slot past the queue's end reads its first entry's row (ignored)
+                            const int s = i + q < n ? i + q : i;
+                            const int4 e = sQ[s];
+                            const int u = __builtin_amdgcn_readfirstlane(e.x);
+                            tq[q] = __builtin_amdgcn_readfirstlane(e.y);
+                            cq[q] = i + q < n ? (uint32_t)__builtin_amdgcn_readfirstlane(e.z) : kNone;
+                            wq[q] = sW[s];
+                            const T* row = rows + (int64_t)u * a.lda;
+#pragma unroll
+                            for (int t = 0; t < RV; ++t) {
+                                const int j = lane + TEAM * t;
+                                x[q][t] = j < Wg ? row[j] : T(0);
+                            }
+                        }
+                        T dot[U];
+#pragma unroll
+                        for (int q = 0; q < U; ++q) {
+                            dot[q] = T(0);
+                            if (cq[q] == kNone) continue;
+                            if (cq[q] != cur) {
+                                if (cur != kNone) flush();
+                                cur = cq[q];
+                                enter(cur);
+                            }
+#pragma unroll
+                            for (int t = 0; t < RV; ++t) dot[q] = dot[q] + xo[t] * x[q][t];
+#pragma unroll
+                            for (int t = 0; t < RV; ++t)
+                                acc[t] = a.w ? acc[t] + wq[q] * x[q][t] : acc[t] + x[q][t];
+                        }
+                        int qs;
+                        const T sum = team_sums<T, U>(dot, lane, &qs);
+                        bool live = false;
+                        int tgt = 0;
+#pragma unroll
+                        for (int q = 0; q < U; ++q)
+                            if (q == qs) {
+                                live = cq[q] != kNone;
+                                tgt = tq[q];
+                            }
+                        if (lane < U && live) (void)unsafeAtomicAdd(&a.grad[tgt], (double)sum);
+                    }
+                    if (cur != kNone) flush();
+                    wave_sync();  // the next window rewrites the queue
                 }
             }
             wave_sync();
@@ -639,7 +680,7 @@ template <typename T, int RV>
 hipError_t launch_runs_tg_t(const SubArgs<T>& a, int num_cus, int blocks_per_cu, hipStream_t st)
 {
     constexpr int U = RV <= 2 ? 8 : 4;
-    const size_t lds = rank_bytes(a.K) + (size_t)(kBlock / 64) * a.Wy * sizeof(T);
+    const size_t lds = runs_tg_lds_bytes<T>(a.K, a.Wy);
     const int64_t need = ((int64_t)a.n_items + (kBlock / 64) - 1) / (kBlock / 64);
     if (need <= 0) return hipSuccess;
     static thread_local std::map<size_t, int> per_cu_cache;
@@ -950,6 +991,15 @@ hipError_t launch_runs(int mode, int main_mode, const SubArgs<T>& a, int num_cus
     return main_mode == MAIN_LEAF ? launch_runs_rv<T, RUNS_FWD, MAIN_LEAF>(rv, a, num_cus, blocks_per_cu, st)
                                   : launch_runs_rv<T, RUNS_FWD, MAIN_TABLE>(rv, a, num_cus, blocks_per_cu, st);
 }
+
+template <typename T>
+size_t runs_tg_lds_bytes(int K, int Wt)
+{
+    return rank_bytes(K) + (size_t)(kBlock / 64) * runs_tg_team_bytes<T>(Wt);
+}
+
+template size_t runs_tg_lds_bytes<float>(int, int);
+template size_t runs_tg_lds_bytes<double>(int, int);
 
 template hipError_t launch_runs<float>(int, int, const SubArgs<float>&, int, int, hipStream_t);
 template hipError_t launch_runs<double>(int, int, const SubArgs<double>&, int, int, hipStream_t);
