@@ -566,6 +566,11 @@ __global__ __launch_bounds__(kBlock) void k_runs_tg(const SubArgs<T> a)
                     const int ee = w0 + lane;
                     const bool ok = ee < e1;
                     const uint32_t cc = ok ? (uint32_t)a.ecol[(int64_t)ee * B + b] : kNone;
+                    // the entry's neighbour, transposed entry and weight load
+                    // beside its colour
+                    const int uid = ok ? a.col[ee] : 0;
+                    const int tre = ok ? a.tr[ee] : 0;
+                    const T wv = ok && a.w ? a.w[ee] : T(1);
                     const bool use = ok && cc != cv;
                     int pos = 0, n = 0;
                     for (int c = 0; c < a.k; ++c) {
@@ -575,8 +580,8 @@ __global__ __launch_bounds__(kBlock) void k_runs_tg(const SubArgs<T> a)
                         n += __popcll(m);
                     }
                     if (use) {
-                        sQ[pos] = make_int4(a.col[ee], a.tr[ee], (int)cc, 0);
-                        sW[pos] = a.w ? a.w[ee] : T(1);
+                        sQ[pos] = make_int4(uid, tre, (int)cc, 0);
+                        sW[pos] = wv;
                     }
                     wave_sync();
                     // the current colour: each of the lane's dY positions'
