@@ -322,8 +322,12 @@ def headline(args, sync):
         ctx.count_grad_device(plan, B, gbuf.data_ptr(), seed=5000 + k)
 
     t0 = time.perf_counter()
-    ctx.count(plan, B, seed=4999)        # the forward tile's tuning (autotune 1), untimed
-    step(-1)                             # the gradient call's own tile and captured graphs, untimed
+    if args.autotune:
+        ctx.count(plan, B, seed=4999)    # the forward tile's tuning, untimed
+    # the gradient call's own tile and captured graphs, untimed.  It plans
+    # first: a forward call's buffers beside it would leave its kept-table
+    # plan 3 colorings per pass instead of 4
+    step(-1)
     sync()
     tune_s = time.perf_counter() - t0
     settings = {k: ctx.get_option(k) for k in TUNED_KEYS}

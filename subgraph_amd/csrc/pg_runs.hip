@@ -40,6 +40,20 @@ __device__ __forceinline__ uint32_t squeeze_bit_r(uint32_t m, uint32_t c)
     return (m & ((1u << c) - 1u)) | ((m >> (c + 1u)) << c);
 }
 
+// lane l's value of x, in every lane (l uniform)
+__device__ __forceinline__ float read_lane(float x, int l)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l));
+}
+
+__device__ __forceinline__ double read_lane(double x, int l)
+{
+    const int64_t b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_readlane((int)(uint32_t)b, l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double((int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo));
+}
+
 __host__ __device__ constexpr size_t rank_bytes(int K) { return (((size_t)2 << K) + 15) & ~(size_t)15; }
 
 // LDS: rank table, optional split table [P][Wo], then per team the summed
@@ -781,17 +795,26 @@ __global__ __launch_bounds__(kBlock) void k_runs_seq(const SubArgs<T> a)
             const uint32_t cv = a.colors[(int64_t)v * B + b];
             const T* rows = a.aux + (int64_t)b * Wg;
             const int16_t* xrow = a.xidx_g + (int64_t)cv * a.k * a.Wa4;
-            for (int e = e0; e < e1; e += U) {
+            // entries 64 at a time: lane l loads entry w0 + l's neighbour,
+            // colour and weight once, and every lane reads a batch's from
+            // there (v_readlane), so a batch waits only on its rows
+            for (int w0 = e0; w0 < e1; w0 += TEAM) {
+              const int nw = min(TEAM, e1 - w0);
+              const bool okw = lane < nw;
+              const int uw = okw ? a.col[w0 + lane] : 0;
+              const uint32_t cw = okw ? (uint32_t)a.ecol[(int64_t)(w0 + lane) * B + b] : cv;
+              const T ww = okw && a.w ? a.w[w0 + lane] : T(1);
+              for (int i = 0; i < nw; i += U) {
                 int u[U];
                 uint32_t cu[U];
                 T wq[U];
 #pragma unroll
                 for (int q = 0; q < U; ++q) {
-                    const bool ok = e + q < e1;
-                    const int ec = ok ? e + q : e;  // a slot past the end repeats the first (ignored)
-                    u[q] = a.col[ec];
-                    cu[q] = ok ? (uint32_t)a.ecol[(int64_t)ec * B + b] : cv;
-                    wq[q] = a.w ? a.w[ec] : T(1);
+                    const bool ok = i + q < nw;
+                    const int s = ok ? i + q : i;  // a slot past the end repeats the first (ignored)
+                    u[q] = __builtin_amdgcn_readlane(uw, s);
+                    cu[q] = ok ? (uint32_t)__builtin_amdgcn_readlane((int)cw, s) : cv;
+                    wq[q] = read_lane(ww, s);
                 }
                 T x[U][RV];
                 int16_t id[U][RV];
@@ -811,10 +834,11 @@ __global__ __launch_bounds__(kBlock) void k_runs_seq(const SubArgs<T> a)
                 for (int q = 0; q < U; ++q) {
 #pragma unroll
                     for (int t = 0; t < RV; ++t) {
-                        const int i = id[q][t];
-                        if (i >= 0) sY[i] = a.w ? sY[i] + wq[q] * x[q][t] : sY[i] + x[q][t];
+                        const int yi = id[q][t];
+                        if (yi >= 0) sY[yi] = a.w ? sY[yi] + wq[q] * x[q][t] : sY[yi] + x[q][t];
                     }
                 }
+              }
             }
         }
         wave_sync();
