@@ -517,6 +517,8 @@ def plain_s20(args, sync):
     out["algorithmic_gbs_whole_pass"] = fp["algorithmic_bytes_per_coloring"] * value / 1e9
     log(f"plain s20: {value:.1f} colorings/s; dominant {roof['kernel']} {roof['avg_launch_ms']:.4f} ms/launch, "
         f"{roof['achieved']:.0f} GB/s algorithmic ({100 * roof['frac']:.2f}%)")
+    if args.only == "plain":  # a profiler's run: the line's own launches only
+        return out
     ctx.count(plan, 1, seed=900, precision="fp64")
     f64 = _timed(lambda r=0: ctx.count(plan, 1, seed=2000 + r, precision="fp64"), steps, sync)
     out["fp64"] = {"value": 1 / f64, "vs_fp32": (1 / f64) / value}
@@ -611,7 +613,8 @@ def main():
                     default="stage_rows=1,bucket_high=512,bucket_mid=64,hub_degree=512,hub_chunk=1024,exact_order=0",
                     metavar="KEY=VAL,...", help="context options of the plain_s20 line")
     ap.add_argument("--only", default="", choices=("", "headline", "plain"),
-                    help="run just that measurement (a short run for profilers); headline skips its comparisons")
+                    help="run just that measurement (a short run for profilers); headline and plain skip their "
+                         "comparisons")
     ap.add_argument("--eager", action="store_true",
                     help="with --only headline: run the steps eagerly with a HIP event pair around every launch "
                          "(one stream), as the roofline figure is measured, for a kernel trace that matches it")

@@ -106,6 +106,9 @@ const char* pg_last_error(const pg_ctx* ctx);
  *                  thread per value index over a group's rows (default), 0 = per (row, value)
  *   "runs_g_min"   pg_count_grad: gradient gathers by colour runs from rows of this many values
  *                  (default 128; narrower rows use the per-lane gathers)
+ *   "seq_max_width" exact_order 1, color_runs -1: forward gathers of rows of 32 up to this many
+ *                  values (at most 512), half or more of each used, read whole rows in the
+ *                  reference's order (default 512; wider rows use the per-lane gathers)
  *   "vertex_order" internal vertex order set at the next pg_graph_upload: 0 = descending
  *                  degree (default); 1 = degree classes aligned with the bucket thresholds,
  *                  breadth-first within a class (one-coloring passes of smaller graphs gain

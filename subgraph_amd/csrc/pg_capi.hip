@@ -159,6 +159,7 @@ struct pg_ctx {
     int64_t node_fp32 = 1;      // adjoint node sums of FP32 tables: 1 FP32 fused multiply-adds, 0 FP64
     int64_t runs_g_min = 128;   // gradient gathers by colour runs from this row width (color_runs -1)
     int64_t fuse_min_width = 128;   // fused gradient also for dY rows of at least this many values
+    int64_t seq_max_width = 512;    // exact-order gathers by whole rows (RUNS_SEQ) up to this row width
     int64_t node_rows = 1;      // adjoint node launches: a thread per value over a group's rows (1) or per (row, value)
     int64_t reg_cols = 0;       // values per lane of gathers: 0 one for rows under kRegCols, else 1 or 4
     int64_t adjoint_recompute = -1;  // adjoint tables: 0 kept, 1 recomputed, -1 recomputed when that fits 4x more
@@ -631,18 +632,19 @@ int runs_used_per_mille(const pg_plan* p, const DevSub& d, int aux)
 }
 
 // The exact-order forward sum by whole rows (RUNS_SEQ): with colour runs on
-// (1), for rows up to 512 values; by default (-1), for rows of 32 to 128
-// values of which a vertex uses at least half.  Wider rows keep the per-lane
-// gathers: a whole 462-value row costs 1.8 KB per entry and coloring against
-// the 55% a vertex uses, and a hub's single team adds its 166k rows through
-// one LDS row (the 12-vertex tree's (7;1,6) step at R-MAT s22: 323 ms per
-// 4-coloring launch by rows, 256 ms by gathers; its 55-value steps: 90 ms by
-// rows, 164 ms by gathers).
+// (1), for rows up to 512 values; by default (-1), for rows of 32 to
+// seq_max_width values (option, default 512) of which a vertex uses at least
+// half.  A team streams each neighbour's whole row once and takes a window's
+// entry ids with one load per lane, so a row costs one round trip; the
+// per-lane gathers re-walk the entry list for every chunk of 256 Y values.
+// Measured on the 12-vertex tree at R-MAT s22, 16 colorings in passes of 4:
+// the (7;1,6) step's 462-value rows 431 ms by rows, 523 ms by gathers; its
+// (10;7,3) step's 55-value rows 227 ms by rows.
 bool runs_seq_for(const pg_ctx* c, int wa, int used_per_mille)
 {
     const int rv = runs_values_per_lane(wa);
     if (c->color_runs == 0 || rv == 0) return false;
-    return c->color_runs > 0 || (wa >= kRunsMinWidth && rv <= 2 && used_per_mille >= 500);
+    return c->color_runs > 0 || (wa >= kRunsMinWidth && wa <= c->seq_max_width && used_per_mille >= 500);
 }
 
 std::string runs_name(size_t tsz, int mode, int main_mode)
@@ -2515,6 +2517,12 @@ extern "C" int pg_set_option(pg_ctx* c, const char* key, int64_t v)
             drop_graphs(c);
             c->fuse_min_width = v;
         }
+    } else if (k == "seq_max_width") {
+        if (v < 1 || v > 4096) return set_err(c, PG_ERR_INVALID, "seq_max_width must be in 1..4096");
+        if (v != c->seq_max_width) {
+            drop_graphs(c);
+            c->seq_max_width = v;
+        }
     } else if (k == "runs_g_min") {
         if (v < 1 || v > 4096) return set_err(c, PG_ERR_INVALID, "runs_g_min must be in 1..4096");
         if (v != c->runs_g_min) {
@@ -2593,6 +2601,7 @@ extern "C" int pg_get_option(const pg_ctx* c, const char* key, int64_t* v)
     else if (k == "node_rows") *v = c->node_rows;
     else if (k == "runs_g_min") *v = c->runs_g_min;
     else if (k == "fuse_min_width") *v = c->fuse_min_width;
+    else if (k == "seq_max_width") *v = c->seq_max_width;
     else if (k == "reg_cols") *v = c->reg_cols;
     else if (k == "adjoint_recompute") *v = c->adjoint_recompute;
     else if (k == "vertex_order") *v = c->vertex_order;
