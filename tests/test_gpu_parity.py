@@ -618,3 +618,45 @@ def test_exact_order_is_the_reference_arithmetic(gpu_ctx, hub_degree, color_runs
     finally:
         for key, v in zip(keys, saved):
             gpu_ctx.set_option(key, v)
+
+
+def _giant_hub_graph():
+    """R-MAT s15/ef8 plus a hub joined to 20,000 vertices: one neighbour list
+    of more than 16,384 entries."""
+    s, d = sg.rmat_edges(15, 8, seed=5)
+    rng = np.random.default_rng(23)
+    nb = rng.choice(np.arange(1, 1 << 15), size=20000, replace=False)
+    s = np.concatenate([s, np.zeros(len(nb), s.dtype)])
+    d = np.concatenate([d, nb.astype(d.dtype)])
+    n = 1 << 15
+    colptr, rowidx = sg.csc_from_edges(n, s, d)
+    return n, colptr, rowidx
+
+
+def test_giant_hub_rows_keep_the_reference_arithmetic(gpu_ctx):
+    """A hub of degree above 16,384 (20,000 added neighbours on R-MAT s15):
+    its rows are chains of that many ordered adds in the hub kernel, and the
+    FP32 counts still equal the oracle's restatement of the reference, one
+    and several colorings per pass."""
+    keys = ("hub_degree", "exact_order")
+    saved = [gpu_ctx.get_option(k) for k in keys]
+    try:
+        n, colptr, rowidx = _giant_hub_graph()
+        assert np.diff(colptr).max() >= 16384
+        gpu_ctx.set_option("exact_order", 1)
+        gpu_ctx.set_option("hub_degree", 2048)
+        gpu_ctx.upload_graph(colptr, rowidx)
+        rng = np.random.default_rng(29)
+        for k, te in EXACT_TEMPLATES[:2]:
+            plan = gpu_ctx.plan(k, te)
+            op = oracle.Plan(k, te)
+            for B in (1, 3):
+                print(f"[giant hub] k={k} B={B}", flush=True)
+                cols = rng.integers(0, k, size=(B, n)).astype(np.uint8)
+                g32 = gpu_ctx.count(plan, B, colors=cols, precision="fp32")
+                for i in range(B):
+                    o32 = oracle.count_f32(op, colptr, rowidx, cols[i])
+                    assert _rel(g32[i], o32) <= 1e-12, (k, B, i, g32[i], o32)
+    finally:
+        for key, v in zip(keys, saved):
+            gpu_ctx.set_option(key, v)
