@@ -286,13 +286,14 @@ def s22_graph():
 # the 12-vertex tree's fixed settings (--autotune 0, the default).  Measured
 # at s22, forward+backward of 16 colorings replayed from captured graphs:
 # 4 colorings per pass 4.85-4.89 s, 2 per pass 4.95-4.98 s, 3 per pass
-# 5.07-5.09 s (five passes of 3 and a remainder pass of 1); hubs from degree
-# 512, 1024 or 2048 within 1%.  The autotuner settles on 3 per pass (it
-# times forward passes per coloring, not the call's remainder), so the line
-# fixes the tile.  In the reference's summation order every forward gather
-# of this template runs teams of its own width, so the bucket thresholds
-# matter little.
-TREE12_SETTINGS = {"stage_rows": 1, "bucket_high": 32, "bucket_mid": 8, "hub_degree": 2048, "hub_chunk": 1024,
+# 5.07-5.09 s (five passes of 3 and a remainder pass of 1), with the earlier
+# kernels; 4 per pass is also the most the kept tables fit.  The autotuner
+# settles on 3 per pass (it times forward passes per coloring, not the call's
+# remainder), so the line fixes the tile.  Hubs from degree 4096: 4.18 s
+# against 4.26 s from 2048, 4.29 s from 1024 and 4.21 s from 8192; hub
+# chunks of 1024 entries (512: 4.20 s, 2048: 4.19 s, 4096: 4.24 s).  Degree
+# buckets 16-128 / 4-16 all within 0.2%.
+TREE12_SETTINGS = {"stage_rows": 1, "bucket_high": 32, "bucket_mid": 8, "hub_degree": 4096, "hub_chunk": 1024,
                    "batch_tile": 4}
 HEADLINE = "tree12_fwd_bwd_rmat_s22_ef16"
 
