@@ -1040,9 +1040,22 @@ __global__ __launch_bounds__(256) void k_bwd_node_lds(const NodeArgs<T> a, int v
         for (int i = threadIdx.x; i < nv * wm; i += blockDim.x) sM[i] = a.Tm[vb0 * wm + i];
         for (int i = threadIdx.x; i < nv * wy; i += blockDim.x) sY[i] = a.Y[vb0 * yw + i];
         __syncthreads();
+        // items kind-major: every dY value of the group's rows, then every
+        // G_main value, so the lanes of a wavefront sum lists of one length
+        // (Pa or Pm terms; 210 against 14 in the (6;2,4) step of the
+        // 12-vertex tree) instead of waiting on the longest in a mixed row
+        const int n_dy = nv * ny;
         for (int it = threadIdx.x; it < nv * per; it += blockDim.x) {
-            const int lv = it / per;
-            const int x0 = it - lv * per;
+            int lv, x0;
+            if (it < n_dy) {
+                lv = it / ny;
+                x0 = it - lv * ny;
+            } else {
+                const int j = it - n_dy;
+                const int nmv = per - ny;
+                lv = j / nmv;
+                x0 = ny + (j - lv * nmv);
+            }
             const int64_t vb = vb0 + lv;
             const int b = (int)(vb % a.B);
             const T* G = wg ? sG + lv * wg : nullptr;
