@@ -570,9 +570,9 @@ def fwd_bwd_variant(sync):
     for name, te in (("star5", [(0, 1), (0, 2), (0, 3), (0, 4)]), ("u5", U5)):
         plan = ctx.plan(5, te)
         t_plain = _timed(lambda r=0: ctx.count(plan, 1, seed=50 + r), 40, sync)
-        t_fwd = _timed(lambda r=0: ctx.count(plan, 16, seed=60 + r), 5, sync)
+        t_fwd = _timed(lambda r=0: ctx.count(plan, 16, seed=60 + r), 20, sync)
         t_fb1 = _timed(lambda r=0: ctx.count_grad_device(plan, 1, grad.data_ptr(), seed=80 + r), 40, sync)
-        t_fb = _timed(lambda r=0: ctx.count_grad_device(plan, 16, grad.data_ptr(), seed=70 + r), 5, sync)
+        t_fb = _timed(lambda r=0: ctx.count_grad_device(plan, 16, grad.data_ptr(), seed=70 + r), 20, sync)
         rec = {"plain_fwd_colorings_per_s": 1.0 / t_plain, "batched_fwd_colorings_per_s": 16 / t_fwd,
                "plain_fwd_bwd_colorings_per_s": 1.0 / t_fb1, "fwd_bwd_colorings_per_s": 16 / t_fb}
         rec["fwd_bwd_vs_plain_fwd_bwd"] = rec["fwd_bwd_colorings_per_s"] / rec["plain_fwd_bwd_colorings_per_s"]
