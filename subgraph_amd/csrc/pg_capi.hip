@@ -117,6 +117,8 @@ struct DevSub {
     DevBuf xidx_g;   // the same table for rows up to 512 values, read from memory (RUNS_SEQ)
     // the adjoint's transposed gather (AUX_GATHER_T) through its own index table
     DevBuf gidxT;    // [(k+1)*k][Wa4] int16
+    // the fused transposed gather's per-lane index table (SubArgs::ttab)
+    DevBuf ttab;     // [k*k][64][RV] int16, RV = runs_values_per_lane(Wy)
 };
 
 constexpr size_t kGidxMaxBytes = 24 * 1024;
@@ -1830,6 +1832,7 @@ int enqueue_sub_backward(pg_ctx* c, const pg_plan* p, const DevSub& d, int B, co
         t.own = Ta;
         t.ldown = B * d.Wa;
         t.tr = c->d_tr.as<int32_t>();
+        t.ttab = d.ttab.as<int16_t>();
         t.grad = c->d_grad.as<double>();
         // reads the neighbours' dY, the own aux rows and the transposed
         // entry ids, writes G_aux, reads and writes each entry's gradient
@@ -2339,6 +2342,32 @@ std::vector<int16_t> gather_index_table_t(int k, const std::vector<uint16_t>& am
 // SubArgs::xidx: value j of the row of a neighbour of color cu (the set
 // amask[j] relative to cu, plus cu) adds, at a vertex of color cv, to the Y
 // value of that set relative to cv, when cv is outside it
+// SubArgs::ttab of the fused transposed gather: for a vertex of colour cv
+// and a neighbour of colour c, position j = lane + 64 t of the neighbour's dY
+// row (the set ymask[j] relative to c, plus c) adds to the vertex's row entry
+// of that set minus cv (relative to cv) when the set holds cv, else -1; one
+// row of rv values per lane, so a lane reads its rv indices with one load
+std::vector<int16_t> runs_tg_table(int k, const std::vector<uint16_t>& ymask, const std::vector<int16_t>& rank, int rv)
+{
+    const int Wy = (int)ymask.size();
+    std::vector<int16_t> g((size_t)k * k * 64 * rv, (int16_t)-1);
+    auto expand = [](uint32_t m, uint32_t c) { return (m & ((1u << c) - 1u)) | ((m >> c) << (c + 1u)); };
+    auto squeeze = [](uint32_t m, uint32_t c) { return (m & ((1u << c) - 1u)) | ((m >> (c + 1u)) << c); };
+    for (int cv = 0; cv < k; ++cv)
+        for (int c = 0; c < k; ++c) {
+            if (c == cv) continue;  // entries of the vertex's own colour add nothing
+            for (int lane = 0; lane < 64; ++lane)
+                for (int t = 0; t < rv; ++t) {
+                    const int j = lane + 64 * t;
+                    if (j >= Wy) continue;
+                    const uint32_t A = expand(ymask[j], (uint32_t)c);
+                    if (!((A >> cv) & 1u)) continue;
+                    g[(((size_t)cv * k + c) * 64 + lane) * rv + t] = rank[squeeze(A & ~(1u << cv), (uint32_t)cv)];
+                }
+        }
+    return g;
+}
+
 std::vector<int16_t> rows_index_table(int k, const std::vector<uint16_t>& amask, const std::vector<int16_t>& rank,
                                       int Wa4)
 {
@@ -3025,6 +3054,11 @@ extern "C" int pg_plan_create(pg_ctx* c, int k, int nsub, const int32_t* sub_siz
                 const std::vector<int16_t> g = gather_index_table_t(k, am, rank, d->Wa4);
                 HIP_TRY(c, d->gidxT.alloc(tbytes));
                 HIP_TRY(c, hipMemcpy(d->gidxT.p, g.data(), tbytes, hipMemcpyHostToDevice));
+            }
+            if (const int rv = runs_values_per_lane(d->Wy)) {
+                const std::vector<int16_t> g = runs_tg_table(k, ym, rank, rv);
+                HIP_TRY(c, d->ttab.alloc(g.size() * 2));
+                HIP_TRY(c, hipMemcpy(d->ttab.p, g.data(), g.size() * 2, hipMemcpyHostToDevice));
             }
         }
         plan->subs.push_back(std::move(d));

@@ -116,6 +116,10 @@ struct SubArgs {
     const T* own;            // [n_act][ldown]
     int32_t ldown;           // B * (aux child's row width)
     const int32_t* tr;       // [nnz] entry (u, v) of entry (v, u)
+    // ttab[((cv * k + c) * 64 + lane) * RV + t]: the entry of the vertex's
+    // row (colour cv) that position lane + 64 t of a neighbour's dY row
+    // (colour c) adds to, or -1; RV = runs_values_per_lane(dY width)
+    const int16_t* ttab;
     // forward steps, optional: Y of every vertex written out as well
     // (ykeep[v * ldy + b * Wy + i]) for the adjoint
     T* ykeep;

@@ -512,9 +512,10 @@ __global__ __launch_bounds__(kBlock) void k_runs_g(const SubArgs<T> a)
 // place: colours ascending, entry order within a colour; entries of col(v)
 // add nothing and are dropped), then streams the queue U rows at a time
 // regardless of colour boundaries: a row load never waits on a colour's
-// setup, and the colour's map (and the own-row values it selects, staged in
-// LDS once per coloring) is set up when the queue reaches a new colour,
-// where the previous colour's sum moves into the vertex's row.  For an item
+// setup, and the colour's map (one load of the lane's indices from the
+// table SubArgs::ttab, and the own-row values they select, staged in LDS
+// once per coloring) is set up when the queue reaches a new colour, where
+// the previous colour's sum moves into the vertex's row.  For an item
 // of at most 64 entries the sums are formed exactly as colour runs form
 // them; a longer item moves each colour's sum once per window.  A team owns
 // an item for every coloring, in order, and each entry belongs to one item,
@@ -538,20 +539,11 @@ __global__ __launch_bounds__(kBlock) void k_runs_tg(const SubArgs<T> a)
     const int B = a.B;
     const int Wg = a.Wa;  // values per colouring of a neighbour's dY row
     const int Wt = a.Wy;  // values per colouring of the G_aux row (and of the own row)
-    int16_t* s_rank = reinterpret_cast<int16_t*>(smem_raw);
-    unsigned char* tb = smem_raw + rank_bytes(a.K) + (size_t)team * runs_tg_team_bytes<T>(Wt);
+    unsigned char* tb = smem_raw + (size_t)team * runs_tg_team_bytes<T>(Wt);
     int4* sQ = reinterpret_cast<int4*>(tb);  // (neighbour, transposed entry, colour, -)
     T* sW = reinterpret_cast<T*>(tb + 16 * TEAM);
     T* sY = sW + TEAM;
     T* sO = sY + Wt;
-    for (int i = threadIdx.x; i < (1 << a.K); i += kBlock) s_rank[i] = a.rank_tab[i];
-    uint32_t xm[RV];
-#pragma unroll
-    for (int t = 0; t < RV; ++t) {
-        const int j = lane + TEAM * t;
-        xm[t] = j < Wg ? (uint32_t)a.xmask[j] : 0u;
-    }
-    __syncthreads();
     for (int64_t base = (int64_t)blockIdx.x * TEAMS; base < a.n_items; base += (int64_t)gridDim.x * TEAMS) {
         const int64_t item = base + team;
         const bool active = item < a.n_items;
@@ -621,14 +613,33 @@ __global__ __launch_bounds__(kBlock) void k_runs_tg(const SubArgs<T> a)
                         }
                     };
                     auto enter = [&](uint32_t c) {
+                        // the lane's RV indices for (cv, c): one load from the table
+                        const int16_t* tp = a.ttab + (((size_t)cv * a.k + c) * TEAM + lane) * RV;
+                        int16_t iv[RV];
+                        if constexpr (RV == 8) {
+                            const int4 w = *reinterpret_cast<const int4*>(tp);
+                            const int ws[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+                            for (int q = 0; q < 4; ++q) {
+                                iv[2 * q] = (int16_t)(ws[q] & 0xffff);
+                                iv[2 * q + 1] = (int16_t)(ws[q] >> 16);
+                            }
+                        } else if constexpr (RV == 4) {
+                            const int2 w = *reinterpret_cast<const int2*>(tp);
+                            iv[0] = (int16_t)(w.x & 0xffff);
+                            iv[1] = (int16_t)(w.x >> 16);
+                            iv[2] = (int16_t)(w.y & 0xffff);
+                            iv[3] = (int16_t)(w.y >> 16);
+                        } else if constexpr (RV == 2) {
+                            const int w = *reinterpret_cast<const int*>(tp);
+                            iv[0] = (int16_t)(w & 0xffff);
+                            iv[1] = (int16_t)(w >> 16);
+                        } else {
+                            iv[0] = tp[0];
+                        }
 #pragma unroll
                         for (int t = 0; t < RV; ++t) {
-                            const int j = lane + TEAM * t;
-                            idx[t] = -1;
-                            if (j < Wg) {
-                                const uint32_t A = expand_bit_r(xm[t], c);
-                                if ((A >> cv) & 1u) idx[t] = s_rank[squeeze_bit_r(A & ~(1u << cv), cv)];
-                            }
+                            idx[t] = iv[t];
                             xo[t] = idx[t] >= 0 ? sO[idx[t]] : T(0);
                         }
                     };
@@ -1006,7 +1017,7 @@ hipError_t launch_runs(int mode, int main_mode, const SubArgs<T>& a, int num_cus
     }
     if (mode == RUNS_TG) {
         if (main_mode != MAIN_LEAF || (a.out_mode != OUT_Y && a.out_mode != OUT_PARTIAL) || !a.own || !a.tr ||
-            !a.grad)
+            !a.grad || !a.ttab)
             return hipErrorInvalidValue;
         switch (rv) {
         case 1: return launch_runs_tg_t<T, 1>(a, num_cus, blocks_per_cu, st);
@@ -1024,7 +1035,8 @@ hipError_t launch_runs(int mode, int main_mode, const SubArgs<T>& a, int num_cus
 template <typename T>
 size_t runs_tg_lds_bytes(int K, int Wt)
 {
-    return rank_bytes(K) + (size_t)(kBlock / 64) * runs_tg_team_bytes<T>(Wt);
+    (void)K;  // the index table is read from memory (SubArgs::ttab)
+    return (size_t)(kBlock / 64) * runs_tg_team_bytes<T>(Wt);
 }
 
 template size_t runs_tg_lds_bytes<float>(int, int);
