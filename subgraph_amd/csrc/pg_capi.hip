@@ -114,7 +114,7 @@ struct DevSub {
     // row-wise index table of AUX_ROWS (SubArgs::xidx), when it fits kGidxMaxBytes
     DevBuf xidx;     // [k*k][Wa4] int16
     int Wa4 = 0;
-    DevBuf xidx_g;   // the same table for rows up to 512 values, read from memory (RUNS_SEQ)
+    DevBuf xidx_g;   // the same targets per lane for rows up to 512 values (RUNS_SEQ): [k*k][64][RV]
     // the adjoint's transposed gather (AUX_GATHER_T) through its own index table
     DevBuf gidxT;    // [(k+1)*k][Wa4] int16
     // the fused transposed gather's per-lane index table (SubArgs::ttab)
@@ -3044,9 +3044,17 @@ extern "C" int pg_plan_create(pg_ctx* c, int k, int nsub, const int32_t* sub_siz
                     HIP_TRY(c, d->xidx.alloc(xbytes));
                     HIP_TRY(c, hipMemcpy(d->xidx.p, x.data(), xbytes, hipMemcpyHostToDevice));
                 }
-                if (runs_values_per_lane(d->Wa) > 0) {
-                    HIP_TRY(c, d->xidx_g.alloc(xbytes));
-                    HIP_TRY(c, hipMemcpy(d->xidx_g.p, x.data(), xbytes, hipMemcpyHostToDevice));
+                if (const int rv = runs_values_per_lane(d->Wa)) {
+                    // per lane: value lane + 64 t of the row at [lane][t]
+                    std::vector<int16_t> xl((size_t)k * k * 64 * rv, (int16_t)-1);
+                    for (int cc = 0; cc < k * k; ++cc)
+                        for (int lane = 0; lane < 64; ++lane)
+                            for (int t = 0; t < rv; ++t) {
+                                const int j = lane + 64 * t;
+                                if (j < d->Wa) xl[((size_t)cc * 64 + lane) * rv + t] = x[(size_t)cc * d->Wa4 + j];
+                            }
+                    HIP_TRY(c, d->xidx_g.alloc(xl.size() * 2));
+                    HIP_TRY(c, hipMemcpy(d->xidx_g.p, xl.data(), xl.size() * 2, hipMemcpyHostToDevice));
                 }
             }
             const size_t tbytes = (size_t)2 * (k + 1) * k * d->Wa4;

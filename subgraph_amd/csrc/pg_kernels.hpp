@@ -102,7 +102,10 @@ struct SubArgs {
     // cv) that value j of a neighbour row (color cu) adds to, or -1
     const int16_t* xidx;
     int32_t Wa4;             // Wa rounded up to 4
-    const int16_t* xidx_g;   // the same table in device memory for any width (RUNS_SEQ reads it there)
+    // RUNS_SEQ: the same targets for rows of up to 512 values, per lane:
+    // xidx_g[((cv * k + cu) * 64 + lane) * RV + t] for value lane + 64 t,
+    // RV = runs_values_per_lane(Wa) (-1 past the row's end)
+    const int16_t* xidx_g;
     const int32_t* slot_begin;  // hub h's partial slots [slot_begin[h], slot_begin[h+1])
     // AUX_GATHER_G: each neighbour's contribution x to the vertex's Y row is
     // also contracted with dY there, and grad[e] += sum over the row of dY * x

@@ -40,6 +40,33 @@ __device__ __forceinline__ uint32_t squeeze_bit_r(uint32_t m, uint32_t c)
     return (m & ((1u << c) - 1u)) | ((m >> (c + 1u)) << c);
 }
 
+// RV consecutive int16 values (RV * 2 bytes, aligned) with one load
+template <int RV>
+__device__ __forceinline__ void load_i16(const int16_t* p, int16_t (&v)[RV])
+{
+    if constexpr (RV == 8) {
+        const int4 w = *reinterpret_cast<const int4*>(p);
+        const int ws[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            v[2 * q] = (int16_t)(ws[q] & 0xffff);
+            v[2 * q + 1] = (int16_t)(ws[q] >> 16);
+        }
+    } else if constexpr (RV == 4) {
+        const int2 w = *reinterpret_cast<const int2*>(p);
+        v[0] = (int16_t)(w.x & 0xffff);
+        v[1] = (int16_t)(w.x >> 16);
+        v[2] = (int16_t)(w.y & 0xffff);
+        v[3] = (int16_t)(w.y >> 16);
+    } else if constexpr (RV == 2) {
+        const int w = *reinterpret_cast<const int*>(p);
+        v[0] = (int16_t)(w & 0xffff);
+        v[1] = (int16_t)(w >> 16);
+    } else {
+        v[0] = p[0];
+    }
+}
+
 // lane l's value of x, in every lane (l uniform)
 __device__ __forceinline__ float read_lane(float x, int l)
 {
@@ -614,29 +641,8 @@ __global__ __launch_bounds__(kBlock) void k_runs_tg(const SubArgs<T> a)
                     };
                     auto enter = [&](uint32_t c) {
                         // the lane's RV indices for (cv, c): one load from the table
-                        const int16_t* tp = a.ttab + (((size_t)cv * a.k + c) * TEAM + lane) * RV;
                         int16_t iv[RV];
-                        if constexpr (RV == 8) {
-                            const int4 w = *reinterpret_cast<const int4*>(tp);
-                            const int ws[4] = {w.x, w.y, w.z, w.w};
-#pragma unroll
-                            for (int q = 0; q < 4; ++q) {
-                                iv[2 * q] = (int16_t)(ws[q] & 0xffff);
-                                iv[2 * q + 1] = (int16_t)(ws[q] >> 16);
-                            }
-                        } else if constexpr (RV == 4) {
-                            const int2 w = *reinterpret_cast<const int2*>(tp);
-                            iv[0] = (int16_t)(w.x & 0xffff);
-                            iv[1] = (int16_t)(w.x >> 16);
-                            iv[2] = (int16_t)(w.y & 0xffff);
-                            iv[3] = (int16_t)(w.y >> 16);
-                        } else if constexpr (RV == 2) {
-                            const int w = *reinterpret_cast<const int*>(tp);
-                            iv[0] = (int16_t)(w & 0xffff);
-                            iv[1] = (int16_t)(w >> 16);
-                        } else {
-                            iv[0] = tp[0];
-                        }
+                        load_i16<RV>(a.ttab + (((size_t)cv * a.k + c) * TEAM + lane) * RV, iv);
 #pragma unroll
                         for (int t = 0; t < RV; ++t) {
                             idx[t] = iv[t];
@@ -805,7 +811,8 @@ __global__ __launch_bounds__(kBlock) void k_runs_seq(const SubArgs<T> a)
         if (active) {
             const uint32_t cv = a.colors[(int64_t)v * B + b];
             const T* rows = a.aux + (int64_t)b * Wg;
-            const int16_t* xrow = a.xidx_g + (int64_t)cv * a.k * a.Wa4;
+            // the vertex colour's rows of the per-lane index table
+            const int16_t* xrow = a.xidx_g + (size_t)cv * a.k * TEAM * RV;
             // entries 64 at a time: lane l loads entry w0 + l's neighbour,
             // colour and weight once, and every lane reads a batch's from
             // there (v_readlane), so a batch waits only on its rows
@@ -832,12 +839,12 @@ __global__ __launch_bounds__(kBlock) void k_runs_seq(const SubArgs<T> a)
 #pragma unroll
                 for (int q = 0; q < U; ++q) {
                     const T* row = rows + (int64_t)u[q] * a.lda;
-                    const int16_t* ix = xrow + cu[q] * a.Wa4;
+                    // the lane's RV targets for this neighbour colour, one load
+                    load_i16<RV>(xrow + ((size_t)cu[q] * TEAM + lane) * RV, id[q]);
 #pragma unroll
                     for (int t = 0; t < RV; ++t) {
                         const int j = lane + TEAM * t;
                         x[q][t] = j < Wg ? row[j] : T(0);
-                        id[q][t] = j < Wg ? ix[j] : (int16_t)-1;
                     }
                 }
                 // in the neighbours' order: every value into its Y value
