@@ -554,8 +554,13 @@ __host__ __device__ constexpr size_t runs_tg_team_bytes(int Wt)
     return (16 * 64 + 64 * sizeof(T) + 2 * (size_t)Wt * sizeof(T) + 15) & ~(size_t)15;
 }
 
+// FP32 rows of up to 512 values at 5 wavefronts per SIMD: 96 VGPRs and a
+// 48-byte spill, against 107 VGPRs at 4 (the 12-tree's fused gathers at
+// R-MAT s22: 1,105 ms against 1,159).  FP64 keeps the compiler's choice (it
+// would spill 316 bytes).
 template <typename T, int RV, int U>
-__global__ __launch_bounds__(kBlock) void k_runs_tg(const SubArgs<T> a)
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && RV == 8 ? 5 : 1)))
+void k_runs_tg(const SubArgs<T> a)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     constexpr int TEAM = 64;
