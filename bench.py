@@ -627,12 +627,19 @@ def main():
                          "--copy-calibration MIB, the same MIB given here) into the traffic JSON for --workload, "
                          "and exit")
     ap.add_argument("--workload", default=HEADLINE, help="workload name of --summarize-pmc")
+    ap.add_argument("--summarize-sq", default="", metavar="CSV",
+                    help="print a rocprofv3 --pmc counter_collection.csv of the SQ counters SQ_WAVES, "
+                         "SQ_INSTS_VALU, SQ_INSTS_LDS, SQ_INSTS_VMEM_RD, SQ_WAIT_INST_LDS, SQ_LDS_BANK_CONFLICT, "
+                         "SQ_WAVE_CYCLES, SQ_BUSY_CYCLES as a markdown table by the library's kernel names, and exit")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run one elementwise kernel that reads and writes MIB MiB (the known-size "
                          "kernel rocprofv3 FETCH_SIZE / WRITE_SIZE readings are scaled by)")
     args = ap.parse_args()
     if args.summarize_stats:
         print(stats_table(args.summarize_stats))
+        return
+    if args.summarize_sq:
+        print(sq_table(args.summarize_sq))
         return
     if args.summarize_pmc:
         if args.copy_calibration <= 0:
@@ -732,6 +739,38 @@ def record_name(symbol):
     if name in ("k_edge_grad", "k_fascia_step"):
         return f"{name}<{a[0]}>"
     return s
+
+
+def sq_table(path, rows=12):
+    """Per library kernel name: kernel time, waves, and per wave the VALU,
+    LDS and vector-memory read instructions; LDS wait cycles over wave
+    cycles; LDS bank-conflict cycles per LDS instruction; and the VALU
+    instructions issued per second over the kernel's time."""
+    import collections
+    import csv
+    agg = collections.defaultdict(lambda: collections.defaultdict(float))
+    dur = collections.defaultdict(float)
+    seen = set()
+    with open(path) as f:
+        for row in csv.DictReader(f):
+            name = record_name(row["Kernel_Name"])
+            if not name.startswith("k_"):
+                continue
+            agg[name][row["Counter_Name"]] += float(row["Counter_Value"])
+            if (name, row["Dispatch_Id"]) not in seen:
+                seen.add((name, row["Dispatch_Id"]))
+                dur[name] += (int(row["End_Timestamp"]) - int(row["Start_Timestamp"])) / 1e6
+    out = ["| kernel | ms | waves | VALU/wave | LDS/wave | VMEM rd/wave | LDS wait/wave cycles | "
+           "bank-conflict cycles/LDS inst | VALU inst/s |", "|---|---|---|---|---|---|---|---|---|"]
+    for name in sorted(agg, key=lambda k: -dur[k])[:rows]:
+        c = agg[name]
+        w = max(c["SQ_WAVES"], 1.0)
+        out.append(f"| `{name}` | {dur[name]:.0f} | {c['SQ_WAVES']:.0f} | {c['SQ_INSTS_VALU'] / w:.0f} | "
+                   f"{c['SQ_INSTS_LDS'] / w:.0f} | {c['SQ_INSTS_VMEM_RD'] / w:.0f} | "
+                   f"{c['SQ_WAIT_INST_LDS'] / max(c['SQ_WAVE_CYCLES'], 1.0):.3f} | "
+                   f"{c['SQ_LDS_BANK_CONFLICT'] / max(c['SQ_INSTS_LDS'], 1.0):.2f} | "
+                   f"{c['SQ_INSTS_VALU'] / max(dur[name] * 1e-3, 1e-9):.2e} |")
+    return "\n".join(out)
 
 
 def stats_table(path):
