@@ -1007,6 +1007,30 @@ __device__ __forceinline__ Acc node_sum(List entry_at, int n, const T* G, Acc g0
     return acc;
 }
 
+// dst[at(i)] = src[i] for i < nvalid and T(0) for nvalid <= i < n
+// (1 <= nvalid <= n): each thread issues 8 global loads before it writes
+// them to LDS (a copy loop of single load-wait-write steps waits out the
+// memory latency once per element).  Indexes past nvalid load
+// src[nvalid - 1], so the loads need no branches.
+template <typename T, typename At>
+__device__ __forceinline__ void stage_copy(T* dst, const T* src, int n, int nvalid, At at)
+{
+    constexpr int SU = 8;
+    for (int i0 = threadIdx.x; i0 < n; i0 += SU * (int)blockDim.x) {
+        T v[SU];
+#pragma unroll
+        for (int q = 0; q < SU; ++q) {
+            const int i = i0 + q * (int)blockDim.x;
+            v[q] = src[i < nvalid ? i : nvalid - 1];
+        }
+#pragma unroll
+        for (int q = 0; q < SU; ++q) {
+            const int i = i0 + q * (int)blockDim.x;
+            if (i < n) dst[at(i)] = i < nvalid ? v[q] : T(0);
+        }
+    }
+}
+
 template <typename T, typename Acc>
 __global__ __launch_bounds__(256) void k_bwd_node_lds(const NodeArgs<T> a, int vpb, int lists_lds)
 {
@@ -1036,9 +1060,10 @@ __global__ __launch_bounds__(256) void k_bwd_node_lds(const NodeArgs<T> a, int v
         const int64_t vb0 = g * vpb;
         const int nv = (int)(nvb - vb0 < vpb ? nvb - vb0 : vpb);
         __syncthreads();  // the previous group's reads are done
-        for (int i = threadIdx.x; i < nv * wg; i += blockDim.x) sG[i] = a.G[vb0 * wg + i];
-        for (int i = threadIdx.x; i < nv * wm; i += blockDim.x) sM[i] = a.Tm[vb0 * wm + i];
-        for (int i = threadIdx.x; i < nv * wy; i += blockDim.x) sY[i] = a.Y[vb0 * yw + i];
+        const auto same = [](int i) { return i; };
+        if (wg) stage_copy(sG, a.G + vb0 * wg, nv * wg, nv * wg, same);
+        if (wm) stage_copy(sM, a.Tm + vb0 * wm, nv * wm, nv * wm, same);
+        if (wy) stage_copy(sY, a.Y + vb0 * yw, nv * wy, nv * wy, same);
         __syncthreads();
         // items kind-major: every dY value of the group's rows, then every
         // G_main value, so the lanes of a wavefront sum lists of one length
@@ -1092,72 +1117,125 @@ __global__ __launch_bounds__(256) void k_bwd_node_lds(const NodeArgs<T> a, int v
 
 // The staged kernel with the loops the other way round: a thread takes one
 // value index for every staged row of the group, so each split-list entry is
-// read once per group instead of once per row, and the group's rows share it
-// (two LDS reads per term instead of three).  Per row the same sum, in the
-// same order, as k_bwd_node_lds.  vpb <= kNodeRows; steps below the root
-// (a G table) whose Y is not the histogram.
-constexpr int kNodeRows = 16;
+// read once per group instead of once per row, and the group's rows share it.
+// The staged rows are interleaved, value-major ([value][R] with R rows per
+// group): the R values one term reads from a table sit side by side, so a
+// lane takes them with R * sizeof(T) / 16 16-byte LDS reads instead of R
+// 4- or 8-byte ones, and lanes that name different values of one bank group
+// collide once per 16-byte read instead of once per row.  Rows past the
+// group's last are zero-filled and their sums not stored.  Per row the same
+// sum, in the same order, as k_bwd_node_lds.  Steps below the root (a G
+// table) whose Y is not the histogram.
+constexpr int kNodeRows = 8;
 
-template <typename T, typename Acc>
-__global__ __launch_bounds__(256) void k_bwd_node_rows(const NodeArgs<T> a, int vpb, int lists_lds)
+// R consecutive values of T from 16-byte aligned LDS
+template <typename T, int R>
+__device__ __forceinline__ void lds_take(const T* p, T (&v)[R])
+{
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    static_assert((R * sizeof(T)) % 16 == 0, "R values of T span whole 16-byte words");
+    constexpr int N = (int)(R * sizeof(T) / 16);
+#pragma unroll
+    for (int q = 0; q < N; ++q) {
+        const f4 w = reinterpret_cast<const f4*>(p)[q];
+        __builtin_memcpy(reinterpret_cast<char*>(v) + 16 * q, &w, 16);
+    }
+}
+
+// rows [vb0, vb0 + nv) of a row-major global table of width w into LDS as
+// [w][R], zeros past row nv
+template <typename T, int R>
+__device__ __forceinline__ void stage_interleaved(T* dst, const T* src, int64_t vb0, int nv, int w)
+{
+    stage_copy(dst, src + vb0 * w, R * w, nv * w, [w](int i) {
+        const int r = i / w;
+        return (i - r * w) * R + r;
+    });
+}
+
+// acc[r] += G_r[o] * row_r[i] over a split list's entries (o << 16 | i) in
+// list order, for the R interleaved rows; the LDS reads of U entries are
+// issued before their terms are added
+template <typename T, typename Acc, int R, typename List>
+__device__ __forceinline__ void rows_sum(List entry_at, int n, const T* sG, const T* rows, Acc (&acc)[R])
+{
+    constexpr int U = R * sizeof(T) <= 32 ? 4 : 2;
+    int p = 0;
+    for (; p + U <= n; p += U) {
+        uint32_t e[U];
+#pragma unroll
+        for (int q = 0; q < U; ++q) e[q] = entry_at(p + q);
+        T gv[U][R], rv[U][R];
+#pragma unroll
+        for (int q = 0; q < U; ++q) {
+            lds_take<T, R>(sG + (int)(e[q] >> 16) * R, gv[q]);
+            lds_take<T, R>(rows + (int)(e[q] & 0xffffu) * R, rv[q]);
+        }
+#pragma unroll
+        for (int q = 0; q < U; ++q)
+#pragma unroll
+            for (int r = 0; r < R; ++r) acc[r] = node_term(acc[r], (Acc)gv[q][r], (Acc)rv[q][r]);
+    }
+    for (; p < n; ++p) {
+        const uint32_t e = entry_at(p);
+        T gv[R], rv[R];
+        lds_take<T, R>(sG + (int)(e >> 16) * R, gv);
+        lds_take<T, R>(rows + (int)(e & 0xffffu) * R, rv);
+#pragma unroll
+        for (int r = 0; r < R; ++r) acc[r] = node_term(acc[r], (Acc)gv[r], (Acc)rv[r]);
+    }
+}
+
+template <typename T, typename Acc, int R>
+__global__ __launch_bounds__(256) void k_bwd_node_rows(const NodeArgs<T> a, int lists_lds)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const bool do_dy = a.part != 2;
     const bool do_gm = a.part != 1 && a.Tm != nullptr;
-    const int wg = a.G ? a.Wo : 0;
+    const int wg = a.Wo;
     const int wm = (do_dy && a.Tm) ? a.Wm : 0;
-    const int yw = a.y_from_h ? a.k : a.Wy;
-    const int wy = do_gm ? yw : 0;
+    const int wy = do_gm ? a.Wy : 0;
     const int la = lists_lds && do_dy && a.Tm ? a.Pa * a.Wy : 0;
     const int lm = lists_lds && do_gm ? a.Pm * a.Wm : 0;
     uint32_t* s_la = reinterpret_cast<uint32_t*>(smem_raw);
     uint32_t* s_lm = s_la + la;
     for (int i = threadIdx.x; i < la; i += blockDim.x) s_la[i] = a.by_auxT[i];
     for (int i = threadIdx.x; i < lm; i += blockDim.x) s_lm[i] = a.by_mainT[i];
-    T* sG = reinterpret_cast<T*>(smem_raw + (((size_t)4 * (la + lm) + 15) & ~(size_t)15));  // [vpb][wg]
-    T* sM = sG + (size_t)vpb * wg;           // [vpb][wm]
-    T* sY = sM + (size_t)vpb * wm;           // [vpb][wy]
+    T* sG = reinterpret_cast<T*>(smem_raw + (((size_t)4 * (la + lm) + 15) & ~(size_t)15));  // [wg][R]
+    T* sM = sG + (size_t)R * wg;            // [wm][R]
+    T* sY = sM + (size_t)R * wm;            // [wy][R]
     const int ny = do_dy ? a.Wy : 0;
     const int per = ny + (do_gm ? a.Wm : 0);
     const int64_t nvb = (int64_t)a.n_act * a.B;
-    const int64_t ngroups = (nvb + vpb - 1) / vpb;
+    const int64_t ngroups = (nvb + R - 1) / R;
     for (int64_t g = blockIdx.x; g < ngroups; g += gridDim.x) {
-        const int64_t vb0 = g * vpb;
-        const int nv = (int)(nvb - vb0 < vpb ? nvb - vb0 : vpb);
+        const int64_t vb0 = g * R;
+        const int nv = (int)(nvb - vb0 < R ? nvb - vb0 : R);
         __syncthreads();
-        for (int i = threadIdx.x; i < nv * wg; i += blockDim.x) sG[i] = a.G[vb0 * wg + i];
-        for (int i = threadIdx.x; i < nv * wm; i += blockDim.x) sM[i] = a.Tm[vb0 * wm + i];
-        for (int i = threadIdx.x; i < nv * wy; i += blockDim.x) sY[i] = a.Y[vb0 * yw + i];
+        stage_interleaved<T, R>(sG, a.G, vb0, nv, wg);
+        if (wm) stage_interleaved<T, R>(sM, a.Tm, vb0, nv, wm);
+        if (wy) stage_interleaved<T, R>(sY, a.Y, vb0, nv, wy);
         __syncthreads();
         for (int x0 = threadIdx.x; x0 < per; x0 += blockDim.x) {
             const bool is_dy = x0 < ny;
             const int x = is_dy ? x0 : x0 - ny;
-            Acc acc[kNodeRows];
+            Acc acc[R];
 #pragma unroll
-            for (int r = 0; r < kNodeRows; ++r) acc[r] = Acc(0);
+            for (int r = 0; r < R; ++r) acc[r] = Acc(0);
             if (is_dy && !a.Tm) {
+                T gv[R];
+                lds_take<T, R>(sG + x * R, gv);
 #pragma unroll
-                for (int r = 0; r < kNodeRows; ++r)
-                    if (r < nv) acc[r] = (Acc)sG[r * wg + x];
+                for (int r = 0; r < R; ++r) acc[r] = (Acc)gv[r];
+            } else if (is_dy) {
+                if (la) rows_sum<T, Acc, R>([&](int p) { return s_la[p * a.Wy + x]; }, a.Pa, sG, sM, acc);
+                else rows_sum<T, Acc, R>([&](int p) { return a.by_auxT[(size_t)p * a.Wy + x]; }, a.Pa, sG, sM, acc);
             } else {
-                const int n = is_dy ? a.Pa : a.Pm;
-                const int stride = is_dy ? a.Wy : a.Wm;
-                const uint32_t* lst = is_dy ? (la ? s_la : a.by_auxT) : (lm ? s_lm : a.by_mainT);
-                const T* rows = is_dy ? sM : sY;
-                const int rw = is_dy ? wm : wy;
-                for (int p = 0; p < n; ++p) {
-                    const uint32_t e = lst[(size_t)p * stride + x];
-                    const T* gp = sG + (int)(e >> 16);
-                    const T* rp = rows + (int)(e & 0xffffu);
-#pragma unroll
-                    for (int r = 0; r < kNodeRows; ++r) {
-                        if (r >= nv) break;
-                        acc[r] = node_term(acc[r], (Acc)gp[r * wg], (Acc)rp[r * rw]);
-                    }
-                }
+                if (lm) rows_sum<T, Acc, R>([&](int p) { return s_lm[p * a.Wm + x]; }, a.Pm, sG, sY, acc);
+                else rows_sum<T, Acc, R>([&](int p) { return a.by_mainT[(size_t)p * a.Wm + x]; }, a.Pm, sG, sY, acc);
             }
 #pragma unroll
-            for (int r = 0; r < kNodeRows; ++r) {
+            for (int r = 0; r < R; ++r) {
                 if (r >= nv) break;
                 const int64_t vb = vb0 + r;
                 if (is_dy) a.dY[vb * a.Wy + x] = (T)(acc[r] * (Acc)a.scale);
@@ -1212,13 +1290,19 @@ hipError_t launch_bwd_node_t(const NodeArgs<T>& a, int num_cus, hipStream_t st)
     // the 12-vertex tree, 176 values of 14-210 terms: 272 ms as against 458
     // ms rows inner; the (10;7,3) step, 627 values: 213 as against 283)
     const int per_row = (do_dy ? a.Wy : 0) + (do_gm ? a.Wm : 0);
-    const bool rows_inner = a.rows_inner && a.G && !a.y_from_h && per_row >= 256;
-    if (rows_inner) vpb = std::min(vpb, kNodeRows);
+    // groups of kNodeRows rows, or 4 where the LDS budget holds fewer
+    const int rows_r = vpb >= kNodeRows ? kNodeRows : vpb >= 4 ? 4 : 0;
+    const bool rows_inner = a.rows_inner && a.G && !a.y_from_h && per_row >= 256 && rows_r > 0;
+    if (rows_inner) vpb = rows_r;
     const int64_t ngroups = ((int64_t)a.n_act * a.B + vpb - 1) / vpb;
     const unsigned grid = (unsigned)std::min<int64_t>(ngroups, (int64_t)std::max(num_cus, 1) * 8);
     if (rows_inner) {
-        hipLaunchKernelGGL((k_bwd_node_rows<T, Acc>), dim3(grid), dim3(256), head + (size_t)vpb * row, st, a, vpb,
-                           lists_lds);
+        if (rows_r == kNodeRows)
+            hipLaunchKernelGGL((k_bwd_node_rows<T, Acc, kNodeRows>), dim3(grid), dim3(256), head + (size_t)vpb * row,
+                               st, a, lists_lds);
+        else
+            hipLaunchKernelGGL((k_bwd_node_rows<T, Acc, 4>), dim3(grid), dim3(256), head + (size_t)vpb * row, st, a,
+                               lists_lds);
         return hipGetLastError();
     }
     hipLaunchKernelGGL((k_bwd_node_lds<T, Acc>), dim3(grid), dim3(256), head + (size_t)vpb * row, st, a, vpb,
