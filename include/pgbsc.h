@@ -102,7 +102,7 @@ const char* pg_last_error(const pg_ctx* ctx);
  *                  values, by colour runs whatever share of a row a vertex uses (default 128)
  *   "node_fp32"    pg_count_grad with FP32 tables: 1 = the adjoint's split sums in FP32
  *                  (fused multiply-adds, default), 0 = in FP64
- *   "node_rows"    pg_count_grad: 1 = staged node launches over rows of 256+ values take a
+ *   "node_rows"    pg_count_grad: 1 = staged node launches over rows of 32+ values take a
  *                  thread per value index over a group's rows (default), 0 = per (row, value)
  *   "runs_g_min"   pg_count_grad: gradient gathers by colour runs from rows of this many values
  *                  (default 128; narrower rows use the per-lane gathers)

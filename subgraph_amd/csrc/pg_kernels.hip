@@ -1032,7 +1032,7 @@ __device__ __forceinline__ void stage_copy(T* dst, const T* src, int n, int nval
 }
 
 template <typename T, typename Acc>
-__global__ __launch_bounds__(256) void k_bwd_node_lds(const NodeArgs<T> a, int vpb, int lists_lds)
+__global__ __launch_bounds__(256) void k_bwd_node_lds(const NodeArgs<T> a, int vpb, int lists_lds, int t_dy)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const bool do_dy = a.part != 2;
@@ -1065,12 +1065,18 @@ __global__ __launch_bounds__(256) void k_bwd_node_lds(const NodeArgs<T> a, int v
         if (wm) stage_copy(sM, a.Tm + vb0 * wm, nv * wm, nv * wm, same);
         if (wy) stage_copy(sY, a.Y + vb0 * yw, nv * wy, nv * wy, same);
         __syncthreads();
-        // items kind-major: every dY value of the group's rows, then every
-        // G_main value, so the lanes of a wavefront sum lists of one length
-        // (Pa or Pm terms; 210 against 14 in the (6;2,4) step of the
-        // 12-vertex tree) instead of waiting on the longest in a mixed row
+        // items by kind: threads [0, t_dy) take the group's dY values and
+        // threads [t_dy, blockDim) its G_main values (t_dy a whole number of
+        // wavefronts), so the lanes of a wavefront sum lists of one length
+        // (Pa or Pm terms; 14 against 210 in the (6;2,4) step of the
+        // 12-vertex tree) and the host sizes each kind's share of threads by
+        // its longest thread's terms
         const int n_dy = nv * ny;
-        for (int it = threadIdx.x; it < nv * per; it += blockDim.x) {
+        const bool take_dy = (int)threadIdx.x < t_dy;
+        const int n_kind = take_dy ? n_dy : nv * (per - ny);
+        const int step = take_dy ? t_dy : (int)blockDim.x - t_dy;
+        for (int jt = take_dy ? (int)threadIdx.x : (int)threadIdx.x - t_dy; jt < n_kind; jt += step) {
+            const int it = take_dy ? jt : n_dy + jt;
             int lv, x0;
             if (it < n_dy) {
                 lv = it / ny;
@@ -1285,14 +1291,15 @@ hipError_t launch_bwd_node_t(const NodeArgs<T>& a, int num_cus, hipStream_t st)
                           lists_head + (size_t)vpb_for(kBwdLdsBudget - kBwdListsTrim) * row <= 64 * 1024;
     const size_t head = lists_lds ? lists_head : 0;
     int vpb = vpb_for(lists_lds ? kBwdLdsBudget - kBwdListsTrim : kBwdLdsBudget);
-    // rows inner only where a row's values fill a workgroup: with fewer, the
-    // per (row, value) threads keep more of them busy (the (6;2,4) step of
-    // the 12-vertex tree, 176 values of 14-210 terms: 272 ms as against 458
-    // ms rows inner; the (10;7,3) step, 627 values: 213 as against 283)
+    // rows inner from 32 values per row.  Measured on the 12-vertex tree at
+    // R-MAT scale 22, forward+backward of 16 colorings: the (10;7,3) step
+    // (627 values) 118 ms rows inner against 316 per (row, value); the
+    // (6;2,4) step's launches (176 values of 14-210 terms, and narrower)
+    // 205 ms with none rows inner, 133 ms from 128 values, 103 ms from 32
     const int per_row = (do_dy ? a.Wy : 0) + (do_gm ? a.Wm : 0);
     // groups of kNodeRows rows, or 4 where the LDS budget holds fewer
     const int rows_r = vpb >= kNodeRows ? kNodeRows : vpb >= 4 ? 4 : 0;
-    const bool rows_inner = a.rows_inner && a.G && !a.y_from_h && per_row >= 256 && rows_r > 0;
+    const bool rows_inner = a.rows_inner && a.G && !a.y_from_h && per_row >= 32 && rows_r > 0;
     if (rows_inner) vpb = rows_r;
     const int64_t ngroups = ((int64_t)a.n_act * a.B + vpb - 1) / vpb;
     const unsigned grid = (unsigned)std::min<int64_t>(ngroups, (int64_t)std::max(num_cus, 1) * 8);
@@ -1305,8 +1312,20 @@ hipError_t launch_bwd_node_t(const NodeArgs<T>& a, int num_cus, hipStream_t st)
                                lists_lds);
         return hipGetLastError();
     }
+    // threads on dY values: the whole-wavefront share whose longer kind's
+    // thread sums the fewest terms (all threads when one kind is absent)
+    const int64_t n_dy = do_dy ? (int64_t)vpb * a.Wy : 0, n_gm = do_gm ? (int64_t)vpb * a.Wm : 0;
+    const int64_t len_dy = a.Tm ? a.Pa : 1, len_gm = a.Pm;
+    int t_dy = n_gm == 0 ? 256 : n_dy == 0 ? 0 : 128;
+    if (n_dy > 0 && n_gm > 0) {
+        int64_t best = -1;
+        for (int t = 64; t < 256; t += 64) {
+            const int64_t worst = std::max((n_dy + t - 1) / t * len_dy, (n_gm + (256 - t) - 1) / (256 - t) * len_gm);
+            if (best < 0 || worst < best) best = worst, t_dy = t;
+        }
+    }
     hipLaunchKernelGGL((k_bwd_node_lds<T, Acc>), dim3(grid), dim3(256), head + (size_t)vpb * row, st, a, vpb,
-                       lists_lds);
+                       lists_lds, t_dy);
     return hipGetLastError();
 }
 

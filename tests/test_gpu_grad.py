@@ -465,8 +465,8 @@ def test_random_tree_grads_fd(gpu_ctx):
 
 def test_node_kernel_variants_agree_bit_for_bit(gpu_ctx):
     """The adjoint's node launches of the 12- and 10-vertex trees' main-table
-    steps run the staged kernel (a thread per (row, value), items kind-major)
-    or, for rows of 256+ values with option node_rows 1, the rows-inner
+    steps run the staged kernel (a thread per (row, value), items by kind)
+    or, for rows of 32+ values with option node_rows 1, the rows-inner
     kernel (a thread per value over a group's rows).  Every value is the same
     sum in the same order either way: counts and gradients equal bit for bit,
     FP32 (FP32 split sums) and FP64.  With 2045 vertices the rows-inner
