@@ -1799,6 +1799,11 @@ int enqueue_sub_backward(pg_ctx* c, const pg_plan* p, const DevSub& d, int B, co
     na.scale = (d.scaled && fp32) ? (double)1.0e-12f : 1.0;
     na.sums32 = fp32 && c->node_fp32 ? 1 : 0;
     na.rows_inner = c->node_rows ? 1 : 0;
+    // A main leaf's node values are dY = G * scale.  Where G is a table of
+    // dY's width and the scale is 1 they equal G bit for bit, so no node
+    // launch copies them and the launches below read G as dY.
+    const bool dy_is_g = na.Tm == nullptr && G != nullptr && d.Wo == d.Wy && na.scale == 1.0;
+    const T* dYin = dy_is_g ? G : dYbuf;
     const std::string node_name = std::string("k_bwd_node<") + (fp32 ? "float>" : "double>");
     // Algorithmic bytes, in the forward's units (full-width FP32 columns of
     // n_act x B values; Y and dY are as wide as the aux child's table; the
@@ -1811,14 +1816,16 @@ int enqueue_sub_backward(pg_ctx* c, const pg_plan* p, const DevSub& d, int B, co
         // dY and G_main from G, the main row and the kept Y
         na.Y = Yk;
         na.part = 0;
-        // reads G, the main row and Y, writes dY and G_main
-        record(c, node_name, col * (Cs + 2.0 * Ca + 2.0 * Cm), true);
-        HIP_TRY(c, launch_bwd_node<T>(na, c->num_cus, st));
-        record(c, "", 0, false);
+        if (!dy_is_g) {
+            // reads G, the main row and Y, writes dY and G_main
+            record(c, node_name, col * (Cs + 2.0 * Ca + 2.0 * Cm), true);
+            HIP_TRY(c, launch_bwd_node<T>(na, c->num_cus, st));
+            record(c, "", 0, false);
+        }
         SubArgs<T> t = base_args;
         t.out_mode = OUT_Y;
         if (c->weighted) t.w = fp32 ? (const T*)c->d_wT32.p : (const T*)c->d_wT64.p;
-        t.aux = dYbuf;
+        t.aux = dYin;
         t.lda = B * d.Wy;
         t.Wa = d.Wy;  // per-coloring stride of the neighbours' dY rows
         t.Wy = d.Wa;  // values per coloring: the aux child's row
@@ -1842,9 +1849,11 @@ int enqueue_sub_backward(pg_ctx* c, const pg_plan* p, const DevSub& d, int B, co
     }
     if (d.aa > 1) {
         na.part = 1;
-        record(c, node_name, col * (Cs + Cm + Ca), true);
-        HIP_TRY(c, launch_bwd_node<T>(na, c->num_cus, st));
-        record(c, "", 0, false);
+        if (!dy_is_g) {
+            record(c, node_name, col * (Cs + Cm + Ca), true);
+            HIP_TRY(c, launch_bwd_node<T>(na, c->num_cus, st));
+            record(c, "", 0, false);
+        }
         SubArgs<T> a = base_args;
         a.out_mode = OUT_Y;
         a.aux = Ta;
@@ -1858,7 +1867,7 @@ int enqueue_sub_backward(pg_ctx* c, const pg_plan* p, const DevSub& d, int B, co
         a.Wy4 = d.Wy4;
         a.out = Ybuf;
         a.ldo = B * d.Wy;
-        a.dy = dYbuf;
+        a.dy = dYin;
         a.ldd = B * d.Wy;
         a.grad = c->d_grad.as<double>();
         // reads T_aux and dY, writes Y
@@ -1875,7 +1884,7 @@ int enqueue_sub_backward(pg_ctx* c, const pg_plan* p, const DevSub& d, int B, co
         SubArgs<T> t = base_args;
         t.out_mode = OUT_Y;
         if (c->weighted) t.w = fp32 ? (const T*)c->d_wT32.p : (const T*)c->d_wT64.p;
-        t.aux = dYbuf;
+        t.aux = dYin;
         t.lda = B * d.Wy;
         t.Wa = d.Wy;  // per-coloring stride of the neighbours' dY rows
         t.Wy = d.Wa;  // values per coloring: the aux child's row
@@ -1894,9 +1903,11 @@ int enqueue_sub_backward(pg_ctx* c, const pg_plan* p, const DevSub& d, int B, co
         // aux child a leaf: Y is the histogram H; the entry's derivative is
         // dY at the neighbour's color.  The node launch reads G and the main
         // row, writes dY, and with a main table also reads H and writes G_main.
-        record(c, node_name, col * (Cs + 2.0 * Cm + (d.mm > 1 ? Ca : 0.0) + Ca), true);
-        HIP_TRY(c, launch_bwd_node<T>(na, c->num_cus, st));
-        record(c, "", 0, false);
+        if (!dy_is_g) {
+            record(c, node_name, col * (Cs + 2.0 * Cm + (d.mm > 1 ? Ca : 0.0) + Ca), true);
+            HIP_TRY(c, launch_bwd_node<T>(na, c->num_cus, st));
+            record(c, "", 0, false);
+        }
         EdgeGradArgs<T> eg{};
         eg.nnz = c->nnz;
         eg.B = B;
@@ -1906,7 +1917,7 @@ int enqueue_sub_backward(pg_ctx* c, const pg_plan* p, const DevSub& d, int B, co
         eg.col = c->d_col.as<int32_t>();
         eg.colors = pb.colors;
         eg.ecol = pb.ecol;
-        eg.dY = dYbuf;
+        eg.dY = dYin;
         eg.Wy = d.Wy;
         eg.Ta = nullptr;
         eg.Wa = d.Wa;

@@ -201,20 +201,28 @@ def test_adjoint_byte_model():
     def gathered(Ca, Cs, Cm, with_main_table):
         """aux child a table: dY, the gradient gather (T_aux, dY in; Y out),
         G_main when the main child is a table, the transposed gather (dY in,
-        G_aux out)"""
-        b = col * (Cs + Cm + Ca) + (csc + 3 * col * Ca + grad_rw) + (csc + 2 * col * Ca)
+        G_aux out).  Below the root (Cs > 0) a main leaf's dY is its G, read
+        in place: no node launch."""
+        node = col * (Cs + Cm + Ca) if (with_main_table or Cs == 0) else 0.0
+        b = node + (csc + 3 * col * Ca + grad_rw) + (csc + 2 * col * Ca)
         return b + (col * (Cs + Ca + Cm) if with_main_table else 0.0)
 
-    def leaf_aux(Cs, Cm):
+    def leaf_aux(Cs, Cm, scaled=False):
         """aux child a leaf: one node launch (G, main row, H in; dY, G_main
-        out), then the per-entry launch (entry rows and columns, neighbour
-        colors, vertex colors, dY, gradient)"""
-        node = col * (Cs + 2 * Cm + (k if Cm else 0) + k)
+        out; none for a main leaf, whose dY is its G, unless the step is the
+        FP32 scaled one, whose dY is G * 1e-12), then the per-entry launch
+        (entry rows and columns, neighbour colors, vertex colors, dY,
+        gradient)"""
+        node = col * (Cs + 2 * Cm + (k if Cm else 0) + k) if (Cm or scaled) else 0.0
         return node + 8 * Z + Z * B + na * B + col * k + grad_rw
+
+    # the scaled step is the first non-leaf the DP processes: one of the two
+    # (2;1,1) steps, whose bytes are the same, so D2 stands for it
+    assert plan.scaled_sub >= 0
 
     bwd = {"R": gathered(Ck[6], 0, 0, False), "A": gathered(Ck[2], Ck[6], Ck[4], True),
            "B4": gathered(Ck[3], Ck[4], 0, False), "C3": leaf_aux(Ck[3], Ck[2]),
-           "D2": leaf_aux(Ck[2], 0), "E2": leaf_aux(Ck[2], 0)}
+           "D2": leaf_aux(Ck[2], 0, scaled=True), "E2": leaf_aux(Ck[2], 0)}
     subtree = {"A": ["A", "B4", "C3", "D2", "E2"], "B4": ["B4", "C3", "E2"], "C3": ["C3", "E2"],
                "D2": ["D2"], "E2": ["E2"]}
     rebuilt = {"R": ["A"], "A": ["B4", "D2"], "B4": ["C3"], "C3": ["E2"]}  # tables each backward step rebuilds
