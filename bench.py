@@ -613,9 +613,9 @@ def main():
     ap.add_argument("--plain-settings",
                     default="stage_rows=1,bucket_high=512,bucket_mid=64,hub_degree=512,hub_chunk=1024,exact_order=0",
                     metavar="KEY=VAL,...", help="context options of the plain_s20 line")
-    ap.add_argument("--only", default="", choices=("", "headline", "plain"),
+    ap.add_argument("--only", default="", choices=("", "headline", "plain", "grad"),
                     help="run just that measurement (a short run for profilers); headline and plain skip their "
-                         "comparisons")
+                         "comparisons; grad is the s18 forward+backward workload of the line's fwd_bwd entry")
     ap.add_argument("--eager", action="store_true",
                     help="with --only headline: run the steps eagerly with a HIP event pair around every launch "
                          "(one stream), as the roofline figure is measured, for a kernel trace that matches it")
@@ -659,6 +659,9 @@ def main():
     if args.only == "plain":
         args.no_cpu = True
         print(json.dumps({"plain_s20": plain_s20(args, sync)}), flush=True)
+        return
+    if args.only == "grad":
+        print(json.dumps({"fwd_bwd": fwd_bwd_variant(sync)}), flush=True)
         return
     if args.only == "headline":
         args.no_cpu = True
